@@ -1,0 +1,242 @@
+/*
+ * soapfilm.h -- C ABI of the MI355X (gfx950) soap-film physics library.
+ *
+ * This is the drop-in boundary for the data-parallel physics path of
+ * pjt222/soap_bubble_simulation: the film-thickness drainage step on the
+ * bubble's latitude-longitude grid and the thin-film interference evaluation
+ * and LUT generation.  The reference has no FFI; each entry point below names
+ * the Rust API it replaces (file:line in the reference crate).  All compute
+ * runs in hand-written HIP kernels; the library fails with SF_ERR_DEVICE when
+ * no GPU is present (there is no CPU fallback).
+ *
+ * Conventions
+ *   - Every function returns SF_OK (0) or a negative SF_ERR_* code;
+ *     sf_last_error() gives a message for the calling thread.
+ *   - Grids are row-major [theta][phi]: index = i * n_phi + j,
+ *     theta_i = i * pi / (n_theta - 1) (both poles included),
+ *     phi_j = j * 2 pi / n_phi (drainage.rs:121-157).
+ *   - Thickness is in metres (drainage.rs:103-105); LUT thickness in nm.
+ *   - One sf_drainage context belongs to one host thread.  It owns its
+ *     device buffers and one HIP stream; sf_drainage_step only enqueues
+ *     work, every copy/query/stats call synchronises that stream.
+ */
+#ifndef SOAPFILM_H
+#define SOAPFILM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define SF_OK             0
+#define SF_ERR_INVALID   -1   /* bad argument                                */
+#define SF_ERR_DEVICE    -2   /* HIP runtime error or no GPU present         */
+#define SF_ERR_NOMEM     -3   /* device allocation failed                    */
+#define SF_ERR_INTERNAL  -4
+
+/* ---- drainage flags ------------------------------------------------------ */
+/* Marangoni term on h and thickness coupling on Γ (drainage.wgsl:150-171, 193-198). */
+#define SF_FLAG_MARANGONI        (1u << 0)
+/* Evolve the surfactant field Γ (drainage.wgsl:178-201). */
+#define SF_FLAG_SURFACTANT       (1u << 1)
+/* Marginal-regeneration fronts (drainage.rs:387-389, 395-477). */
+#define SF_FLAG_TFE              (1u << 2)
+/* h poles copied through (drainage.wgsl:85-89) instead of ring mean (drainage.rs:480-503). */
+#define SF_FLAG_POLES_COPY_H     (1u << 3)
+/* Γ poles set to the ring mean instead of copied through. */
+#define SF_FLAG_POLES_AVG_GAMMA  (1u << 4)
+/* Per-step FP64 film-volume (mass) and CFL reductions fused into the step. */
+#define SF_FLAG_DIAG             (1u << 5)
+
+/*
+ * Physical and numerical parameters of one bubble.  Union of the GPU uniform
+ * DrainageParams (gpu_drainage.rs:12-44) and the DrainageSimulator fields
+ * (drainage.rs:212-256), in FP64; the kernels round them to FP32 where the
+ * stencil computes in FP32.
+ */
+typedef struct sf_params {
+    double dt;                    /* default step, s (gpu_drainage.rs:14)            */
+    double gravity;               /* m/s^2                                           */
+    double viscosity;             /* Pa s                                            */
+    double density;               /* kg/m^3                                          */
+    double diffusion_coeff;       /* D_h, m^2/s (drainage.rs:277)                    */
+    double bubble_radius;         /* m                                               */
+    double critical_thickness;    /* m; cells below it are frozen (drainage.rs:331)  */
+    double gamma_air;             /* N/m (drainage.wgsl:25)                          */
+    double gamma_reduction;       /* N/m per unit Γ (drainage.wgsl:26)               */
+    double surfactant_diffusion;  /* D_s, m^2/s (drainage.wgsl:27)                   */
+    double marangoni_coeff;       /* drainage.wgsl:28                                */
+    double sin_guard;             /* |sin θ| floor: 1e-10 CPU, 1e-4 WGSL             */
+    double tfe_thickness_ratio;   /* drainage.rs:283, clamped to [0.5, 0.95]         */
+    double tfe_first_spawn;       /* s until the first front (drainage.rs:285)       */
+    double tfe_spawn_interval;    /* s (drainage.rs:286)                             */
+    double cfl_safety;            /* dt = cfl_safety * dt_cfl when step dt <= 0      */
+    uint32_t flags;               /* SF_FLAG_*                                       */
+    uint32_t rebase_interval;     /* steps between float-float renormalisations      */
+} sf_params;
+
+typedef struct sf_stats {
+    double min_thickness;         /* over all cells incl. poles (drainage.rs:180-186) */
+    double max_thickness;         /* drainage.rs:188-193                              */
+    double mass;                  /* FP64 film volume  Σ h_ij A_i, m^3                */
+    double dt_cfl;                /* FP64 forward-Euler stability limit, s            */
+    double current_time;          /* s (drainage.rs:560)                              */
+    int64_t n_ruptured;           /* cells with h < critical_thickness                */
+    int32_t n_tfe;                /* active fronts (drainage.rs:611)                  */
+    int32_t has_critical_region;  /* drainage.rs:565-567                              */
+    int64_t steps;                /* steps taken since create/reset                   */
+} sf_stats;
+
+typedef struct sf_drainage sf_drainage;
+
+/* Message for the last failing call on this thread ("" when none). */
+const char* sf_last_error(void);
+/* Library version string. */
+const char* sf_version(void);
+/* 1 when a HIP device is usable, 0 otherwise (never fails). */
+int sf_device_available(void);
+
+/* Defaults of DrainageSimulator::new(&SimulationConfig::default())
+ * (drainage.rs:263-288, config.rs:305-316): 10 nm rupture, TFE on, h poles
+ * averaged, Γ frozen, sin guard 1e-10. */
+int sf_params_cpu_default(sf_params* out);
+/* Defaults of DrainageParams::default() (gpu_drainage.rs:46-68): dt 1e-3,
+ * 30 nm rupture, Γ evolving, poles copied through, sin guard 1e-4, no TFE. */
+int sf_params_gpu_default(sf_params* out);
+
+/* DrainageSimulator::new (drainage.rs:263-288) / GPUDrainageSimulator::new
+ * (gpu_drainage.rs:118-331): uniform h = h0 (m) and Γ = gamma0.
+ * Needs n_theta >= 3 and n_phi >= 1. */
+int sf_drainage_create(const sf_params* params, int32_t n_theta, int32_t n_phi, double h0,
+                       double gamma0, sf_drainage** out);
+void sf_drainage_destroy(sf_drainage* ctx);
+
+/* GPUDrainageSimulator::update_params / set_dt / set_physics /
+ * set_bubble_radius / set_marangoni_enabled / set_marangoni_params
+ * (gpu_drainage.rs:377-456). */
+int sf_drainage_set_params(sf_drainage* ctx, const sf_params* params);
+int sf_drainage_get_params(const sf_drainage* ctx, sf_params* out);
+
+/* Upload h and Γ from host arrays of n_theta*n_phi values (either may be
+ * NULL to keep the field).  The FP64 variant keeps h to ~48 bits. */
+int sf_drainage_set_fields(sf_drainage* ctx, const float* h, const float* gamma);
+int sf_drainage_set_fields_f64(sf_drainage* ctx, const double* h, const double* gamma);
+
+/* DrainageSimulator::step (drainage.rs:298-392) repeated n_steps times, or
+ * GPUDrainageSimulator::step's dispatch loop (gpu_drainage.rs:354-369).
+ * dt > 0: fixed step.  dt <= 0: dt = cfl_safety * FP64 CFL estimate of the
+ * state at the start of the call, held for the call's n_steps.  Asynchronous. */
+int sf_drainage_step(sf_drainage* ctx, int32_t n_steps, double dt);
+
+/* Download the current fields (either pointer may be NULL).  The FP32 form
+ * rounds h to float; the FP64 form returns hi + lo. */
+int sf_drainage_copy_field(sf_drainage* ctx, float* h, float* gamma);
+int sf_drainage_copy_field_f64(sf_drainage* ctx, double* h, double* gamma);
+
+/* DrainageSimulator::get_thickness (drainage.rs:516-552): bilinear query at
+ * (theta, phi) radians; index and weight math in FP64 on the host, identical
+ * to the reference; the four corner values are gathered on the device. */
+int sf_drainage_get_thickness(sf_drainage* ctx, double theta, double phi, double* out);
+/* Batched form of the same query (n points). */
+int sf_drainage_get_thickness_batch(sf_drainage* ctx, int32_t n, const double* theta,
+                                    const double* phi, double* out);
+
+/* min/max (drainage.rs:180-193), FP64 mass and CFL, rupture count, time. */
+int sf_drainage_stats(sf_drainage* ctx, sf_stats* out);
+
+/* DrainageSimulator::reset (drainage.rs:575-583) / GPUDrainageSimulator::reset
+ * (gpu_drainage.rs:415-432): uniform h0 and gamma0, time 0, fronts cleared. */
+int sf_drainage_reset(sf_drainage* ctx, double h0, double gamma0);
+
+/* set_marginal_regeneration (drainage.rs:588-593), set_tfe_thickness_ratio
+ * (601-603), active_tfe_count (611-613), current_time (560-562). */
+int sf_drainage_set_marginal_regeneration(sf_drainage* ctx, int32_t enabled);
+int sf_drainage_set_tfe_thickness_ratio(sf_drainage* ctx, double ratio);
+int sf_drainage_active_tfe_count(const sf_drainage* ctx, int32_t* out);
+int sf_drainage_current_time(const sf_drainage* ctx, double* out);
+
+/* Per-step FP64 mass and CFL values recorded while SF_FLAG_DIAG is set: the
+ * most recent min(n, recorded) entries, oldest first; *n_out = count. */
+int sf_drainage_history(sf_drainage* ctx, int32_t n, double* mass, double* dt_cfl,
+                        int32_t* n_out);
+
+/* TFE membership of the fronts active after the last step: mask[i*n_phi+j] = 1
+ * when a front contains cell (i, j) (TfeFront::contains, drainage.rs:55-71). */
+int sf_drainage_tfe_mask(sf_drainage* ctx, uint8_t* mask);
+/* Rupture mask of the current state: h < critical_thickness (drainage.rs:331). */
+int sf_drainage_rupture_mask(sf_drainage* ctx, uint8_t* mask);
+
+/* Device pointers of the current FP32 fields for GPU consumers (the
+ * reference hands its thickness buffer to caustics and branched flow,
+ * pipeline.rs:685-694): h = hi + lo, Γ.  Valid until the next call on ctx.
+ * The stream is the context's hipStream_t. */
+int sf_drainage_device_fields(sf_drainage* ctx, const float** h_hi, const float** h_lo,
+                              const float** gamma);
+int sf_drainage_stream(sf_drainage* ctx, void** stream);
+/* Average duration in ms of the drainage kernel launches recorded with HIP
+ * events on the context's stream since the last call (0 when none). */
+int sf_drainage_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
+                              int64_t* launches);
+
+/* ---- interference --------------------------------------------------------- */
+
+/* generate_interference_lut (interference_lut.rs:18-41): 64 angles x 256
+ * thicknesses, angle-major RGBA8, 65536 bytes. */
+int sf_lut_rgba8(float refractive_index, float intensity, uint8_t* out);
+/* The same LUT before quantisation: [64][256][3] floats. */
+int sf_lut_rgb_f32(float refractive_index, float intensity, float* out);
+
+/* InterferenceCalculator::generate_2d_color_lookup_table (interference.rs:491-512)
+ * with the calculator's three wavelengths: out[t][a][3], cos from 0 to 1;
+ * gamma != 0 applies the 1/2.2 power of calculate_interference_color
+ * (interference.rs:380-400), 0 gives calculate_interference_color_linear. */
+int sf_lut_2d(double refractive_index, const double wavelengths_nm[3], double tmin_nm,
+              double tmax_nm, int32_t n_thickness, int32_t n_angle, int32_t gamma, float* out);
+/* InterferenceCalculator::generate_color_lookup_table (interference.rs:461-477):
+ * out[n][3] at one cos(theta). */
+int sf_lut_1d(double refractive_index, const double wavelengths_nm[3], double tmin_nm,
+              double tmax_nm, int32_t n, double cos_theta, float* out);
+/* calculate_reflected_intensity (interference.rs:312-353) at n points:
+ * out[k] = I(d_nm[k], cos_i[k], lambda_nm[k]). */
+int sf_reflected_intensity(double refractive_index, int32_t n, const float* d_nm,
+                           const float* cos_i, const float* lambda_nm, float* out);
+/* calculate_interference_color / _linear (interference.rs:380-426) at n
+ * points: out[k][3], gamma != 0 applies the 1/2.2 power. */
+int sf_interference_color(double refractive_index, const double wavelengths_nm[3], int32_t n,
+                          const float* d_nm, const float* cos_i, int32_t gamma, float* out);
+/* The intermediate terms of interference.rs:197-298 at n points, out[k][6] =
+ * {cos_t (197-211), optical path difference (226-238), phase (248-254),
+ *  R_s, R_p, R_avg (266-298)}. */
+int sf_interference_terms(double refractive_index, int32_t n, const float* d_nm,
+                          const float* cos_i, const float* lambda_nm, float* out);
+
+/* Camera of the spectral evaluation: world position, bubble centred at the
+ * origin, y up (theta = 0 is the top). */
+typedef struct sf_camera {
+    double position[3];
+} sf_camera;
+
+/* Per-cell spectral interference: 81 wavelengths (380-780 nm, 5 nm step) of
+ * the Airy/Fresnel model of interference_lut.rs:54-72 at each cell's view
+ * angle |dot(normalize(cam - p), n)| (bubble.wgsl:491-494), contracted with
+ * cmf[81][3] on BF16 MFMA with FP32 accumulation:
+ *   rgb[c][cell] = (1/81) sum_k I_k cmf[k][c].
+ * Reads the context's current h; writes 3 planes of n_theta*n_phi floats
+ * to host memory (rgb_host) and/or keeps them on the device. */
+int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_index,
+                    const float* cmf, float* rgb_host);
+/* Device pointer of the last spectral RGB planes of ctx (3*n_theta*n_phi). */
+int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb);
+/* Average duration in ms of the spectral kernel launches since the last call
+ * (HIP events on the context's stream); enable switches recording on/off. */
+int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
+                              int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOAPFILM_H */

@@ -1,0 +1,117 @@
+// drainage.hpp -- the sf_drainage context (one bubble on one GPU).
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "drainage_kernels.hpp"
+
+namespace sf {
+
+struct StatsPart;
+
+// Host copy of one marginal-regeneration front (TfeFront, drainage.rs:23-33)
+// plus its precomputed column run.
+struct HostFront {
+    double phi_center, extent, width, velocity;
+    int j0, len;
+};
+
+// HIP-event timing of one kernel's launches on a stream: an event pair
+// brackets every launch, and drain() sums the elapsed times.
+struct KernelTimer {
+    static constexpr size_t kPool = 4096;
+    bool enabled = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    size_t used = 0;
+    double ms = 0.0;
+    int64_t launches = 0;
+
+    std::pair<hipEvent_t, hipEvent_t>& next(hipStream_t s);
+    void drain(hipStream_t s);
+    void destroy();
+};
+
+struct Drainage;
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf);
+
+struct Drainage {
+    static constexpr int64_t kHistCap = 1 << 16;
+    static constexpr size_t kStatsBlocks = 1024;
+
+    sf_params p;
+    int nt, np;
+    size_t n;
+    double dtheta, dphi;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // launch geometry
+    int wx = 1, block = 64, ntiles = 1, rows_per_block = 16;
+
+    // device state (see drainage_kernels.hpp for the layout)
+    float* hi[2] = {nullptr, nullptr};
+    float* lo[2] = {nullptr, nullptr};
+    float* g[2] = {nullptr, nullptr};
+    int cur_hi = 0, cur = 0, cur_g = 0;
+    int since_rebase = 0;
+    float2* hpole[2] = {nullptr, nullptr};
+    float2* gpole[2] = {nullptr, nullptr};
+    int cur_pole = 0;
+    bool poles_virtual = false;   // pole rows are held as ring partials
+    RowF* rows = nullptr;
+    RowD* rowd = nullptr;
+    bool tables_dirty = true;
+    double pole_area_top = 0.0, pole_area_bot = 0.0, pole_mass_fixed = 0.0;
+
+    // FP64 diagnostics
+    size_t nblocks_max = 0;
+    double2* diag_part = nullptr;
+    unsigned int* ticket = nullptr;
+    double2* diag_hist = nullptr;
+    int64_t hist_count = 0;
+    StatsPart* stats_part = nullptr;
+    StatsPart* stats_out = nullptr;
+
+    // queries
+    int64_t* gather_idx = nullptr;
+    float2* gather_out = nullptr;
+    size_t gather_cap = 0;
+
+    // spectral interference
+    float* rgb = nullptr;
+    float* cmf_dev = nullptr;
+    float4* spec_rows = nullptr;
+    float2* spec_cols = nullptr;
+
+    // time and TFE schedule (drainage.rs:237-255)
+    double current_time = 0.0;
+    int64_t steps_taken = 0;
+    bool tfe_enabled = false;
+    double next_spawn = 0.5;
+    std::vector<HostFront> fronts;
+    std::vector<FrontBox> last_boxes;
+
+    // kernel timing with HIP events on `stream`
+    KernelTimer step_timer, spec_timer;
+
+    Drainage(const sf_params& prm, int n_theta, int n_phi);
+    ~Drainage();
+    Drainage(const Drainage&) = delete;
+    Drainage& operator=(const Drainage&) = delete;
+
+    void upload_tables();
+    void update_fixed_pole_mass();
+    void upload(const float* h_hi, const float* h_lo, const float* gam);
+    void fill(double h0, double gamma0);
+    void front_columns(HostFront& f) const;
+    void spawn_front();
+    void schedule_fronts(double dt, StepArgs& a);
+    void fill_common_args(StepArgs& a, float dt_f) const;
+    void materialize();
+    void step(int n_steps, double dt);
+    StatsPart compute_stats();
+    void ensure_gather(size_t count);
+};
+
+}  // namespace sf

@@ -1,0 +1,625 @@
+// interference.hip -- thin-film interference kernels (gfx950).
+//
+//   lut_rgb_kernel      runtime LUT of interference_lut.rs:18-163 (FP32,
+//                       7 wavelengths, CIE fits, RGBA8)
+//   calc_kernel         InterferenceCalculator of interference.rs:197-512
+//                       in FP32 (3 wavelengths, gamma 1/2.2): 1-D/2-D LUTs,
+//                       per-point colour, intensity and intermediate terms
+//   spectral_kernel     81-wavelength Airy/Fresnel reflectance at every grid
+//                       cell's view angle, contracted with cmf[81][3] on BF16
+//                       MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "drainage.hpp"
+
+namespace sf {
+
+constexpr int kLutT = 256;          // interference_lut.rs:9
+constexpr int kLutA = 64;           // interference_lut.rs:10
+constexpr float kLutMaxNm = 2000.f; // interference_lut.rs:13
+constexpr float kPiF = 3.14159265358979323846f;
+
+// ---------------------------------------------------------------------------
+// Runtime RGBA8 LUT.  Operation order follows interference_lut.rs with FMA
+// contraction disabled, so that values land within an ulp or two of the
+// CPU build (transcendentals come from the device math library).
+
+__device__ inline float max_f(float a, float b) { return fmaxf(a, b); }
+
+__device__ inline float lut_gauss(float x, float mean, float sigma) {
+#pragma clang fp contract(off)
+    const float t = (x - mean) / sigma;
+    return expf(-0.5f * t * t);
+}
+
+__device__ inline void lut_thin_film(float t_nm, float cos_theta, float n_film, float intensity,
+                                     float rgb[3]) {
+#pragma clang fp contract(off)
+    const float W[7] = {400.f, 450.f, 500.f, 550.f, 600.f, 650.f, 700.f};
+    // snells_law (96-100)
+    const float sin_i = sqrtf(max_f(1.f - cos_theta * cos_theta, 0.f));
+    const float sin_t = sin_i / n_film;
+    const float cos_t = sqrtf(max_f(1.f - sin_t * sin_t, 0.f));
+    // fresnel_unpolarized (103-120), n1 = 1
+    const float n1_cos_i = 1.f * cos_theta;
+    const float n2_cos_t = n_film * cos_t;
+    const float r_s = (n1_cos_i - n2_cos_t) / max_f(n1_cos_i + n2_cos_t, 0.0001f);
+    const float n2_cos_i = n_film * cos_theta;
+    const float n1_cos_t = 1.f * cos_t;
+    const float r_p = (n2_cos_i - n1_cos_t) / max_f(n2_cos_i + n1_cos_t, 0.0001f);
+    const float fres = (r_s * r_s + r_p * r_p) * 0.5f;
+    const float optical_path = 2.f * n_film * t_nm * cos_t;
+    float x = 0.f, y = 0.f, z = 0.f;
+    for (int k = 0; k < 7; ++k) {
+        const float w = W[k];
+        const float phase = 2.f * kPiF * optical_path / w + kPiF;
+        // airy_interference (123-133)
+        const float omr = max_f(1.f - fres, 0.001f);
+        const float f = 4.f * fres / (omr * omr);
+        const float s = sinf(phase * 0.5f);
+        const float s2 = s * s;
+        const float a = (f * s2) / max_f(1.f + f * s2, 0.001f);
+        // cie_color_matching (136-148)
+        const float cx = max_f(1.056f * lut_gauss(w, 599.8f, 37.9f) + 0.362f * lut_gauss(w, 442.f, 16.f) -
+                                   0.065f * lut_gauss(w, 501.1f, 20.4f), 0.f);
+        const float cy = max_f(0.821f * lut_gauss(w, 568.8f, 46.9f) + 0.286f * lut_gauss(w, 530.9f, 31.1f), 0.f);
+        const float cz = max_f(1.217f * lut_gauss(w, 437.f, 11.8f) + 0.681f * lut_gauss(w, 459.f, 26.f), 0.f);
+        x += cx * a;
+        y += cy * a;
+        z += cz * a;
+    }
+    x /= 7.f;
+    y /= 7.f;
+    z /= 7.f;
+    // xyz_to_rgb (158-163)
+    rgb[0] = (3.2404542f * x - 1.5371385f * y - 0.4985314f * z) * intensity;
+    rgb[1] = (-0.969266f * x + 1.8760108f * y + 0.0415560f * z) * intensity;
+    rgb[2] = (0.0556434f * x - 0.2040259f * y + 1.0572252f * z) * intensity;
+}
+
+// `(v.clamp(0, 1) * 255.0) as u8` (interference_lut.rs:33-35): NaN -> 0.
+__device__ inline uint8_t quant_u8(float v) {
+#pragma clang fp contract(off)
+    if (v != v) return 0;
+    const float c = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+    return static_cast<uint8_t>(c * 255.f);
+}
+
+__global__ void lut_rgb_kernel(float n_film, float intensity, float* rgb_out, uint8_t* rgba_out) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // thickness index
+    const int a = blockIdx.y;                               // angle index
+    if (t >= kLutT || a >= kLutA) return;
+    const float t_nm = (static_cast<float>(t) / static_cast<float>(kLutT - 1)) * kLutMaxNm;
+    const float cos_theta = static_cast<float>(a) / static_cast<float>(kLutA - 1);
+    float rgb[3];
+    lut_thin_film(t_nm, cos_theta, n_film, intensity, rgb);
+    const size_t c = static_cast<size_t>(a) * kLutT + t;
+    if (rgb_out) {
+        rgb_out[3 * c + 0] = rgb[0];
+        rgb_out[3 * c + 1] = rgb[1];
+        rgb_out[3 * c + 2] = rgb[2];
+    }
+    if (rgba_out) {
+        uchar4 px;
+        px.x = quant_u8(rgb[0]);
+        px.y = quant_u8(rgb[1]);
+        px.z = quant_u8(rgb[2]);
+        px.w = 255;
+        reinterpret_cast<uchar4*>(rgba_out)[c] = px;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// InterferenceCalculator in FP32.
+
+struct CalcTerms {
+    float cos_t, opd, phase, rs, rp, ravg;
+};
+
+// calculate_transmission_angle_cos / optical path / phase / Fresnel
+// (interference.rs:197-298), n_medium = 1 (REFRACTIVE_INDEX_AIR).
+__device__ inline CalcTerms calc_terms(float n_film, float d_nm, float cos_i, float lambda) {
+    CalcTerms r;
+    const float c = fminf(fmaxf(cos_i, -1.f), 1.f);
+    const float sin_i = sqrtf(1.f - c * c);
+    const float sin_t = (1.f / n_film) * sin_i;
+    r.cos_t = fabsf(sin_t) > 1.f ? 0.f : sqrtf(1.f - sin_t * sin_t);
+    r.opd = 2.f * n_film * d_nm * r.cos_t + lambda / 2.f;
+    r.phase = 2.f * kPiF * r.opd / lambda;
+    const float n1_cos_i = c, n2_cos_t = n_film * r.cos_t;
+    const float n2_cos_i = n_film * c, n1_cos_t = r.cos_t;
+    const float as = (n1_cos_i - n2_cos_t) / (n1_cos_i + n2_cos_t);
+    const float ap = (n2_cos_i - n1_cos_t) / (n2_cos_i + n1_cos_t);
+    r.rs = as * as;
+    r.rp = ap * ap;
+    r.ravg = (r.rs + r.rp) / 2.f;
+    return r;
+}
+
+// calculate_reflected_intensity (interference.rs:312-353).  The phase is
+// reduced to one period as OPD / lambda - round(OPD / lambda) before the
+// cosine, which keeps FP32 accurate for thick films.
+__device__ inline float calc_intensity(float n_film, float d_nm, float cos_i, float lambda) {
+    if (d_nm <= 0.f) return 0.f;
+    const CalcTerms t = calc_terms(n_film, d_nm, cos_i, lambda);
+    const float cyc = t.opd / lambda;
+    const float red = cyc - rintf(cyc);
+    const float cos_phase = cospif(2.f * red);
+    const float R = t.ravg;
+    const float num = 2.f * R * (1.f - cos_phase);
+    const float den = 1.f + R * R - 2.f * R * cos_phase;
+    if (fabsf(den) < 2.220446049250313e-16f) return R;
+    return fminf(fmaxf(num / den, 0.f), 1.f);
+}
+
+__device__ inline float calc_channel(float n_film, float d_nm, float cos_i, float lambda, int gamma) {
+    float v = calc_intensity(n_film, d_nm, cos_i, lambda);
+    if (gamma) v = powf(v, 1.f / 2.2f);
+    return fminf(fmaxf(v, 0.f), 1.f);      // RgbColor::new (interference.rs:54-60)
+}
+
+// mode 0: 2-D LUT [t][a][3]; mode 1: 1-D LUT [t][3] at fixed cos; mode 2:
+// per-point colour; mode 3: per-point intensity; mode 4: per-point terms.
+// LUT thickness and cos values arrive as arrays formed on the host in FP64
+// exactly as interference.rs:469-470 and 497-507 form them, then rounded to
+// FP32, so that the LUT addressing matches the reference bit for bit.
+struct CalcArgs {
+    int mode, nt, na, gamma, count;
+    float n_film, l0, l1, l2;
+    const float* d_nm;
+    const float* cos_i;
+    const float* lambda;
+    float* out;
+};
+
+__global__ void calc_kernel(CalcArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float lam[3] = {a.l0, a.l1, a.l2};
+    if (a.mode == 0) {
+        if (k >= a.nt * a.na) return;
+        const int ti = k / a.na, ai = k % a.na;
+        const float d = a.d_nm[ti];
+        const float c = a.cos_i[ai];
+        for (int ch = 0; ch < 3; ++ch)
+            a.out[3 * static_cast<size_t>(k) + ch] = calc_channel(a.n_film, d, c, lam[ch], a.gamma);
+    } else if (a.mode == 1) {
+        if (k >= a.nt) return;
+        const float d = a.d_nm[k];
+        const float c = a.cos_i[0];
+        for (int ch = 0; ch < 3; ++ch)
+            a.out[3 * static_cast<size_t>(k) + ch] = calc_channel(a.n_film, d, c, lam[ch], 1);
+    } else if (a.mode == 2) {
+        if (k >= a.count) return;
+        for (int ch = 0; ch < 3; ++ch)
+            a.out[3 * static_cast<size_t>(k) + ch] =
+                calc_channel(a.n_film, a.d_nm[k], a.cos_i[k], lam[ch], a.gamma);
+    } else if (a.mode == 3) {
+        if (k >= a.count) return;
+        a.out[k] = calc_intensity(a.n_film, a.d_nm[k], a.cos_i[k], a.lambda[k]);
+    } else {
+        if (k >= a.count) return;
+        const CalcTerms t = calc_terms(a.n_film, a.d_nm[k], a.cos_i[k], a.lambda[k]);
+        float* o = a.out + 6 * static_cast<size_t>(k);
+        o[0] = t.cos_t; o[1] = t.opd; o[2] = t.phase; o[3] = t.rs; o[4] = t.rp; o[5] = t.ravg;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral 81-wavelength interference with the BF16 MFMA contraction.
+//
+// One wave handles 16 consecutive cells per iteration.  For the
+// 16x16x16 BF16 MFMA, lane l holds A[m][k] for m = l % 16 and
+// k = 4 (l / 16) + v (v = 0..3), B[k][n] for n = l % 16, and receives
+// D[4 (l / 16) + v][n].  A is the Airy reflectance of cell m at wavelength
+// 16 t + k (t = 0..5 chunks, 96 slots for 81 wavelengths), B the colour
+// curves, so each lane evaluates 21 reflectances per 16 cells with no
+// redundancy beyond the padding.  A and B are split into BF16 hi + lo parts
+// and three MFMAs (hi*hi + hi*lo + lo*hi) keep ~16 mantissa bits.
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kSpecLambdas = 81;
+constexpr int kSpecChunks = 6;
+
+struct SpecArgs {
+    const float* hi;
+    const float* lo;
+    const float4* rows;     // per theta row: sin, cos
+    const float2* cols;     // per phi column: sin, cos
+    const uint2* bfrag;     // [2 (hi, lo)][6 chunks][64 lanes] packed bf16x4
+    float* rgb;             // 3 planes of n cells
+    int np;
+    size_t n;
+    float cx, cy, cz, cam2;  // camera and |cam|^2
+    float radius, n_film, inv_n_film;
+};
+
+__device__ inline uint32_t bf16_hi_bits(float x) { return __float_as_uint(x) & 0xFFFF0000u; }
+
+// Truncating split x = hi + lo (+ O(2^-16 x)), packed as bf16 pairs.
+__device__ inline void split_pack(float a, float b, uint32_t& hi2, uint32_t& lo2) {
+    const uint32_t ah = bf16_hi_bits(a), bh = bf16_hi_bits(b);
+    const float al = a - __uint_as_float(ah);
+    const float bl = b - __uint_as_float(bh);
+    hi2 = (ah >> 16) | bh;
+    lo2 = (bf16_hi_bits(al) >> 16) | bf16_hi_bits(bl);
+}
+
+__global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15;
+    const int q = lane >> 4;
+    const size_t wave_global = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const size_t nwaves = (static_cast<size_t>(gridDim.x) * blockDim.x) >> 6;
+
+    // B fragments and 1 / (2 lambda) of this lane's wavelength slots.
+    v4s bh[kSpecChunks], bl[kSpecChunks];
+    float u[kSpecChunks][4];
+    for (int t = 0; t < kSpecChunks; ++t) {
+        const uint2 h2 = a.bfrag[t * 64 + lane];
+        const uint2 l2 = a.bfrag[kSpecChunks * 64 + t * 64 + lane];
+        bh[t] = v4s{static_cast<short>(h2.x & 0xFFFF), static_cast<short>(h2.x >> 16),
+                    static_cast<short>(h2.y & 0xFFFF), static_cast<short>(h2.y >> 16)};
+        bl[t] = v4s{static_cast<short>(l2.x & 0xFFFF), static_cast<short>(l2.x >> 16),
+                    static_cast<short>(l2.y & 0xFFFF), static_cast<short>(l2.y >> 16)};
+        for (int v = 0; v < 4; ++v) {
+            const int k = 16 * t + 4 * q + v;
+            u[t][v] = k < kSpecLambdas ? 1.f / (2.f * (380.f + 5.f * static_cast<float>(k))) : 0.f;
+        }
+    }
+    const float scale = 1.f / static_cast<float>(kSpecLambdas);
+
+    for (size_t grp = wave_global; grp * 16 < a.n; grp += nwaves) {
+        const size_t cell = grp * 16 + m;
+        const bool valid = cell < a.n;
+        const size_t cc = valid ? cell : a.n - 1;
+        const int i = static_cast<int>(cc / a.np);
+        const int j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
+        const float4 rw = a.rows[i];
+        const float2 cl = a.cols[j];
+        // View angle |dot(normalize(cam - p), n)|, p = R n (bubble.wgsl:491-494).
+        const float nx = rw.x * cl.y, ny = rw.y, nz = rw.x * cl.x;
+        const float cn = a.cx * nx + a.cy * ny + a.cz * nz;
+        const float len2 = a.cam2 - 2.f * a.radius * cn + a.radius * a.radius;
+        const float cos_i = fminf(fabsf(cn - a.radius) * rsqrtf(len2), 1.f);
+        // Snell and unpolarised Fresnel (interference_lut.rs:96-120).
+        const float sin_i = sqrtf(fmaxf(1.f - cos_i * cos_i, 0.f));
+        const float sin_t = sin_i * a.inv_n_film;
+        const float cos_t = sqrtf(fmaxf(1.f - sin_t * sin_t, 0.f));
+        const float rs = (cos_i - a.n_film * cos_t) / fmaxf(cos_i + a.n_film * cos_t, 1e-4f);
+        const float rp = (a.n_film * cos_i - cos_t) / fmaxf(a.n_film * cos_i + cos_t, 1e-4f);
+        const float R = (rs * rs + rp * rp) * 0.5f;
+        const float omr = fmaxf(1.f - R, 1e-3f);
+        const float F = 4.f * R / (omr * omr);
+        const float d_nm = (a.hi[cc] + a.lo[cc]) * 1e9f;
+        const float op = 2.f * a.n_film * d_nm * cos_t;
+
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < kSpecChunks; ++t) {
+            float I[4];
+            const int nv = (t == kSpecChunks - 1) ? 1 : 4;   // slots 84..95 are padding
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (v < nv) {
+                    // sin(phase / 2) with phase = 2 pi op / lambda + pi is
+                    // cos(pi op / lambda) = cos(2 pi x), x = op / (2 lambda).
+                    const float x = op * u[t][v];
+                    const float xr = x - rintf(x);
+                    const float c = __builtin_amdgcn_cosf(xr);
+                    const float fs2 = F * (c * c);
+                    I[v] = fs2 * __builtin_amdgcn_rcpf(1.f + fs2);
+                } else {
+                    I[v] = 0.f;
+                }
+            }
+            uint32_t h01, l01, h23, l23;
+            split_pack(I[0], I[1], h01, l01);
+            split_pack(I[2], I[3], h23, l23);
+            const v4s ah = v4s{static_cast<short>(h01 & 0xFFFF), static_cast<short>(h01 >> 16),
+                               static_cast<short>(h23 & 0xFFFF), static_cast<short>(h23 >> 16)};
+            const v4s al = v4s{static_cast<short>(l01 & 0xFFFF), static_cast<short>(l01 >> 16),
+                               static_cast<short>(l23 & 0xFFFF), static_cast<short>(l23 >> 16)};
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[t], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[t], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[t], acc, 0, 0, 0);
+        }
+        // acc[v] = D[4 q + v][m]: cell grp*16 + 4q + v, colour m.
+        if (m < 3) {
+            for (int v = 0; v < 4; ++v) {
+                const size_t oc = grp * 16 + 4 * q + v;
+                if (oc < a.n) a.rgb[static_cast<size_t>(m) * a.n + oc] = acc[v] * scale;
+            }
+        }
+    }
+}
+
+static inline unsigned cdiv_u(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+template <class T>
+static T* dalloc_i(size_t count) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
+    if (e != hipSuccess) {
+        set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        throw Error{SF_ERR_NOMEM};
+    }
+    return static_cast<T*>(p);
+}
+
+// Device scratch for the stateless calls, freed on exit of the call.
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) {
+        hipError_t e = hipMalloc(&p, bytes + 16);
+        if (e != hipSuccess) {
+            set_error("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+            throw Error{SF_ERR_NOMEM};
+        }
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+static void run_calc(CalcArgs a, size_t out_floats, float* host_out, size_t threads,
+                     const float* d_nm, size_t n_d, const float* cos_i, size_t n_c,
+                     const float* lam, size_t n_l) {
+    require_device();
+    DevBuf out(out_floats * sizeof(float));
+    DevBuf in((n_d + n_c + n_l) * sizeof(float) + 16);
+    a.out = out.as<float>();
+    float* base = in.as<float>();
+    if (n_d) SF_HIP(hipMemcpy(base, d_nm, n_d * sizeof(float), hipMemcpyHostToDevice));
+    if (n_c) SF_HIP(hipMemcpy(base + n_d, cos_i, n_c * sizeof(float), hipMemcpyHostToDevice));
+    if (n_l) SF_HIP(hipMemcpy(base + n_d + n_c, lam, n_l * sizeof(float), hipMemcpyHostToDevice));
+    a.d_nm = base;
+    a.cos_i = base + n_d;
+    a.lambda = base + n_d + n_c;
+    if (threads) {
+        hipLaunchKernelGGL(calc_kernel, dim3(cdiv_u(threads, 256)), dim3(256), 0, nullptr, a);
+        SF_HIP(hipGetLastError());
+    }
+    SF_HIP(hipMemcpy(host_out, a.out, out_floats * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+extern "C" int sf_lut_rgba8(float refractive_index, float intensity, uint8_t* out) {
+    return guard([&] {
+        SF_REQUIRE(out != nullptr, SF_ERR_INVALID, "out is NULL");
+        require_device();
+        DevBuf d(static_cast<size_t>(kLutT) * kLutA * 4);
+        hipLaunchKernelGGL(lut_rgb_kernel, dim3(kLutT / 64, kLutA), dim3(64), 0, nullptr,
+                           refractive_index, intensity, static_cast<float*>(nullptr), d.as<uint8_t>());
+        SF_HIP(hipGetLastError());
+        SF_HIP(hipMemcpy(out, d.p, static_cast<size_t>(kLutT) * kLutA * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" int sf_lut_rgb_f32(float refractive_index, float intensity, float* out) {
+    return guard([&] {
+        SF_REQUIRE(out != nullptr, SF_ERR_INVALID, "out is NULL");
+        require_device();
+        const size_t nb = static_cast<size_t>(kLutT) * kLutA * 3 * sizeof(float);
+        DevBuf d(nb);
+        hipLaunchKernelGGL(lut_rgb_kernel, dim3(kLutT / 64, kLutA), dim3(64), 0, nullptr,
+                           refractive_index, intensity, d.as<float>(), static_cast<uint8_t*>(nullptr));
+        SF_HIP(hipGetLastError());
+        SF_HIP(hipMemcpy(out, d.p, nb, hipMemcpyDeviceToHost));
+    });
+}
+
+static CalcArgs calc_base(double n, const double* w) {
+    CalcArgs a{};
+    a.n_film = static_cast<float>(n);
+    a.l0 = static_cast<float>(w ? w[0] : 650.0);   // wavelengths::RED_NM (interference.rs:26)
+    a.l1 = static_cast<float>(w ? w[1] : 532.0);   // GREEN_NM (28)
+    a.l2 = static_cast<float>(w ? w[2] : 450.0);   // BLUE_NM (30)
+    return a;
+}
+
+extern "C" int sf_lut_2d(double refractive_index, const double wavelengths_nm[3], double tmin_nm,
+                         double tmax_nm, int32_t n_thickness, int32_t n_angle, int32_t gamma,
+                         float* out) {
+    return guard([&] {
+#pragma clang fp contract(off)
+        SF_REQUIRE(out != nullptr && n_thickness >= 0 && n_angle >= 0, SF_ERR_INVALID,
+                   "bad arguments");
+        SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
+        CalcArgs a = calc_base(refractive_index, wavelengths_nm);
+        a.mode = 0;
+        a.nt = n_thickness;
+        a.na = n_angle;
+        a.gamma = gamma != 0;
+        const size_t cells = static_cast<size_t>(n_thickness) * n_angle;
+        if (cells == 0) return;
+        // generate_2d_color_lookup_table addressing (interference.rs:497-507).
+        const double tstep = (tmax_nm - tmin_nm) / static_cast<double>(std::max(n_thickness - 1, 1));
+        const double astep = 1.0 / static_cast<double>(std::max(n_angle - 1, 1));
+        std::vector<float> t(n_thickness), c(n_angle);
+        for (int i = 0; i < n_thickness; ++i) t[i] = static_cast<float>(tmin_nm + static_cast<double>(i) * tstep);
+        for (int j = 0; j < n_angle; ++j) c[j] = static_cast<float>(static_cast<double>(j) * astep);
+        run_calc(a, 3 * cells, out, cells, t.data(), t.size(), c.data(), c.size(), nullptr, 0);
+    });
+}
+
+extern "C" int sf_lut_1d(double refractive_index, const double wavelengths_nm[3], double tmin_nm,
+                         double tmax_nm, int32_t n, double cos_theta, float* out) {
+    return guard([&] {
+#pragma clang fp contract(off)
+        SF_REQUIRE(out != nullptr && n >= 0, SF_ERR_INVALID, "bad arguments");
+        SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
+        CalcArgs a = calc_base(refractive_index, wavelengths_nm);
+        a.mode = 1;
+        a.nt = n;
+        if (n == 0) return;
+        // generate_color_lookup_table addressing (interference.rs:469-473).
+        const double step = (tmax_nm - tmin_nm) / static_cast<double>(n - 1);
+        std::vector<float> t(n);
+        for (int i = 0; i < n; ++i) t[i] = static_cast<float>(tmin_nm + static_cast<double>(i) * step);
+        const float c = static_cast<float>(cos_theta);
+        run_calc(a, 3 * static_cast<size_t>(n), out, n, t.data(), t.size(), &c, 1, nullptr, 0);
+    });
+}
+
+extern "C" int sf_reflected_intensity(double refractive_index, int32_t n, const float* d_nm,
+                                      const float* cos_i, const float* lambda_nm, float* out) {
+    return guard([&] {
+        SF_REQUIRE(n >= 0 && (n == 0 || (d_nm && cos_i && lambda_nm && out)), SF_ERR_INVALID,
+                   "bad arguments");
+        if (n == 0) return;
+        CalcArgs a = calc_base(refractive_index, nullptr);
+        a.mode = 3;
+        a.count = n;
+        run_calc(a, n, out, n, d_nm, n, cos_i, n, lambda_nm, n);
+    });
+}
+
+extern "C" int sf_interference_color(double refractive_index, const double wavelengths_nm[3],
+                                     int32_t n, const float* d_nm, const float* cos_i,
+                                     int32_t gamma, float* out) {
+    return guard([&] {
+        SF_REQUIRE(n >= 0 && (n == 0 || (d_nm && cos_i && out)), SF_ERR_INVALID, "bad arguments");
+        if (n == 0) return;
+        CalcArgs a = calc_base(refractive_index, wavelengths_nm);
+        a.mode = 2;
+        a.count = n;
+        a.gamma = gamma != 0;
+        run_calc(a, 3 * static_cast<size_t>(n), out, n, d_nm, n, cos_i, n, nullptr, 0);
+    });
+}
+
+extern "C" int sf_interference_terms(double refractive_index, int32_t n, const float* d_nm,
+                                     const float* cos_i, const float* lambda_nm, float* out) {
+    return guard([&] {
+        SF_REQUIRE(n >= 0 && (n == 0 || (d_nm && cos_i && lambda_nm && out)), SF_ERR_INVALID,
+                   "bad arguments");
+        if (n == 0) return;
+        CalcArgs a = calc_base(refractive_index, nullptr);
+        a.mode = 4;
+        a.count = n;
+        run_calc(a, 6 * static_cast<size_t>(n), out, n, d_nm, n, cos_i, n, lambda_nm, n);
+    });
+}
+
+// Packs cmf[81][3] into the MFMA B-operand layout (see spectral_kernel),
+// split into truncated-BF16 hi and lo parts.
+static std::vector<uint2> pack_cmf(const float* cmf) {
+    std::vector<uint2> frag(2 * kSpecChunks * 64);
+    auto bits = [](float x) {
+        uint32_t u;
+        std::memcpy(&u, &x, 4);
+        return u;
+    };
+    for (int t = 0; t < kSpecChunks; ++t)
+        for (int lane = 0; lane < 64; ++lane) {
+            uint16_t hv[4], lv[4];
+            for (int v = 0; v < 4; ++v) {
+                const int k = 16 * t + 4 * (lane >> 4) + v;
+                const int col = lane & 15;
+                const float x = (k < kSpecLambdas && col < 3) ? cmf[3 * k + col] : 0.f;
+                const uint32_t hb = bits(x) & 0xFFFF0000u;
+                float hf;
+                std::memcpy(&hf, &hb, 4);
+                const float rem = x - hf;
+                hv[v] = static_cast<uint16_t>(hb >> 16);
+                lv[v] = static_cast<uint16_t>(bits(rem) >> 16);
+            }
+            frag[t * 64 + lane] = make_uint2(hv[0] | (uint32_t(hv[1]) << 16), hv[2] | (uint32_t(hv[3]) << 16));
+            frag[kSpecChunks * 64 + t * 64 + lane] =
+                make_uint2(lv[0] | (uint32_t(lv[1]) << 16), lv[2] | (uint32_t(lv[3]) << 16));
+        }
+    return frag;
+}
+
+namespace sf {
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf) {
+    if (!d->rgb) d->rgb = dalloc_i<float>(3 * d->n);
+    if (!d->spec_rows) {
+        d->spec_rows = dalloc_i<float4>(d->nt);
+        d->spec_cols = dalloc_i<float2>(d->np);
+        std::vector<float4> rw(d->nt);
+        std::vector<float2> cl(d->np);
+        for (int i = 0; i < d->nt; ++i) {
+            const double th = static_cast<double>(i) * d->dtheta;
+            rw[i] = make_float4(static_cast<float>(std::sin(th)), static_cast<float>(std::cos(th)), 0.f, 0.f);
+        }
+        for (int j = 0; j < d->np; ++j) {
+            const double ph = static_cast<double>(j) * d->dphi;
+            cl[j] = make_float2(static_cast<float>(std::sin(ph)), static_cast<float>(std::cos(ph)));
+        }
+        SF_HIP(hipMemcpyAsync(d->spec_rows, rw.data(), d->nt * sizeof(float4), hipMemcpyHostToDevice, d->stream));
+        SF_HIP(hipMemcpyAsync(d->spec_cols, cl.data(), d->np * sizeof(float2), hipMemcpyHostToDevice, d->stream));
+        SF_HIP(hipStreamSynchronize(d->stream));
+    }
+    if (cmf) {
+        if (!d->cmf_dev) d->cmf_dev = reinterpret_cast<float*>(dalloc_i<uint2>(2 * kSpecChunks * 64));
+        const std::vector<uint2> frag = pack_cmf(cmf);
+        SF_HIP(hipMemcpyAsync(d->cmf_dev, frag.data(), frag.size() * sizeof(uint2), hipMemcpyHostToDevice, d->stream));
+        SF_HIP(hipStreamSynchronize(d->stream));
+    }
+    SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
+    d->materialize();
+    SpecArgs a;
+    a.hi = d->hi[d->cur_hi];
+    a.lo = d->lo[d->cur];
+    a.rows = d->spec_rows;
+    a.cols = d->spec_cols;
+    a.bfrag = reinterpret_cast<const uint2*>(d->cmf_dev);
+    a.rgb = d->rgb;
+    a.np = d->np;
+    a.n = d->n;
+    const sf_camera c = cam ? *cam : sf_camera{{0.0, 0.0, 4.0 * d->p.bubble_radius}};
+    a.cx = static_cast<float>(c.position[0]);
+    a.cy = static_cast<float>(c.position[1]);
+    a.cz = static_cast<float>(c.position[2]);
+    a.cam2 = static_cast<float>(c.position[0] * c.position[0] + c.position[1] * c.position[1] +
+                                c.position[2] * c.position[2]);
+    a.radius = static_cast<float>(d->p.bubble_radius);
+    a.n_film = static_cast<float>(n_film);
+    a.inv_n_film = static_cast<float>(1.0 / n_film);
+    const size_t groups = (d->n + 15) / 16;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((groups + 3) / 4, 256 * 16));
+    if (d->spec_timer.enabled) {
+        auto& ev = d->spec_timer.next(d->stream);
+        SF_HIP(hipEventRecord(ev.first, d->stream));
+        hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, a);
+        SF_HIP(hipEventRecord(ev.second, d->stream));
+    } else {
+        hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, a);
+    }
+    SF_HIP(hipGetLastError());
+}
+}  // namespace sf
+
+extern "C" int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_index,
+                               const float* cmf, float* rgb_host) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
+        SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
+        Drainage* d = reinterpret_cast<Drainage*>(ctx);
+        spectral_launch(d, cam, refractive_index, cmf);
+        if (rgb_host) {
+            SF_HIP(hipMemcpyAsync(rgb_host, d->rgb, 3 * d->n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+            SF_HIP(hipStreamSynchronize(d->stream));
+        }
+    });
+}
+
+extern "C" int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && rgb != nullptr, SF_ERR_INVALID, "NULL argument");
+        Drainage* d = reinterpret_cast<Drainage*>(ctx);
+        SF_REQUIRE(d->rgb != nullptr, SF_ERR_INVALID, "sf_spectral_rgb has not run on this context");
+        *rgb = d->rgb;
+    });
+}
