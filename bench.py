@@ -1,0 +1,232 @@
+"""Benchmark of the drainage + interference path on one MI355X.
+
+Default workload (one "step" = one drainage step of the whole grid followed
+by the 81-wavelength spectral RGB of the new state):
+  single bubble, 2048 x 4096 lat-lon grid, FP32 float-float thickness + FP32
+  surfactant, smooth spherical-harmonic (degree <= 16, 10 %) initial film of
+  200-1000 nm, CFL-limited dt from the FP64 reduction, per-step FP64 film
+  volume and CFL reductions, fixed synthetic camera, three seeded Gaussian
+  colour curves contracted on BF16 MFMA.
+
+Prints one JSON line.  ``value`` is cell-steps/s of the whole step with the
+fields resident in HBM; ``roofline`` is the drainage stencil kernel against
+MI355X HBM peak (16 B per cell-step: h and surfactant read + write in FP32),
+timed with HIP events on the context's stream; ``cpu_baseline`` is the FP64
+oracle on a bounded sample of the same workload on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c2-drainage|c1]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "drainage cell-steps/s plain vs surrogate at ≤1% rel-L2 h; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STEP = 16.0       # h and surfactant, FP32, read + write
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the float4 copy kernel (counter calibration)")
+    return ap.parse_args()
+
+
+def copy_probe(L, nbytes=256 << 20, iters=20):
+    ms = C.c_double()
+    L.check(L.lib().sf_copy_probe(nbytes, iters, C.byref(ms)))
+    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+def pmc_traffic():
+    """HBM bytes per drainage launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE, WRITE_SIZE per dispatch, scaled by the copy probe)."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            d = json.load(f)
+        return float(d["traffic_bytes_per_launch"]), d
+    except (OSError, ValueError, KeyError):
+        return None, None
+
+
+def cpu_baseline(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s):
+    """FP64 oracle on the host: full-grid drainage steps plus (when the
+    workload renders) the spectral RGB of sampled cells, extrapolated per
+    cell-step.  Single thread (the reference loop is sequential)."""
+    q = O.Params()
+    for name, _ in O.Params._fields_:
+        setattr(q, name, getattr(params, name))
+    sim = O.Sim(q, n_theta, n_phi)
+    sim.set_fields(h0, np.full_like(h0, 0.5))
+    n_cells = n_theta * n_phi
+    t0 = time.perf_counter()
+    steps = 0
+    drain_budget = budget_s * (0.6 if spectral else 1.0)
+    while steps < 1000 and (time.perf_counter() - t0) < drain_budget:
+        sim.step(dt)
+        steps += 1
+    t_drain = (time.perf_counter() - t0) / (steps * n_cells)
+    sample = "%d FP64 oracle drainage steps of the full %dx%d grid" % (steps, n_theta, n_phi)
+    t_spec = 0.0
+    if spectral:
+        h, _ = sim.fields()
+        rng = np.random.default_rng(0)
+        m = 20000
+        t1 = time.perf_counter()
+        done = 0
+        while done < 10 * m and (time.perf_counter() - t1) < budget_s * 0.4:
+            cells = rng.choice(n_cells, m, replace=False)
+            O.spectral_cells(h, cells, params.bubble_radius, cam, 1.33, cmf.astype(np.float64))
+            done += m
+        t_spec = (time.perf_counter() - t1) / done
+        sample += " + 81-wavelength FP64 spectral RGB of %d sampled cells" % done
+    per = t_drain + t_spec
+    return {"value": 1.0 / per, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": sample + ", 1 thread, extrapolated per cell-step"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the per-GPU figure at --gpus 1 is the measurement)", file=sys.stderr)
+        sys.exit(2)
+
+    from soap_bubble_simulation_amd import _lib as L
+    from soap_bubble_simulation_amd import workloads as W
+    from soap_bubble_simulation_amd.drainage import DrainageContext
+
+    if not L.device_available():
+        print("no GPU visible: bench.py needs an MI355X", file=sys.stderr)
+        sys.exit(1)
+    if args.copy_probe:
+        ms, gbs = copy_probe(L)
+        print(json.dumps({"copy_probe_ms": ms, "copy_probe_GBps": gbs, "bytes": 2 * (256 << 20)}))
+        return
+
+    spectral = args.workload == "c2"
+    if args.workload == "c1":
+        n_theta, n_phi = 128, 256
+        h0 = W.noise_field(n_theta, n_phi, seed=1)
+        flags = L.FLAG_TFE | L.FLAG_DIAG
+        desc = "single bubble 128x256 FP32 drainage (reference default grid), TFE fronts, CFL dt"
+    else:
+        n_theta, n_phi = 2048, 4096
+        h0 = W.smooth_bubble(n_theta, n_phi, seed=3)
+        flags = L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG
+        desc = ("single bubble 2048x4096 FP32 drainage, CFL dt"
+                + (" + 81-wavelength spectral RGB per step (BF16 MFMA)" if spectral else ""))
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    p.flags = flags
+    ctx = DrainageContext(p, n_theta, n_phi, 500e-9)
+    ctx.set_fields(h0, np.full_like(h0, 0.5))
+    dt = ctx.stats().dt_cfl * p.cfl_safety
+    cam = W.camera(p.bubble_radius)
+    cmf = W.spectral_cmf(3).astype(np.float32)
+    camera = L.Camera()
+    for k in range(3):
+        camera.position[k] = cam[k]
+    lib = L.lib()
+
+    def run(n):
+        if spectral:
+            L.check(lib.sf_drainage_step_spectral(ctx.handle, n, dt, C.byref(camera), 1.33,
+                                                  L.fptr(cmf)))
+        else:
+            L.check(lib.sf_drainage_step(ctx.handle, n, dt))
+
+    # warmup (also uploads the colour curves)
+    run(max(1, args.warmup))
+    L.check(lib.sf_drainage_synchronize(ctx.handle))
+
+    def timed_pass():
+        L.check(lib.sf_drainage_synchronize(ctx.handle))
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.steps:
+            n = min(500, args.steps - done)
+            run(n)
+            done += n
+        L.check(lib.sf_drainage_synchronize(ctx.handle))
+        return time.perf_counter() - t0
+
+    # Kernel-timing pass: a HIP event pair brackets every launch on the
+    # context's stream (average kernel durations for the roofline).
+    ctx.kernel_timing(True)
+    ms_spec = C.c_double()
+    n_spec = C.c_int64()
+    L.check(lib.sf_spectral_kernel_timing(ctx.handle, 1 if spectral else 0, C.byref(ms_spec),
+                                          C.byref(n_spec)))
+    timed_pass()
+    drain_ms, drain_n = ctx.kernel_timing(False)
+    L.check(lib.sf_spectral_kernel_timing(ctx.handle, 0, C.byref(ms_spec), C.byref(n_spec)))
+
+    # Headline pass: exactly K steps, no per-launch events, synchronised on
+    # both sides.
+    elapsed = timed_pass()
+
+    cells = n_theta * n_phi
+    value = cells * args.steps / elapsed
+    ms_per_step = elapsed * 1e3 / args.steps
+    achieved = BYTES_PER_CELL_STEP * cells / (drain_ms * 1e-3) / 1e9
+    traffic, pmc = pmc_traffic()
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "drainage_step_kernel", "avg_launch_ms": drain_ms,
+                "launches_timed": drain_n,
+                "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells}
+    stats = ctx.stats()
+    h, g = ctx.fields()
+    finite = bool(np.isfinite(h).all() and np.isfinite(g).all())
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "n_theta": n_theta, "n_phi": n_phi, "dt_s": dt,
+                   "cells": cells, "flags": int(flags)},
+        "roofline": roofline,
+        "kernels": {
+            "drainage_step_ms": drain_ms,
+            "spectral_ms": ms_spec.value if spectral else None,
+            "drainage_only_cell_steps_per_s": cells / (drain_ms * 1e-3),
+            "spectral_cells_per_s": cells / (ms_spec.value * 1e-3) if spectral and ms_spec.value else None,
+        },
+        "state": {"min_h": stats.min_thickness, "max_h": stats.max_thickness,
+                  "mass_m3": stats.mass, "time_s": stats.current_time, "finite": finite},
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+        out["cpu_baseline"] = cpu_baseline(O, p, h0, n_theta, n_phi, dt, cam, cmf, spectral,
+                                           args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -98,6 +98,10 @@ const char* sf_last_error(void);
 const char* sf_version(void);
 /* 1 when a HIP device is usable, 0 otherwise (never fails). */
 int sf_device_available(void);
+/* Device-to-device float4 copy of n_bytes, timed over iters launches with
+ * HIP events: the achievable-bandwidth reference of the roofline and the
+ * calibration kernel of the FETCH_SIZE / WRITE_SIZE counters. */
+int sf_copy_probe(size_t n_bytes, int32_t iters, double* avg_ms);
 
 /* Defaults of DrainageSimulator::new(&SimulationConfig::default())
  * (drainage.rs:263-288, config.rs:305-316): 10 nm rupture, TFE on, h poles
@@ -212,6 +216,13 @@ int sf_interference_color(double refractive_index, const double wavelengths_nm[3
  *  R_s, R_p, R_avg (266-298)}. */
 int sf_interference_terms(double refractive_index, int32_t n, const float* d_nm,
                           const float* cos_i, const float* lambda_nm, float* out);
+/* calculate_optical_path_difference and calculate_phase_difference
+ * (interference.rs:226-254) for given transmission cosines:
+ * out[k][2] = {OPD, phase}. */
+int sf_optical_path(double refractive_index, int32_t n, const float* d_nm, const float* cos_t,
+                    const float* lambda_nm, float* out);
+/* calculate_phase_difference (interference.rs:248-254): out[k] = 2 pi OPD / lambda. */
+int sf_phase_difference(int32_t n, const float* opd_nm, const float* lambda_nm, float* out);
 
 /* Camera of the spectral evaluation: world position, bubble centred at the
  * origin, y up (theta = 0 is the top). */
@@ -228,6 +239,14 @@ typedef struct sf_camera {
  * to host memory (rgb_host) and/or keeps them on the device. */
 int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_index,
                     const float* cmf, float* rgb_host);
+/* n_steps of {drainage step, spectral RGB of the new state} back to back on
+ * the context's stream: the per-step workload of a rendered simulation.
+ * dt as in sf_drainage_step; cmf may be NULL after the first call.
+ * Asynchronous; the RGB planes stay on the device. */
+int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, double dt, const sf_camera* cam,
+                              double refractive_index, const float* cmf);
+/* Blocks until all work enqueued on ctx has finished. */
+int sf_drainage_synchronize(sf_drainage* ctx);
 /* Device pointer of the last spectral RGB planes of ctx (3*n_theta*n_phi). */
 int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb);
 /* Average duration in ms of the spectral kernel launches since the last call

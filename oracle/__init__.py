@@ -109,6 +109,8 @@ def lib():
             "orc_lut_rgba8": (None, [C.c_float, C.c_float, u8p]),
             "orc_view_cos": (C.c_double, [C.c_double, C.c_double, C.c_double, dp]),
             "orc_spectral_rgb": (None, [dp, C.c_int, C.c_int, C.c_double, dp, C.c_double, dp, dp]),
+            "orc_spectral_cells": (None, [dp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int,
+                                          C.c_double, dp, C.c_double, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -299,4 +301,16 @@ def spectral_rgb(h, radius, cam, n_film, cmf):
     cm = np.ascontiguousarray(cmf, np.float64).reshape(81 * 3)
     out = np.zeros((3, nt, npf), np.float64)
     lib().orc_spectral_rgb(_dp(h), nt, npf, radius, _dp(c), n_film, _dp(cm), _dp(out))
+    return out
+
+
+def spectral_cells(h, cells, radius, cam, n_film, cmf):
+    h = np.ascontiguousarray(h, np.float64)
+    nt, npf = h.shape
+    cl = np.ascontiguousarray(cells, np.int64)
+    c = np.asarray(cam, np.float64)
+    cm = np.ascontiguousarray(cmf, np.float64).reshape(81 * 3)
+    out = np.zeros((cl.size, 3), np.float64)
+    lib().orc_spectral_cells(_dp(h), nt, npf, cl.ctypes.data_as(C.POINTER(C.c_int64)), cl.size,
+                             radius, _dp(c), n_film, _dp(cm), _dp(out))
     return out

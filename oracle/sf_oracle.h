@@ -113,6 +113,10 @@ double orc_view_cos(double theta, double phi, double radius, const double cam[3]
 void   orc_spectral_rgb(const double* h, int n_theta, int n_phi, double radius,
                         const double cam[3], double n_film, const double* cmf /* 81*3 */,
                         double* rgb /* 3 planes of n_theta*n_phi */);
+/* The same at selected flat cell indices: rgb[k][3]. */
+void   orc_spectral_cells(const double* h, int n_theta, int n_phi, const int64_t* cells,
+                          int n_cells, double radius, const double cam[3], double n_film,
+                          const double* cmf, double* rgb);
 
 #ifdef __cplusplus
 }

@@ -269,39 +269,58 @@ double orc_view_cos(double theta, double phi, double radius, const double cam[3]
 }
 
 /* rgb_c = (1/81) Σ_k Airy(h, cosθ_view, λ_k) cmf[k][c], λ_k = 380 + 5k nm,
- * with the Snell / Fresnel / Airy forms of interference_lut.rs:54-72. */
+ * with the Snell / Fresnel / Airy forms of interference_lut.rs:54-72, at one
+ * point (theta, phi) of thickness h (m). */
+static void spectral_point(double theta, double phi, double h, double radius, const double cam[3],
+                           double n_film, const double* cmf, double out[3])
+{
+    double cos_i = orc_view_cos(theta, phi, radius, cam);
+    double d_nm = h * 1e9;
+    double sin_i = sqrt(fmax(1.0 - cos_i * cos_i, 0.0));
+    double sin_t = sin_i / n_film;
+    double cos_t = sqrt(fmax(1.0 - sin_t * sin_t, 0.0));
+    double rs = (cos_i - n_film * cos_t) / fmax(cos_i + n_film * cos_t, 1e-4);
+    double rp = (n_film * cos_i - cos_t) / fmax(n_film * cos_i + cos_t, 1e-4);
+    double R = (rs * rs + rp * rp) * 0.5;
+    double omr = fmax(1.0 - R, 1e-3);
+    double F = 4.0 * R / (omr * omr);
+    double op = 2.0 * n_film * d_nm * cos_t;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 81; ++k) {
+        double lam = 380.0 + 5.0 * (double)k;
+        double phase = 2.0 * ORC_PI * op / lam + ORC_PI;
+        double s = sin(0.5 * phase);
+        double fs2 = F * s * s;
+        double I = fs2 / fmax(1.0 + fs2, 1e-3);
+        for (int cc = 0; cc < 3; ++cc) acc[cc] += I * cmf[3 * k + cc];
+    }
+    for (int cc = 0; cc < 3; ++cc) out[cc] = acc[cc] / 81.0;
+}
+
 void orc_spectral_rgb(const double* h, int n_theta, int n_phi, double radius, const double cam[3],
                       double n_film, const double* cmf, double* rgb)
 {
     const double dtheta = ORC_PI / (double)(n_theta - 1);
     const double dphi = 2.0 * ORC_PI / (double)n_phi;
     const size_t plane = (size_t)n_theta * n_phi;
-    for (int i = 0; i < n_theta; ++i) {
-        double theta = (double)i * dtheta;
+    for (int i = 0; i < n_theta; ++i)
         for (int j = 0; j < n_phi; ++j) {
-            double phi = (double)j * dphi;
             size_t c = (size_t)i * n_phi + j;
-            double cos_i = orc_view_cos(theta, phi, radius, cam);
-            double d_nm = h[c] * 1e9;
-            double sin_i = sqrt(fmax(1.0 - cos_i * cos_i, 0.0));
-            double sin_t = sin_i / n_film;
-            double cos_t = sqrt(fmax(1.0 - sin_t * sin_t, 0.0));
-            double rs = (cos_i - n_film * cos_t) / fmax(cos_i + n_film * cos_t, 1e-4);
-            double rp = (n_film * cos_i - cos_t) / fmax(n_film * cos_i + cos_t, 1e-4);
-            double R = (rs * rs + rp * rp) * 0.5;
-            double omr = fmax(1.0 - R, 1e-3);
-            double F = 4.0 * R / (omr * omr);
-            double op = 2.0 * n_film * d_nm * cos_t;
-            double acc[3] = {0.0, 0.0, 0.0};
-            for (int k = 0; k < 81; ++k) {
-                double lam = 380.0 + 5.0 * (double)k;
-                double phase = 2.0 * ORC_PI * op / lam + ORC_PI;
-                double s = sin(0.5 * phase);
-                double fs2 = F * s * s;
-                double I = fs2 / fmax(1.0 + fs2, 1e-3);
-                for (int cc = 0; cc < 3; ++cc) acc[cc] += I * cmf[3 * k + cc];
-            }
-            for (int cc = 0; cc < 3; ++cc) rgb[cc * plane + c] = acc[cc] / 81.0;
+            double out[3];
+            spectral_point((double)i * dtheta, (double)j * dphi, h[c], radius, cam, n_film, cmf, out);
+            for (int cc = 0; cc < 3; ++cc) rgb[cc * plane + c] = out[cc];
         }
+}
+
+void orc_spectral_cells(const double* h, int n_theta, int n_phi, const int64_t* cells, int n_cells,
+                        double radius, const double cam[3], double n_film, const double* cmf,
+                        double* rgb)
+{
+    const double dtheta = ORC_PI / (double)(n_theta - 1);
+    const double dphi = 2.0 * ORC_PI / (double)n_phi;
+    for (int k = 0; k < n_cells; ++k) {
+        int64_t c = cells[k];
+        int i = (int)(c / n_phi), j = (int)(c % n_phi);
+        spectral_point((double)i * dtheta, (double)j * dphi, h[c], radius, cam, n_film, cmf, rgb + 3 * k);
     }
 }

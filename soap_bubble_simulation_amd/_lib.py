@@ -90,6 +90,7 @@ SIGNATURES = {
     "sf_last_error": (C.c_char_p, []),
     "sf_version": (C.c_char_p, []),
     "sf_device_available": (C.c_int, []),
+    "sf_copy_probe": (C.c_int, [C.c_size_t, C.c_int32, _dp]),
     "sf_params_cpu_default": (C.c_int, [C.POINTER(Params)]),
     "sf_params_gpu_default": (C.c_int, [C.POINTER(Params)]),
     "sf_drainage_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_double,
@@ -122,6 +123,14 @@ SIGNATURES = {
                             C.c_int32, _fp]),
     "sf_lut_1d": (C.c_int, [C.c_double, _dp, C.c_double, C.c_double, C.c_int32, C.c_double, _fp]),
     "sf_reflected_intensity": (C.c_int, [C.c_double, C.c_int32, _fp, _fp, _fp, _fp]),
+    "sf_interference_color": (C.c_int, [C.c_double, _dp, C.c_int32, _fp, _fp, C.c_int32, _fp]),
+    "sf_interference_terms": (C.c_int, [C.c_double, C.c_int32, _fp, _fp, _fp, _fp]),
+    "sf_optical_path": (C.c_int, [C.c_double, C.c_int32, _fp, _fp, _fp, _fp]),
+    "sf_phase_difference": (C.c_int, [C.c_int32, _fp, _fp, _fp]),
+    "sf_spectral_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "sf_drainage_step_spectral": (C.c_int, [_vp, C.c_int32, C.c_double, C.POINTER(Camera),
+                                            C.c_double, _fp]),
+    "sf_drainage_synchronize": (C.c_int, [_vp]),
     "sf_spectral_rgb": (C.c_int, [_vp, C.POINTER(Camera), C.c_double, _fp, _fp]),
     "sf_spectral_device_rgb": (C.c_int, [_vp, C.POINTER(_vp)]),
 }

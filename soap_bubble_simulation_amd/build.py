@@ -50,6 +50,21 @@ def _compile(src: str) -> str:
     return obj
 
 
+CPP_TEST_SRC = os.path.join(REPO_DIR, "tests", "cpp", "test_api.cpp")
+CPP_TEST_BIN = os.path.join(REPO_DIR, "tests", "cpp", "test_api")
+
+
+def build_cpp_test() -> str:
+    """Compile the C++ API test (tests/cpp/test_api.cpp) against the library."""
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-I", INCLUDE, CPP_TEST_SRC, "-o", CPP_TEST_BIN,
+           "-L", PKG_DIR, "-lsoapfilm", "-Wl,-rpath," + PKG_DIR]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("C++ API test failed to compile")
+    return CPP_TEST_BIN
+
+
 def build(verbose: bool = False) -> str:
     """Compile all HIP sources and link libsoapfilm.so; returns its path."""
     os.makedirs(OBJ_DIR, exist_ok=True)
@@ -64,6 +79,8 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libsoapfilm.so failed")
     os.replace(tmp, LIB_PATH)
+    if os.path.exists(CPP_TEST_SRC):
+        build_cpp_test()
     if verbose:
         print("built", LIB_PATH)
     return LIB_PATH

@@ -454,15 +454,18 @@ void Drainage::materialize() {
     poles_virtual = false;
 }
 
+double Drainage::resolve_dt(double dt) {
+    if (dt > 0.0) return dt;
+    const StatsPart s = compute_stats();
+    const double cfl = s.lam > 0.0 ? 2.0 / s.lam : INFINITY;
+    SF_REQUIRE(std::isfinite(cfl) && cfl > 0.0, SF_ERR_INVALID,
+               "CFL time step is not finite (is diffusion zero?)");
+    return p.cfl_safety * cfl;
+}
+
 void Drainage::step(int n_steps, double dt) {
     if (tables_dirty) upload_tables();
-    if (!(dt > 0.0)) {
-        const StatsPart s = compute_stats();
-        const double cfl = s.lam > 0.0 ? 2.0 / s.lam : INFINITY;
-        SF_REQUIRE(std::isfinite(cfl) && cfl > 0.0, SF_ERR_INVALID,
-                   "CFL time step is not finite (is diffusion zero?)");
-        dt = p.cfl_safety * cfl;
-    }
+    dt = resolve_dt(dt);
     const bool marangoni = (p.flags & SF_FLAG_MARANGONI) != 0;
     const bool surfactant = (p.flags & SF_FLAG_SURFACTANT) != 0;
     const int gmode = surfactant ? 2 : (marangoni ? 1 : 0);
@@ -1015,6 +1018,13 @@ extern "C" int sf_drainage_device_fields(sf_drainage* ctx, const float** h_hi, c
         if (h_hi) *h_hi = d->hi[d->cur_hi];
         if (h_lo) *h_lo = d->lo[d->cur];
         if (gamma) *gamma = d->g[d->cur_g];
+    });
+}
+
+extern "C" int sf_drainage_synchronize(sf_drainage* ctx) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
+        SF_HIP(hipStreamSynchronize(D(ctx)->stream));
     });
 }
 

@@ -109,6 +109,7 @@ struct Drainage {
     void schedule_fronts(double dt, StepArgs& a);
     void fill_common_args(StepArgs& a, float dt_f) const;
     void materialize();
+    double resolve_dt(double dt);
     void step(int n_steps, double dt);
     StatsPart compute_stats();
     void ensure_gather(size_t count);

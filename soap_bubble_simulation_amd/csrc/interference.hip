@@ -122,6 +122,7 @@ struct CalcTerms {
 // calculate_transmission_angle_cos / optical path / phase / Fresnel
 // (interference.rs:197-298), n_medium = 1 (REFRACTIVE_INDEX_AIR).
 __device__ inline CalcTerms calc_terms(float n_film, float d_nm, float cos_i, float lambda) {
+#pragma clang fp contract(off)
     CalcTerms r;
     const float c = fminf(fmaxf(cos_i, -1.f), 1.f);
     const float sin_i = sqrtf(1.f - c * c);
@@ -143,6 +144,7 @@ __device__ inline CalcTerms calc_terms(float n_film, float d_nm, float cos_i, fl
 // reduced to one period as OPD / lambda - round(OPD / lambda) before the
 // cosine, which keeps FP32 accurate for thick films.
 __device__ inline float calc_intensity(float n_film, float d_nm, float cos_i, float lambda) {
+#pragma clang fp contract(off)
     if (d_nm <= 0.f) return 0.f;
     const CalcTerms t = calc_terms(n_film, d_nm, cos_i, lambda);
     const float cyc = t.opd / lambda;
@@ -199,11 +201,23 @@ __global__ void calc_kernel(CalcArgs a) {
     } else if (a.mode == 3) {
         if (k >= a.count) return;
         a.out[k] = calc_intensity(a.n_film, a.d_nm[k], a.cos_i[k], a.lambda[k]);
-    } else {
+    } else if (a.mode == 4) {
         if (k >= a.count) return;
         const CalcTerms t = calc_terms(a.n_film, a.d_nm[k], a.cos_i[k], a.lambda[k]);
         float* o = a.out + 6 * static_cast<size_t>(k);
         o[0] = t.cos_t; o[1] = t.opd; o[2] = t.phase; o[3] = t.rs; o[4] = t.rp; o[5] = t.ravg;
+    } else if (a.mode == 6) {
+        // calculate_phase_difference (interference.rs:248-254) of a given OPD.
+        if (k >= a.count) return;
+        a.out[k] = 2.f * kPiF * a.d_nm[k] / a.lambda[k];
+    } else {
+        // Optical path difference and phase for a given transmission cosine
+        // (interference.rs:226-254).
+        if (k >= a.count) return;
+        const float lam = a.lambda[k];
+        const float opd = 2.f * a.n_film * a.d_nm[k] * a.cos_i[k] + lam / 2.f;
+        a.out[2 * static_cast<size_t>(k)] = opd;
+        a.out[2 * static_cast<size_t>(k) + 1] = 2.f * kPiF * opd / lam;
     }
 }
 
@@ -511,6 +525,32 @@ extern "C" int sf_interference_terms(double refractive_index, int32_t n, const f
     });
 }
 
+extern "C" int sf_optical_path(double refractive_index, int32_t n, const float* d_nm,
+                               const float* cos_t, const float* lambda_nm, float* out) {
+    return guard([&] {
+        SF_REQUIRE(n >= 0 && (n == 0 || (d_nm && cos_t && lambda_nm && out)), SF_ERR_INVALID,
+                   "bad arguments");
+        if (n == 0) return;
+        CalcArgs a = calc_base(refractive_index, nullptr);
+        a.mode = 5;
+        a.count = n;
+        run_calc(a, 2 * static_cast<size_t>(n), out, n, d_nm, n, cos_t, n, lambda_nm, n);
+    });
+}
+
+extern "C" int sf_phase_difference(int32_t n, const float* opd_nm, const float* lambda_nm,
+                                   float* out) {
+    return guard([&] {
+        SF_REQUIRE(n >= 0 && (n == 0 || (opd_nm && lambda_nm && out)), SF_ERR_INVALID,
+                   "bad arguments");
+        if (n == 0) return;
+        CalcArgs a = calc_base(1.0, nullptr);
+        a.mode = 6;
+        a.count = n;
+        run_calc(a, static_cast<size_t>(n), out, n, opd_nm, n, nullptr, 0, lambda_nm, n);
+    });
+}
+
 // Packs cmf[81][3] into the MFMA B-operand layout (see spectral_kernel),
 // split into truncated-BF16 hi and lo parts.
 static std::vector<uint2> pack_cmf(const float* cmf) {
@@ -611,6 +651,22 @@ extern "C" int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double re
         if (rgb_host) {
             SF_HIP(hipMemcpyAsync(rgb_host, d->rgb, 3 * d->n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
             SF_HIP(hipStreamSynchronize(d->stream));
+        }
+    });
+}
+
+extern "C" int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, double dt,
+                                         const sf_camera* cam, double refractive_index,
+                                         const float* cmf) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && n_steps >= 0, SF_ERR_INVALID, "bad arguments");
+        SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
+        SF_REQUIRE(std::isfinite(dt), SF_ERR_INVALID, "dt must be finite");
+        Drainage* d = reinterpret_cast<Drainage*>(ctx);
+        if (n_steps > 0) dt = d->resolve_dt(dt);
+        for (int s = 0; s < n_steps; ++s) {
+            d->step(1, dt);
+            spectral_launch(d, cam, refractive_index, s == 0 ? cmf : nullptr);
         }
     });
 }

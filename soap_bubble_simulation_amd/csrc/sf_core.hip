@@ -1,5 +1,6 @@
 // sf_core.hip -- error state, device probing and version of the C ABI.
 #include <cstdarg>
+#include <cstddef>
 #include <cstring>
 
 #include "sf_common.hpp"
@@ -28,6 +29,60 @@ void require_device() {
 }
 
 }  // namespace sf
+
+namespace sf {
+
+// Plain float4 copy: the reference kernel for achievable HBM bandwidth and
+// for scaling the FETCH_SIZE / WRITE_SIZE counters of the other kernels.
+__global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src,
+                                                          float4* __restrict__ dst, size_t n4) {
+    for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n4;
+         k += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[k] = src[k];
+}
+
+}  // namespace sf
+
+static_assert(sizeof(sf_params) == 16 * 8 + 2 * 4, "sf_params layout");
+static_assert(offsetof(sf_params, flags) == 128, "sf_params layout");
+static_assert(sizeof(sf_stats) == 5 * 8 + 8 + 4 + 4 + 8, "sf_stats layout");
+static_assert(sizeof(sf_camera) == 24, "sf_camera layout");
+
+extern "C" int sf_copy_probe(size_t n_bytes, int32_t iters, double* avg_ms) {
+    return sf::guard([&] {
+        SF_REQUIRE(n_bytes >= 16 && iters >= 1, SF_ERR_INVALID, "bad arguments");
+        sf::require_device();
+        const size_t n4 = n_bytes / 16;
+        void *a = nullptr, *b = nullptr;
+        SF_HIP(hipMalloc(&a, n4 * 16));
+        hipError_t e = hipMalloc(&b, n4 * 16);
+        if (e != hipSuccess) {
+            (void)hipFree(a);
+            SF_HIP(e);
+        }
+        hipEvent_t t0, t1;
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        (void)hipMemset(a, 0, n4 * 16);
+        const unsigned blocks = 256 * 8;
+        hipLaunchKernelGGL(sf::copy_probe_kernel, dim3(blocks), dim3(256), 0, nullptr,
+                           static_cast<const float4*>(a), static_cast<float4*>(b), n4);
+        (void)hipEventRecord(t0, nullptr);
+        for (int k = 0; k < iters; ++k)
+            hipLaunchKernelGGL(sf::copy_probe_kernel, dim3(blocks), dim3(256), 0, nullptr,
+                               static_cast<const float4*>(a), static_cast<float4*>(b), n4);
+        (void)hipEventRecord(t1, nullptr);
+        e = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        SF_HIP(e);
+        if (avg_ms) *avg_ms = static_cast<double>(ms) / iters;
+    });
+}
 
 extern "C" const char* sf_last_error(void) { return sf::last_error(); }
 

@@ -1,0 +1,94 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point declared in include/soapfilm.h, the ctypes mirrors match the C struct
+layouts, defaults match the reference, and compute entry points refuse to
+run (SF_ERR_DEVICE) when no GPU is present instead of falling back to CPU."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from soap_bubble_simulation_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "soapfilm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(sf_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_symbols_exported():
+    names = declared_functions()
+    assert len(names) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    # and the ctypes binding declares every one of them
+    assert not [n for n in names if n not in L.SIGNATURES]
+
+
+def test_struct_layouts():
+    assert C.sizeof(L.Params) == 136
+    assert L.Params.flags.offset == 128
+    assert C.sizeof(L.Stats) == 64
+    assert C.sizeof(L.Camera) == 24
+
+
+def test_reference_defaults():
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    assert (p.gravity, p.viscosity, p.density) == (9.81, 0.001, 1000.0)      # config.rs:58-95
+    assert p.diffusion_coeff == 1e-9 and p.bubble_radius == 0.025
+    assert p.critical_thickness == 10.0 * 1e-9                               # drainage.rs:278
+    assert p.tfe_thickness_ratio == 0.85 and p.tfe_first_spawn == 0.5       # drainage.rs:283-286
+    assert p.tfe_spawn_interval == 2.0 and p.sin_guard == 1e-10
+    assert p.flags == L.FLAG_TFE
+    g = L.Params()
+    L.check(L.lib().sf_params_gpu_default(g))
+    assert g.dt == 0.001 and g.critical_thickness == 30e-9                  # gpu_drainage.rs:49-55
+    assert g.sin_guard == 1e-4 and g.gamma_air == 0.072 and g.marangoni_coeff == 0.01
+    assert g.flags == L.FLAG_SURFACTANT | L.FLAG_POLES_COPY_H
+
+
+@pytest.mark.skipif(L.device_available(), reason="checks the no-GPU behaviour")
+def test_no_cpu_fallback():
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    h = C.c_void_p()
+    rc = L.lib().sf_drainage_create(C.byref(p), 16, 32, 5e-7, 0.5, C.byref(h))
+    assert rc == L.SF_ERR_DEVICE and not h.value
+    assert "no HIP device" in L.last_error()
+    import numpy as np
+    out = np.zeros(64 * 256 * 4, np.uint8)
+    assert L.lib().sf_lut_rgba8(1.33, 1.0, L.u8ptr(out)) == L.SF_ERR_DEVICE
+    with pytest.raises(L.SoapFilmError):
+        from soap_bubble_simulation_amd.drainage import DrainageSimulator
+        DrainageSimulator()
+
+
+def test_argument_validation():
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    h = C.c_void_p()
+    assert L.lib().sf_drainage_create(C.byref(p), 2, 32, 5e-7, 0.5, C.byref(h)) == L.SF_ERR_INVALID
+    assert "n_theta >= 3" in L.last_error()
+    bad = p.copy()
+    bad.bubble_radius = -1.0
+    assert L.lib().sf_drainage_create(C.byref(bad), 16, 32, 5e-7, 0.5, C.byref(h)) == L.SF_ERR_INVALID
+    assert L.lib().sf_drainage_step(None, 1, 0.1) == L.SF_ERR_INVALID
+    assert L.lib().sf_lut_rgba8(1.33, 1.0, None) == L.SF_ERR_INVALID
+
+
+def test_cpp_api_compiles():
+    """The C++ mirror header compiles against the C header."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_api.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -95,7 +95,8 @@ def test_marangoni_surfactant_parity(oracle):
     nt, npf = 96, 192
     h0 = W.noise_field(nt, npf, seed=5, rel=0.05)
     rng = np.random.default_rng(6)
-    g0 = np.clip(0.5 + 0.2 * W.sh_noise(nt, npf, 6, seed=9), 0, 1).astype(np.float32).astype(np.float64)
+    g0 = np.clip(0.5 + 0.2 * W.sh_noise(nt, npf, 6, seed=9) + 0.05 * rng.standard_normal((nt, npf)),
+                 0, 1).astype(np.float32).astype(np.float64)
     p = L.Params()
     L.check(L.lib().sf_params_gpu_default(p))
     p.flags = L.FLAG_SURFACTANT | L.FLAG_MARANGONI | L.FLAG_POLES_COPY_H
@@ -109,8 +110,10 @@ def test_marangoni_surfactant_parity(oracle):
     ho, go = orc.fields()
     assert rel_l2(hg, ho) <= 1e-6
     assert rel_l2(hg - h0, ho - h0) <= 1e-5
+    print("marangoni: rel_l2(h)=%.3g rel_l2(dh)=%.3g max|dG|=%.3g max|G-G0|=%.3g" % (
+        rel_l2(hg, ho), rel_l2(hg - h0, ho - h0), np.max(np.abs(gg - go)), np.max(np.abs(gg - g0))))
     assert np.max(np.abs(gg - go)) <= 3e-6
-    assert np.max(np.abs(gg - g0)) > 1e-5      # Γ actually evolved
+    assert np.max(np.abs(gg - g0)) > 1e-3      # Γ actually evolved
 
 
 def test_pole_ring_mean(oracle):
