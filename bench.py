@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the float4 copy kernel (counter calibration)")
+    ap.add_argument("--copy-bytes", type=int, default=256 << 20)
+    ap.add_argument("--flags", type=int, default=None,
+                    help="override the SF_FLAG_* bits of the workload")
     return ap.parse_args()
 
 
@@ -117,8 +120,8 @@ def main():
         print("no GPU visible: bench.py needs an MI355X", file=sys.stderr)
         sys.exit(1)
     if args.copy_probe:
-        ms, gbs = copy_probe(L)
-        print(json.dumps({"copy_probe_ms": ms, "copy_probe_GBps": gbs, "bytes": 2 * (256 << 20)}))
+        ms, gbs = copy_probe(L, args.copy_bytes)
+        print(json.dumps({"copy_probe_ms": ms, "copy_probe_GBps": gbs, "bytes": 2 * args.copy_bytes}))
         return
 
     spectral = args.workload == "c2"
@@ -133,6 +136,8 @@ def main():
         flags = L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG
         desc = ("single bubble 2048x4096 FP32 drainage, CFL dt"
                 + (" + 81-wavelength spectral RGB per step (BF16 MFMA)" if spectral else ""))
+    if args.flags is not None:
+        flags = args.flags
     p = L.Params()
     L.check(L.lib().sf_params_cpu_default(p))
     p.flags = flags

@@ -4,6 +4,7 @@
 // GPUDrainageSimulator (src/render/gpu_drainage.rs).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -123,6 +124,18 @@ __global__ void tfe_mask_kernel(StepArgs a, uint8_t* mask) {
     mask[static_cast<size_t>(i) * a.np + j] = in_fronts(a, i, j) ? 1 : 0;
 }
 
+// Reduces the last step's per-block FP64 partials into its history slot.
+__global__ __launch_bounds__(256) void diag_finalize_kernel(const double2* part, int nblocks,
+                                                            const float2* hpole, int ntiles,
+                                                            int pole_sum, double pole_area_top,
+                                                            double pole_area_bot,
+                                                            double pole_mass_fixed, double2* hist,
+                                                            int slot) {
+    __shared__ double red_d[2][kBlockMax / 64];
+    diag_reduce(part, nblocks, hpole, ntiles, pole_sum, pole_area_top, pole_area_bot,
+                pole_mass_fixed, hist, slot, red_d);
+}
+
 __global__ void fill_kernel(float* p, size_t n, float v) {
     const size_t c = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (c < n) p[c] = v;
@@ -158,25 +171,37 @@ static void launch_step(dim3 grid, dim3 block, hipStream_t s, const StepArgs& a)
     hipLaunchKernelGGL((drainage_step_kernel<RB, GM, M, D>), grid, block, 0, s, a);
 }
 
-// Table indexed by [rebase][gmode][marangoni][diag].
-static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag) {
+template <bool RB, int GM, bool M, bool D, int ROWS>
+static void launch_vec4(dim3 grid, dim3 block, hipStream_t s, const StepArgs& a) {
+    hipLaunchKernelGGL((drainage_vec4_kernel<RB, GM, M, D, ROWS, kPrefetchRows>), grid, block, 0, s, a);
+}
+
+// Kernel table indexed by [rebase][gmode][marangoni][diag].  The scalar
+// kernel serves any n_phi; the float4 kernel needs n_phi % 4 == 0 and comes
+// in row-band heights of 4 and 8.
+static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag, bool vec4, int rows) {
 #define SF_E(RB, GM, M, D) &launch_step<RB, GM, M, D>
-    static const StepFn table[2][3][2][2] = {
-        {{{SF_E(false, 0, false, false), SF_E(false, 0, false, true)},
-          {SF_E(false, 0, true, false), SF_E(false, 0, true, true)}},
-         {{SF_E(false, 1, false, false), SF_E(false, 1, false, true)},
-          {SF_E(false, 1, true, false), SF_E(false, 1, true, true)}},
-         {{SF_E(false, 2, false, false), SF_E(false, 2, false, true)},
-          {SF_E(false, 2, true, false), SF_E(false, 2, true, true)}}},
-        {{{SF_E(true, 0, false, false), SF_E(true, 0, false, true)},
-          {SF_E(true, 0, true, false), SF_E(true, 0, true, true)}},
-         {{SF_E(true, 1, false, false), SF_E(true, 1, false, true)},
-          {SF_E(true, 1, true, false), SF_E(true, 1, true, true)}},
-         {{SF_E(true, 2, false, false), SF_E(true, 2, false, true)},
-          {SF_E(true, 2, true, false), SF_E(true, 2, true, true)}}},
-    };
+#define SF_V(RB, GM, M, D, R) &launch_vec4<RB, GM, M, D, R>
+#define SF_ROW(E, RB, GM)                                                   \
+    {{E(RB, GM, false, false), E(RB, GM, false, true)},                     \
+     {E(RB, GM, true, false), E(RB, GM, true, true)}}
+#define SF_TAB(E)                                                           \
+    {{SF_ROW(E, false, 0), SF_ROW(E, false, 1), SF_ROW(E, false, 2)},       \
+     {SF_ROW(E, true, 0), SF_ROW(E, true, 1), SF_ROW(E, true, 2)}}
+#define SF_V4(RB, GM, M, D) SF_V(RB, GM, M, D, 4)
+#define SF_V8(RB, GM, M, D) SF_V(RB, GM, M, D, 8)
+    static const StepFn scalar[2][3][2][2] = SF_TAB(SF_E);
+    static const StepFn v4[2][3][2][2] = SF_TAB(SF_V4);
+    static const StepFn v8[2][3][2][2] = SF_TAB(SF_V8);
 #undef SF_E
-    return table[rebase][gmode][marangoni][diag];
+#undef SF_V
+#undef SF_ROW
+#undef SF_TAB
+#undef SF_V4
+#undef SF_V8
+    if (!vec4) return scalar[rebase][gmode][marangoni][diag];
+    if (rows <= 4) return v4[rebase][gmode][marangoni][diag];
+    return v8[rebase][gmode][marangoni][diag];
 }
 
 }  // namespace sf
@@ -192,10 +217,14 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     dphi = 2.0 * kPi / static_cast<double>(np);
     SF_HIP(hipGetDevice(&device));
     SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    wx = static_cast<int>(std::min<size_t>(4, cdiv(np, 64)));
+    vec4 = (np % 4 == 0) && std::getenv("SOAPFILM_SCALAR_STENCIL") == nullptr;
+    const int cols_per_wave = vec4 ? 256 : 64;
+    wx = static_cast<int>(std::min<size_t>(4, cdiv(np, cols_per_wave)));
     block = 64 * wx;
-    ntiles = static_cast<int>(cdiv(np, block));
-    rows_per_block = 16;
+    ntiles = static_cast<int>(cdiv(np, static_cast<size_t>(cols_per_wave) * wx));
+    rows_per_block = vec4 ? 8 : 16;
+    if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK")) rows_per_block = std::max(1, std::atoi(e));
+    if (vec4) rows_per_block = rows_per_block <= 4 ? 4 : 8;
     for (int k = 0; k < 2; ++k) {
         hi[k] = dalloc<float>(n);
         lo[k] = dalloc<float>(n);
@@ -206,9 +235,8 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     rows = dalloc<RowF>(nt);
     rowd = dalloc<RowD>(nt);
     nblocks_max = static_cast<size_t>(ntiles) * cdiv(nt, 1);
-    diag_part = dalloc<double2>(nblocks_max);
-    ticket = dalloc<unsigned int>(1);
-    SF_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned int), stream));
+    diag_part[0] = dalloc<double2>(nblocks_max);
+    diag_part[1] = dalloc<double2>(nblocks_max);
     diag_hist = dalloc<double2>(kHistCap);
     stats_part = dalloc<StatsPart>(kStatsBlocks);
     stats_out = dalloc<StatsPart>(1);
@@ -220,7 +248,7 @@ Drainage::~Drainage() {
     for (int k = 0; k < 2; ++k) {
         dfree(hi[k]); dfree(lo[k]); dfree(g[k]); dfree(hpole[k]); dfree(gpole[k]);
     }
-    dfree(rows); dfree(rowd); dfree(diag_part); dfree(ticket); dfree(diag_hist);
+    dfree(rows); dfree(rowd); dfree(diag_part[0]); dfree(diag_part[1]); dfree(diag_hist);
     dfree(stats_part); dfree(stats_out); dfree(gather_idx); dfree(gather_out);
     dfree(rgb); dfree(cmf_dev); dfree(spec_rows); dfree(spec_cols);
     step_timer.destroy();
@@ -438,12 +466,10 @@ void Drainage::fill_common_args(StepArgs& a, float dt_f) const {
     a.pole_area_top = pole_area_top;
     a.pole_area_bot = pole_area_bot;
     a.pole_mass_fixed = pole_mass_fixed;
-    a.diag_part = diag_part;
-    a.ticket = ticket;
     a.diag_hist = diag_hist;
 }
 
-void Drainage::materialize() {
+void Drainage::materialize_poles() {
     if (!poles_virtual) return;
     const bool do_h = !(p.flags & SF_FLAG_POLES_COPY_H);
     const bool do_g = (p.flags & SF_FLAG_POLES_AVG_GAMMA) && (p.flags & SF_FLAG_SURFACTANT);
@@ -452,6 +478,22 @@ void Drainage::materialize() {
                        ntiles, do_h ? 1 : 0, do_g ? 1 : 0);
     SF_HIP(hipGetLastError());
     poles_virtual = false;
+}
+
+// Reduces the per-block FP64 partials of the last diagnosed step, which no
+// later launch has consumed yet.
+void Drainage::finalize_diag() {
+    if (pending_slot < 0) return;
+    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(256), 0, stream, diag_part[pending_part],
+                       pending_nblocks, hpole[cur_pole], ntiles, pending_pole_sum ? 1 : 0,
+                       pole_area_top, pole_area_bot, pole_mass_fixed, diag_hist, pending_slot);
+    SF_HIP(hipGetLastError());
+    pending_slot = -1;
+}
+
+void Drainage::materialize() {
+    finalize_diag();
+    materialize_poles();
 }
 
 double Drainage::resolve_dt(double dt) {
@@ -463,7 +505,7 @@ double Drainage::resolve_dt(double dt) {
     return p.cfl_safety * cfl;
 }
 
-void Drainage::step(int n_steps, double dt) {
+void Drainage::step(int n_steps, double dt, bool flush) {
     if (tables_dirty) upload_tables();
     dt = resolve_dt(dt);
     const bool marangoni = (p.flags & SF_FLAG_MARANGONI) != 0;
@@ -497,8 +539,17 @@ void Drainage::step(int n_steps, double dt) {
         a.gpole_in = gpole[cur_pole];
         a.hpole_out = hpole[1 - cur_pole];
         a.gpole_out = gpole[1 - cur_pole];
-        a.hist_slot = static_cast<int>(hist_count % kHistCap);
-        const StepFn fn = step_fn(rebase, gmode, marangoni, diag);
+        if (diag) {
+            a.diag_part = diag_part[1 - pending_part];
+            a.diag_prev = diag_part[pending_part];
+            a.hpole_prev = a.hpole_in;
+            a.prev_slot = pending_slot;
+            a.prev_nblocks = pending_nblocks;
+            a.prev_pole_sum = pending_pole_sum ? 1 : 0;
+        } else {
+            finalize_diag();
+        }
+        const StepFn fn = step_fn(rebase, gmode, marangoni, diag, vec4, rows_per_block);
         if (step_timer.enabled) {
             auto& ev = step_timer.next(stream);
             SF_HIP(hipEventRecord(ev.first, stream));
@@ -520,11 +571,17 @@ void Drainage::step(int n_steps, double dt) {
             cur_pole = 1 - cur_pole;
             poles_virtual = true;
         }
-        if (diag) ++hist_count;
+        if (diag) {
+            pending_part = 1 - pending_part;
+            pending_slot = static_cast<int>(hist_count % kHistCap);
+            pending_nblocks = static_cast<int>(grid.x * grid.y);
+            pending_pole_sum = h_avg;
+            ++hist_count;
+        }
         current_time += dt;
         ++steps_taken;
     }
-    materialize();
+    if (flush) materialize();
 }
 
 std::pair<hipEvent_t, hipEvent_t>& KernelTimer::next(hipStream_t s) {
@@ -956,6 +1013,7 @@ extern "C" int sf_drainage_history(sf_drainage* ctx, int32_t n, double* mass, do
     return guard([&] {
         SF_REQUIRE(ctx != nullptr && n >= 0, SF_ERR_INVALID, "bad arguments");
         Drainage* d = D(ctx);
+        d->finalize_diag();
         const int64_t avail = std::min<int64_t>(d->hist_count, Drainage::kHistCap);
         const int64_t m = std::min<int64_t>(avail, n);
         std::vector<double2> all(Drainage::kHistCap);

@@ -47,6 +47,7 @@ struct Drainage {
     hipStream_t stream = nullptr;
 
     // launch geometry
+    bool vec4 = false;   // float4 kernel (n_phi % 4 == 0)
     int wx = 1, block = 64, ntiles = 1, rows_per_block = 16;
 
     // device state (see drainage_kernels.hpp for the layout)
@@ -66,10 +67,13 @@ struct Drainage {
 
     // FP64 diagnostics
     size_t nblocks_max = 0;
-    double2* diag_part = nullptr;
-    unsigned int* ticket = nullptr;
+    double2* diag_part[2] = {nullptr, nullptr};   // per-block partials, ping-pong by step
     double2* diag_hist = nullptr;
     int64_t hist_count = 0;
+    int pending_part = 0;         // diag_part half written by the last diagnosed step
+    int pending_slot = -1;        // its history slot while not yet reduced
+    int pending_nblocks = 0;
+    bool pending_pole_sum = false;
     StatsPart* stats_part = nullptr;
     StatsPart* stats_out = nullptr;
 
@@ -108,9 +112,11 @@ struct Drainage {
     void spawn_front();
     void schedule_fronts(double dt, StepArgs& a);
     void fill_common_args(StepArgs& a, float dt_f) const;
+    void materialize_poles();
+    void finalize_diag();
     void materialize();
     double resolve_dt(double dt);
-    void step(int n_steps, double dt);
+    void step(int n_steps, double dt, bool flush = true);
     StatsPart compute_stats();
     void ensure_gather(size_t count);
 };

@@ -608,7 +608,7 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
         SF_HIP(hipStreamSynchronize(d->stream));
     }
     SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
-    d->materialize();
+    d->materialize_poles();
     SpecArgs a;
     a.hi = d->hi[d->cur_hi];
     a.lo = d->lo[d->cur];
@@ -665,9 +665,10 @@ extern "C" int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, doub
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
         if (n_steps > 0) dt = d->resolve_dt(dt);
         for (int s = 0; s < n_steps; ++s) {
-            d->step(1, dt);
+            d->step(1, dt, false);
             spectral_launch(d, cam, refractive_index, s == 0 ? cmf : nullptr);
         }
+        d->finalize_diag();
     });
 }
 

@@ -124,12 +124,23 @@ inline Float3 split3(double v) {
     return r;
 }
 
-// True when hi + lo < c0 + c1 + c2 holds exactly.
-__host__ __device__ inline bool ff_less(float hi, float lo, float c0, float c1, float c2) {
-    float approx = hi + lo;
-    const float margin = 0x1.0p-10f;
-    if (approx > c0 + fabsf(c0) * margin) return false;
-    if (approx < c0 - fabsf(c0) * margin) return true;
+// Exact test hi + lo < c0 + c1 + c2, kept out of line: it only runs when
+// hi + lo lies within 2^-10 of c0.
+static __device__ __noinline__ bool ff_less_exact(float hi, float lo, float c0, float c1, float c2) {
+    return exact_sign5(hi, lo, -c0, -c1, -c2) < 0;
+}
+
+// True when hi + lo < c0 + c1 + c2 holds exactly.  The fast path decides
+// whenever the rounded sum is further than 2^-10 c0 from c0 (its rounding
+// error is at most 2^-24 of the sum).
+__device__ inline bool ff_less(float hi, float lo, float c0, float c1, float c2) {
+    const float approx = hi + lo;
+    bool r = approx < c0;
+    if (fabsf(approx - c0) <= fabsf(c0) * 0x1.0p-10f) r = ff_less_exact(hi, lo, c0, c1, c2);
+    return r;
+}
+
+inline bool ff_less_host(float hi, float lo, float c0, float c1, float c2) {
     return exact_sign5(hi, lo, -c0, -c1, -c2) < 0;
 }
 

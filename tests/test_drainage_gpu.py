@@ -65,6 +65,33 @@ def test_single_step_parity_default_grid(oracle):
     assert rel_l2(hg - h0, ho - h0) <= 1e-5, rel_l2(hg - h0, ho - h0)
 
 
+@pytest.mark.parametrize("nt,npf", [(64, 128), (40, 96), (37, 74), (9, 6), (3, 4), (20, 1030),
+                                     (130, 2052)])
+def test_grid_shapes_parity(oracle, nt, npf):
+    """Scalar (n_phi % 4 != 0) and float4 kernels, partial waves and tiles,
+    tiny grids with both rings in one row band."""
+    h0 = W.noise_field(nt, npf, seed=nt * 1000 + npf, rel=0.05)
+    p = cpu_params(flags=L.FLAG_SURFACTANT | L.FLAG_DIAG)
+    g0 = W.ff_exact(np.clip(0.5 + 0.05 * np.random.default_rng(1).standard_normal((nt, npf)), 0, 1))
+    g0 = g0.astype(np.float32).astype(np.float64)
+    ctx, orc = pair(oracle, p, nt, npf, h0, g0)
+    dt = 0.5 * orc.cfl_dt()
+    for k in range(5):
+        ctx.step(1, dt)
+        orc.step(dt)
+    hg, gg = ctx.fields()
+    ho, go = orc.fields()
+    assert rel_l2(hg, ho) <= 1e-6
+    assert rel_l2(hg - h0, ho - h0) <= 1e-5
+    assert np.max(np.abs(gg - go)) <= 1e-5
+    # The fused FP64 film-volume reduction equals the oracle's FP64 volume of
+    # the GPU's own field (same data, different summation order).
+    m, _ = ctx.history(5)
+    same = oracle.Sim(to_orc_params(oracle, p), nt, npf)
+    same.set_fields(hg)
+    assert abs(m[-1] - same.mass()) <= 1e-12 * same.mass()
+
+
 def test_rupture_mask_bit_exact(oracle):
     nt, npf = 64, 128
     rng = np.random.default_rng(3)
