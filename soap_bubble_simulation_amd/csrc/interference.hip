@@ -263,6 +263,10 @@ __device__ inline void split_pack(float a, float b, uint32_t& hi2, uint32_t& lo2
     lo2 = (bf16_hi_bits(al) >> 16) | bf16_hi_bits(bl);
 }
 
+__device__ inline float lane_read(float v, int src_lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+}
+
 __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 15;
@@ -282,17 +286,22 @@ __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
                     static_cast<short>(l2.y & 0xFFFF), static_cast<short>(l2.y >> 16)};
         for (int v = 0; v < 4; ++v) {
             const int k = 16 * t + 4 * q + v;
-            u[t][v] = k < kSpecLambdas ? 1.f / (2.f * (380.f + 5.f * static_cast<float>(k))) : 0.f;
+            u[t][v] = k < kSpecLambdas ? 0.5f / (380.f + 5.f * static_cast<float>(k)) : 0.f;
         }
     }
     const float scale = 1.f / static_cast<float>(kSpecLambdas);
+    const float inv_np = 1.f / static_cast<float>(a.np);
 
-    for (size_t grp = wave_global; grp * 16 < a.n; grp += nwaves) {
-        const size_t cell = grp * 16 + m;
-        const bool valid = cell < a.n;
-        const size_t cc = valid ? cell : a.n - 1;
-        const int i = static_cast<int>(cc / a.np);
-        const int j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
+    // Each wave takes 64 consecutive cells at a time: every lane prepares
+    // one cell (view angle, Fresnel, finesse, optical path), then the four
+    // 16-cell MFMA groups fetch their cell's values from the owning lane.
+    for (size_t sg = wave_global; sg * 64 < a.n; sg += nwaves) {
+        const size_t cell = sg * 64 + lane;
+        const size_t cc = cell < a.n ? cell : a.n - 1;
+        int i = static_cast<int>(static_cast<float>(cc) * inv_np);
+        int j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
+        if (j < 0) { --i; j += a.np; }
+        if (j >= a.np) { ++i; j -= a.np; }
         const float4 rw = a.rows[i];
         const float2 cl = a.cols[j];
         // View angle |dot(normalize(cam - p), n)|, p = R n (bubble.wgsl:491-494).
@@ -304,49 +313,55 @@ __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
         const float sin_i = sqrtf(fmaxf(1.f - cos_i * cos_i, 0.f));
         const float sin_t = sin_i * a.inv_n_film;
         const float cos_t = sqrtf(fmaxf(1.f - sin_t * sin_t, 0.f));
-        const float rs = (cos_i - a.n_film * cos_t) / fmaxf(cos_i + a.n_film * cos_t, 1e-4f);
-        const float rp = (a.n_film * cos_i - cos_t) / fmaxf(a.n_film * cos_i + cos_t, 1e-4f);
+        const float rs = (cos_i - a.n_film * cos_t) * __builtin_amdgcn_rcpf(fmaxf(cos_i + a.n_film * cos_t, 1e-4f));
+        const float rp = (a.n_film * cos_i - cos_t) * __builtin_amdgcn_rcpf(fmaxf(a.n_film * cos_i + cos_t, 1e-4f));
         const float R = (rs * rs + rp * rp) * 0.5f;
         const float omr = fmaxf(1.f - R, 1e-3f);
-        const float F = 4.f * R / (omr * omr);
+        const float Fc = 4.f * R * __builtin_amdgcn_rcpf(omr * omr);
         const float d_nm = (a.hi[cc] + a.lo[cc]) * 1e9f;
-        const float op = 2.f * a.n_film * d_nm * cos_t;
+        const float opc = 2.f * a.n_film * d_nm * cos_t;
 
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < kSpecChunks; ++t) {
-            float I[4];
-            const int nv = (t == kSpecChunks - 1) ? 1 : 4;   // slots 84..95 are padding
+        for (int g = 0; g < 4; ++g) {
+            const size_t grp = sg * 4 + g;
+            if (grp * 16 >= a.n) break;
+            const float F = lane_read(Fc, 16 * g + m);
+            const float op = lane_read(opc, 16 * g + m);
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                if (v < nv) {
-                    // sin(phase / 2) with phase = 2 pi op / lambda + pi is
-                    // cos(pi op / lambda) = cos(2 pi x), x = op / (2 lambda).
-                    const float x = op * u[t][v];
-                    const float xr = x - rintf(x);
-                    const float c = __builtin_amdgcn_cosf(xr);
-                    const float fs2 = F * (c * c);
-                    I[v] = fs2 * __builtin_amdgcn_rcpf(1.f + fs2);
-                } else {
-                    I[v] = 0.f;
+            for (int t = 0; t < kSpecChunks; ++t) {
+                float I[4];
+                const int nv = (t == kSpecChunks - 1) ? 1 : 4;   // slots 84..95 are padding
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (v < nv) {
+                        // sin(phase / 2) with phase = 2 pi op / lambda + pi is
+                        // cos(pi op / lambda) = cos(2 pi x), x = op / (2 lambda);
+                        // v_cos_f32 takes its argument in revolutions.
+                        const float c = __builtin_amdgcn_cosf(op * u[t][v]);
+                        const float fs2 = F * (c * c);
+                        I[v] = fs2 * __builtin_amdgcn_rcpf(1.f + fs2);
+                    } else {
+                        I[v] = 0.f;
+                    }
                 }
+                uint32_t h01, l01, h23, l23;
+                split_pack(I[0], I[1], h01, l01);
+                split_pack(I[2], I[3], h23, l23);
+                const v4s ah = v4s{static_cast<short>(h01 & 0xFFFF), static_cast<short>(h01 >> 16),
+                                   static_cast<short>(h23 & 0xFFFF), static_cast<short>(h23 >> 16)};
+                const v4s al = v4s{static_cast<short>(l01 & 0xFFFF), static_cast<short>(l01 >> 16),
+                                   static_cast<short>(l23 & 0xFFFF), static_cast<short>(l23 >> 16)};
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[t], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[t], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[t], acc, 0, 0, 0);
             }
-            uint32_t h01, l01, h23, l23;
-            split_pack(I[0], I[1], h01, l01);
-            split_pack(I[2], I[3], h23, l23);
-            const v4s ah = v4s{static_cast<short>(h01 & 0xFFFF), static_cast<short>(h01 >> 16),
-                               static_cast<short>(h23 & 0xFFFF), static_cast<short>(h23 >> 16)};
-            const v4s al = v4s{static_cast<short>(l01 & 0xFFFF), static_cast<short>(l01 >> 16),
-                               static_cast<short>(l23 & 0xFFFF), static_cast<short>(l23 >> 16)};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[t], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[t], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[t], acc, 0, 0, 0);
-        }
-        // acc[v] = D[4 q + v][m]: cell grp*16 + 4q + v, colour m.
-        if (m < 3) {
-            for (int v = 0; v < 4; ++v) {
-                const size_t oc = grp * 16 + 4 * q + v;
-                if (oc < a.n) a.rgb[static_cast<size_t>(m) * a.n + oc] = acc[v] * scale;
+            // acc[v] = D[4 q + v][m]: cell grp*16 + 4q + v, colour m.
+            if (m < 3) {
+                for (int v = 0; v < 4; ++v) {
+                    const size_t oc = grp * 16 + 4 * q + v;
+                    if (oc < a.n) a.rgb[static_cast<size_t>(m) * a.n + oc] = acc[v] * scale;
+                }
             }
         }
     }
@@ -627,8 +642,8 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.radius = static_cast<float>(d->p.bubble_radius);
     a.n_film = static_cast<float>(n_film);
     a.inv_n_film = static_cast<float>(1.0 / n_film);
-    const size_t groups = (d->n + 15) / 16;
-    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((groups + 3) / 4, 256 * 16));
+    const size_t supergroups = (d->n + 63) / 64;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 20));
     if (d->spec_timer.enabled) {
         auto& ev = d->spec_timer.next(d->stream);
         SF_HIP(hipEventRecord(ev.first, d->stream));
