@@ -14,7 +14,10 @@ MI355X HBM peak (16 B per cell-step: h and surfactant read + write in FP32),
 timed with HIP events on the context's stream; ``cpu_baseline`` is the FP64
 oracle on a bounded sample of the same workload on the host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c2-drainage|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c2-drainage|c1|c3]
+
+``--workload c3`` is the batched foam (4096 bubbles of uneven grids, one
+launch per step, 16-step HIP graphs, tile shapes autotuned per size bucket).
 """
 from __future__ import annotations
 
@@ -41,7 +44,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3"])
+    ap.add_argument("--bubbles", type=int, default=4096, help="foam size of the c3 workload")
+    ap.add_argument("--no-autotune", action="store_true", help="c3: keep the default tile shapes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -105,6 +110,86 @@ def cpu_baseline(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s
             "sample": sample + ", 1 thread, extrapolated per cell-step"}
 
 
+def foam_cpu_baseline(O, params, fm, h, budget_s):
+    """FP64 oracle stepping randomly drawn bubbles of the foam (their own
+    grids, radii, viscosities and dt) on one host thread."""
+    rng = np.random.default_rng(1)
+    dts = fm.bubble_dt()
+    t0 = time.perf_counter()
+    cell_steps = 0
+    t_steps = 0.0
+    picked = 0
+    while time.perf_counter() - t0 < budget_s and picked < 64:
+        k = int(rng.integers(len(fm.bubbles)))
+        b = fm.bubbles[k]
+        q = O.Params()
+        for name, _ in O.Params._fields_:
+            setattr(q, name, getattr(params, name))
+        q.bubble_radius, q.viscosity = b.bubble_radius, b.viscosity
+        sim = O.Sim(q, b.n_theta, b.n_phi)
+        sim.set_fields(fm.split(h)[k], np.full((b.n_theta, b.n_phi), 0.5))
+        n = 20
+        t1 = time.perf_counter()
+        sim.step(float(dts[k]), n)
+        t_steps += time.perf_counter() - t1
+        cell_steps += n * b.n_theta * b.n_phi
+        picked += 1
+    return {"value": cell_steps / t_steps, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle, 20 steps each of %d randomly drawn foam bubbles (%d cell-steps), "
+                      "1 thread" % (picked, cell_steps)}
+
+
+def bench_foam(args, L, W):
+    """Batched foam: every bubble advances one step of its own dt per step."""
+    from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_params
+
+    desc = W.foam_batch(args.bubbles, seed=5)
+    bubbles = [FoamBubble(**d) for d in desc]
+    p = foam_params()
+    fm = FoamDrainage(p, bubbles)
+    sizes = [b.n_theta * b.n_phi for b in bubbles]
+    h = W.foam_noise(fm.offsets, sizes, [b.h0 for b in bubbles], seed=6, dtype=np.float32)
+    fm.set_fields_f32(h)
+    chosen = None if args.no_autotune else fm.autotune(trial_steps=5)
+    fm.step(max(1, args.warmup))
+    fm.synchronize()
+    fm.kernel_timing(True)
+    fm.step(min(args.steps, 200))
+    k_ms, k_n = fm.kernel_timing(False)
+    fm.synchronize()
+    t0 = time.perf_counter()
+    fm.step(args.steps)
+    fm.synchronize()
+    elapsed = time.perf_counter() - t0
+    info = fm.info()
+    cells = info["cells"]
+    st = fm.stats()
+    achieved = BYTES_PER_CELL_STEP * cells / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": cells * args.steps / elapsed, "unit": "cell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "foam of %d bubbles, n_theta ~ logU[32,512], n_phi = 2 n_theta, "
+                               "diameter ~ logU[5,50] mm, h ~ U[200,1000] nm +-5 %%, viscosity ~ "
+                               "U[1,5] mPa s, per-bubble CFL dt, Γ transport, 16-step HIP graphs"
+                               % len(bubbles),
+                   "bubbles": len(bubbles), "cells": cells, "items_per_step": info["items"],
+                   "tile_shapes": info["variants"], "autotuned": chosen is not None},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "foam_step_kernel",
+                     "avg_launch_ms": k_ms, "launches_timed": k_n,
+                     "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells},
+        "state": {"mass_m3_total": float(st["mass"].sum()), "min_h": float(st["min"].min()),
+                  "max_h": float(st["max"].max()),
+                  "finite": bool(np.isfinite(st["mass"]).all())},
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+        out["cpu_baseline"] = foam_cpu_baseline(O, p, fm, h.astype(np.float64), args.cpu_seconds)
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -124,6 +209,9 @@ def main():
         print(json.dumps({"copy_probe_ms": ms, "copy_probe_GBps": gbs, "bytes": 2 * args.copy_bytes}))
         return
 
+    if args.workload == "c3":
+        bench_foam(args, L, W)
+        return
     spectral = args.workload == "c2"
     if args.workload == "c1":
         n_theta, n_phi = 128, 256

@@ -254,6 +254,70 @@ int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb);
 int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
                               int64_t* launches);
 
+/* ---- batched foam --------------------------------------------------------
+ * Many bubbles, each with its own grid, radius, viscosity and time step,
+ * advanced together in one launch per step (a 16-step HIP graph is replayed
+ * for long runs).  The reference keeps a scalar thickness per foam bubble
+ * (foam.rs:32-49); a foam context gives every bubble the drainage field of
+ * DrainageSimulator::step (drainage.rs:298-392) with the surfactant
+ * transport of drainage.wgsl:178-201: h poles are ring means
+ * (drainage.rs:480-503), Γ poles stay fixed, no Marangoni term, no fronts.
+ * params->flags may hold SF_FLAG_SURFACTANT only; bubble_radius and
+ * viscosity of params are ignored (each bubble brings its own).
+ *
+ * Layout: the fields of all bubbles are concatenated; bubble b occupies
+ * [offsets[b], offsets[b] + n_theta*n_phi) of an array of offsets[n]
+ * elements (sf_foam_layout), row-major [theta][phi] as for one bubble. */
+typedef struct sf_bubble {
+    int32_t n_theta;        /* >= 3                                        */
+    int32_t n_phi;          /* even, >= 2                                  */
+    double bubble_radius;   /* m                                           */
+    double viscosity;       /* Pa s                                        */
+    double h0;              /* initial uniform thickness, m                */
+    double dt;              /* s per step; <= 0: CFL step (cfl_safety times
+                               the FP64 stability limit of the initial
+                               state, fixed for the bubble's lifetime)     */
+} sf_bubble;
+
+typedef struct sf_foam sf_foam;
+
+/* Creates the foam on the current device with uniform h0 and Γ = 0.5. */
+int sf_foam_create(const sf_params* params, int32_t n_bubbles, const sf_bubble* bubbles, sf_foam** out);
+void sf_foam_destroy(sf_foam* foam);
+/* offsets[n_bubbles + 1]: start of each bubble in the concatenated fields;
+ * the last entry is the array length. */
+int sf_foam_layout(sf_foam* foam, int64_t* offsets);
+/* Replaces h (and Γ, unless NULL: Γ = 0.5) from FP32 arrays of the layout
+ * length; CFL time steps are re-resolved. */
+int sf_foam_set_fields(sf_foam* foam, const float* h, const float* gamma);
+/* As sf_foam_set_fields from FP64 (h kept to ~48 bits); NULL keeps a field. */
+int sf_foam_set_fields_f64(sf_foam* foam, const double* h, const double* gamma);
+/* Copies h (FP64 value of the float-float pair) and Γ out; NULL skips. */
+int sf_foam_copy_fields_f64(sf_foam* foam, double* h, double* gamma);
+/* Enqueues n_steps steps of every bubble, each with its own dt. */
+int sf_foam_step(sf_foam* foam, int32_t n_steps);
+int sf_foam_synchronize(sf_foam* foam);
+/* Replays 16-step HIP graphs for long step counts (default on). */
+int sf_foam_use_graph(sf_foam* foam, int32_t enable);
+/* Times every tile shape (1 or 2 columns per lane x 4 or 8 rows per wave
+ * item) on each bubble-size bucket (n_phi <= 96, <= 192, <= 384, > 384) for
+ * trial_steps steps on a copy of the state, keeps the fastest, and restores
+ * the state.  chosen[4] (may be NULL) receives the shape index per bucket:
+ * 0 = (1,4), 1 = (1,8), 2 = (2,4), 3 = (2,8). */
+int sf_foam_autotune(sf_foam* foam, int32_t trial_steps, int32_t* chosen);
+/* Sets the tile shape index (0-3, as above) of each of the 4 buckets. */
+int sf_foam_set_tile_shapes(sf_foam* foam, const int32_t* shapes);
+/* dt[n_bubbles]: each bubble's time step. */
+int sf_foam_bubble_dt(sf_foam* foam, double* dt);
+/* Per-bubble FP64 film volume, CFL limit, min and max of h (arrays of
+ * n_bubbles; NULL skips). */
+int sf_foam_stats(sf_foam* foam, double* mass, double* dt_cfl, double* hmin, double* hmax);
+/* Average step-kernel duration in ms since the last call (HIP events,
+ * graphs off while enabled); enable switches recording on/off. */
+int sf_foam_kernel_timing(sf_foam* foam, int32_t enable, double* avg_ms, int64_t* launches);
+/* Total cells, wave work items per step, and the tile shape per bucket. */
+int sf_foam_info(sf_foam* foam, int64_t* total_cells, int32_t* n_items, int32_t* variants);
+
 #ifdef __cplusplus
 }
 #endif

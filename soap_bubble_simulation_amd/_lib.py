@@ -80,6 +80,18 @@ class Camera(C.Structure):
     _fields_ = [("position", C.c_double * 3)]
 
 
+class Bubble(C.Structure):
+    """Mirror of ``sf_bubble`` (one bubble of a foam)."""
+    _fields_ = [
+        ("n_theta", C.c_int32),
+        ("n_phi", C.c_int32),
+        ("bubble_radius", C.c_double),
+        ("viscosity", C.c_double),
+        ("h0", C.c_double),
+        ("dt", C.c_double),
+    ]
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _u8p = C.POINTER(C.c_uint8)
@@ -133,6 +145,21 @@ SIGNATURES = {
     "sf_drainage_synchronize": (C.c_int, [_vp]),
     "sf_spectral_rgb": (C.c_int, [_vp, C.POINTER(Camera), C.c_double, _fp, _fp]),
     "sf_spectral_device_rgb": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sf_foam_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(Bubble), C.POINTER(_vp)]),
+    "sf_foam_destroy": (None, [_vp]),
+    "sf_foam_layout": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "sf_foam_set_fields": (C.c_int, [_vp, _fp, _fp]),
+    "sf_foam_set_fields_f64": (C.c_int, [_vp, _dp, _dp]),
+    "sf_foam_copy_fields_f64": (C.c_int, [_vp, _dp, _dp]),
+    "sf_foam_step": (C.c_int, [_vp, C.c_int32]),
+    "sf_foam_synchronize": (C.c_int, [_vp]),
+    "sf_foam_use_graph": (C.c_int, [_vp, C.c_int32]),
+    "sf_foam_autotune": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),
+    "sf_foam_set_tile_shapes": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "sf_foam_bubble_dt": (C.c_int, [_vp, _dp]),
+    "sf_foam_stats": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "sf_foam_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "sf_foam_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 }
 
 _lib = None

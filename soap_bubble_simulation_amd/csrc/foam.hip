@@ -1,0 +1,995 @@
+// foam.hip -- batched drainage of a foam of bubbles with uneven grids.
+//
+// Every bubble is its own latitude-longitude grid (n_theta x n_phi, any
+// even n_phi) with its own radius, viscosity and time step; all of them
+// advance together in one persistent launch per step.  The state of all
+// bubbles lives in shared concatenated arrays (same float-float h + FP32 Γ
+// layout as a single bubble, see drainage_kernels.hpp), so one launch
+// streams the whole foam.
+//
+// Work is cut into wave-sized items: one item is (bubble, band of RB rows,
+// chunk of 64 * CPL columns) and one wave processes it.  The item shape per
+// bubble (CPL in {1, 2} columns per lane, RB in {4, 8} rows) is chosen per
+// size bucket by the autotuner in this file.  Items are ordered largest
+// first and dealt round-robin to a grid sized to the GPU, so every wave
+// loops over its items and leaves when they are done.  A run of 16 steps
+// (one full period of the ping-pong and rebase buffers) is captured in a
+// HIP graph and replayed.
+//
+// Physics per cell is the drainage step of drainage.rs:298-392 with the
+// surfactant transport of drainage.wgsl:178-201: h poles are ring means, Γ
+// poles are copied through, no marginal-regeneration fronts (foam bubbles
+// have no per-bubble front schedule in the reference: foam.rs:32-49 keeps a
+// scalar thickness per bubble).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "drainage.hpp"
+
+namespace sf {
+
+struct FoamBubble {
+    int64_t off;          // first cell of the bubble in the concatenated arrays
+    int nt, np;
+    int row_off;          // first row in the row-coefficient table
+    int pole_off;         // first ring partial: [nchunks top][nchunks bottom]
+    int nchunks;          // column chunks of the bubble's item shape
+    int variant;          // item shape: 0 (1,4) 1 (1,8) 2 (2,4) 3 (2,8) as (CPL, RB)
+    float dt;             // s
+    float dtD;            // dt D_h / R^2
+    float dtDs;           // dt D_s / R^2
+    float dtc;            // dt rho g / (3 eta)
+    float G0;             // 1 / dtheta^2
+    float np_f;
+    float hc0, hc1, hc2;  // critical thickness, three-float expansion
+};
+
+struct FoamItem {
+    int bubble, r0, c0, chunk;
+};
+
+// Per-row coefficients of one bubble: sin(theta), cot/(2 dtheta),
+// 1/(sin^2 dphi^2).
+struct FoamRow {
+    float sin_t, G1, G2, pad;
+};
+
+struct FoamArgs {
+    const float* hi_in;
+    const float* lo_in;
+    const float* g_in;
+    float* hi_out;
+    float* lo_out;
+    float* g_out;
+    const FoamRow* rows;
+    const FoamBubble* bub;
+    const FoamItem* items;
+    int n_items;
+    const float2* hpole_in;
+    float2* hpole_out;
+    int pole_mean;        // ring neighbours of the poles from hpole_in
+};
+
+constexpr int kVariantCpl[4] = {1, 1, 2, 2};
+constexpr int kVariantRows[4] = {4, 8, 4, 8};
+
+template <int CPL>
+struct FRow {
+    float h[CPL], l[CPL], g[CPL];
+    float hw, lw, gw, he, le, ge;
+};
+
+template <int CPL>
+__device__ inline void foam_load(const FoamArgs& a, int64_t off, int np, int nt, int r, int cc,
+                                 int jw, int je, FRow<CPL>& R) {
+    const int64_t base = off + static_cast<int64_t>(min(r, nt - 1)) * np;
+    if (CPL == 2) {
+        const float2 h = *reinterpret_cast<const float2*>(a.hi_in + base + cc);
+        const float2 l = *reinterpret_cast<const float2*>(a.lo_in + base + cc);
+        const float2 g = *reinterpret_cast<const float2*>(a.g_in + base + cc);
+        R.h[0] = h.x; R.h[CPL - 1] = h.y;
+        R.l[0] = l.x; R.l[CPL - 1] = l.y;
+        R.g[0] = g.x; R.g[CPL - 1] = g.y;
+    } else {
+        R.h[0] = a.hi_in[base + cc];
+        R.l[0] = a.lo_in[base + cc];
+        R.g[0] = a.g_in[base + cc];
+    }
+    R.hw = a.hi_in[base + jw];
+    R.lw = a.lo_in[base + jw];
+    R.gw = a.g_in[base + jw];
+    R.he = a.hi_in[base + je];
+    R.le = a.lo_in[base + je];
+    R.ge = a.g_in[base + je];
+}
+
+template <int CPL>
+__device__ inline void foam_pole_row(FRow<CPL>& R, float2 P) {
+#pragma unroll
+    for (int v = 0; v < CPL; ++v) {
+        R.h[v] = P.x;
+        R.l[v] = P.y;
+    }
+    R.hw = R.he = P.x;
+    R.lw = R.le = P.y;
+}
+
+template <int CPL, int RB, bool REBASE>
+__device__ void foam_item(const FoamArgs& a, const FoamItem it, const FoamBubble& B, int lane) {
+    constexpr int PF = 2;
+    const int np = B.np, nt = B.nt;
+    const int c = it.c0 + CPL * lane;
+    const bool valid = c < np;
+    const int cc = valid ? c : np - CPL;
+    const int jw = cc == 0 ? np - 1 : cc - 1;
+    const int je = cc + CPL >= np ? cc + CPL - np : cc + CPL;
+    const int r0 = it.r0;
+    const int r1 = min(r0 + RB, nt - 1);
+    const bool touches_top = r0 == 1, touches_bot = r1 == nt - 1;
+
+    // Lane t < RB holds the coefficients of row r0 + t; rows read them with
+    // readlane, so the loop never waits on a vector load of its own.
+    const FoamRow rl = a.rows[B.row_off + min(r0 + (lane % RB), nt - 1)];
+
+    float2 ptop = make_float2(0.f, 0.f), pbot = make_float2(0.f, 0.f);
+    if (a.pole_mean) {
+        if (touches_top) ptop = pole_mean(a.hpole_in + B.pole_off, B.nchunks, B.np_f);
+        if (touches_bot) pbot = pole_mean(a.hpole_in + B.pole_off + B.nchunks, B.nchunks, B.np_f);
+    }
+
+    FRow<CPL> S[RB + 2];
+#pragma unroll
+    for (int k = 0; k < PF + 2 && k < RB + 2; ++k) foam_load<CPL>(a, B.off, np, nt, r0 - 1 + k, cc, jw, je, S[k]);
+
+    float top_h[CPL], top_l[CPL], bot_h[CPL], bot_l[CPL];
+#pragma unroll
+    for (int v = 0; v < CPL; ++v) top_h[v] = top_l[v] = bot_h[v] = bot_l[v] = 0.f;
+
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+        if (t + PF + 2 < RB + 2) foam_load<CPL>(a, B.off, np, nt, r0 + t + PF + 1, cc, jw, je, S[t + PF + 2]);
+        const int i = r0 + t;
+        if (i < r1) {
+            FRow<CPL> N = S[t];
+            const FRow<CPL> C = S[t + 1];
+            FRow<CPL> Sx = S[t + 2];
+            if (a.pole_mean && i == 1) foam_pole_row<CPL>(N, ptop);
+            if (a.pole_mean && i == nt - 2) foam_pole_row<CPL>(Sx, pbot);
+            const float sin_t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl.sin_t), t));
+            const float G1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl.G1), t));
+            const float G2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl.G2), t));
+            const float cS = B.dtc * sin_t;
+
+            float wh[CPL], wl[CPL], wg[CPL], eh[CPL], el[CPL], eg[CPL];
+            wh[0] = C.hw; wl[0] = C.lw; wg[0] = C.gw;
+            eh[CPL - 1] = C.he; el[CPL - 1] = C.le; eg[CPL - 1] = C.ge;
+            if (CPL == 2) {
+                wh[CPL - 1] = C.h[0]; wl[CPL - 1] = C.l[0]; wg[CPL - 1] = C.g[0];
+                eh[0] = C.h[CPL - 1]; el[0] = C.l[CPL - 1]; eg[0] = C.g[CPL - 1];
+            }
+            float lo_new[CPL], g_new[CPL];
+#pragma unroll
+            for (int v = 0; v < CPL; ++v) {
+                const float ch = C.h[v], cl = C.l[v], cg = C.g[v];
+                const bool rupt = ff_less(ch, cl, B.hc0, B.hc1, B.hc2);
+                const float dtp = (Sx.h[v] - ch) + (Sx.l[v] - cl);
+                const float dtm = (ch - N.h[v]) + (cl - N.l[v]);
+                const float dpp = (eh[v] - ch) + (el[v] - cl);
+                const float dpm = (ch - wh[v]) + (cl - wl[v]);
+                const float lap = (dtp - dtm) * B.G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
+                const float H = ch + cl;
+                const float ln = fmaxf(cl + fmaf(B.dtD, lap, -cS * (H * H * H)), -ch);
+                lo_new[v] = rupt ? cl : ln;
+                const float gtp = Sx.g[v] - cg, gtm = cg - N.g[v];
+                const float gpp = eg[v] - cg, gpm = cg - wg[v];
+                const float glap = (gtp - gtm) * B.G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
+                const float gn = fminf(fmaxf(fmaf(B.dtDs, glap, cg), 0.f), 1.f);
+                g_new[v] = rupt ? cg : gn;
+            }
+            if (i == 1) {
+#pragma unroll
+                for (int v = 0; v < CPL; ++v) { top_h[v] = C.h[v]; top_l[v] = lo_new[v]; }
+            }
+            if (i == nt - 2) {
+#pragma unroll
+                for (int v = 0; v < CPL; ++v) { bot_h[v] = C.h[v]; bot_l[v] = lo_new[v]; }
+            }
+            if (valid) {
+                const int64_t o = B.off + static_cast<int64_t>(i) * np + cc;
+                if (CPL == 2) {
+                    if (REBASE) {
+                        float2 hh, ll;
+                        two_sum(C.h[0], lo_new[0], hh.x, ll.x);
+                        two_sum(C.h[CPL - 1], lo_new[CPL - 1], hh.y, ll.y);
+                        *reinterpret_cast<float2*>(a.hi_out + o) = hh;
+                        *reinterpret_cast<float2*>(a.lo_out + o) = ll;
+                    } else {
+                        *reinterpret_cast<float2*>(a.lo_out + o) = make_float2(lo_new[0], lo_new[CPL - 1]);
+                    }
+                    *reinterpret_cast<float2*>(a.g_out + o) = make_float2(g_new[0], g_new[CPL - 1]);
+                } else {
+                    if (REBASE) {
+                        float hh, ll;
+                        two_sum(C.h[0], lo_new[0], hh, ll);
+                        a.hi_out[o] = hh;
+                        a.lo_out[o] = ll;
+                    } else {
+                        a.lo_out[o] = lo_new[0];
+                    }
+                    a.g_out[o] = g_new[0];
+                }
+            }
+        }
+    }
+
+    // Ring partials of this chunk for the pole means of the next step.
+    if (touches_top || touches_bot) {
+        float2 st = make_float2(0.f, 0.f), sb = make_float2(0.f, 0.f);
+        if (valid) {
+#pragma unroll
+            for (int v = 0; v < CPL; ++v) {
+                float s_, e_;
+                two_sum(top_h[v], top_l[v], s_, e_);
+                st = ff_add(st, make_float2(s_, e_));
+                two_sum(bot_h[v], bot_l[v], s_, e_);
+                sb = ff_add(sb, make_float2(s_, e_));
+            }
+        }
+        st = wave_ff_sum(st);
+        sb = wave_ff_sum(sb);
+        if (lane == 0) {
+            if (touches_top) a.hpole_out[B.pole_off + it.chunk] = st;
+            if (touches_bot) a.hpole_out[B.pole_off + B.nchunks + it.chunk] = sb;
+        }
+    }
+}
+
+template <bool REBASE>
+__global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a) {
+    const int lane = threadIdx.x & 63;
+    // The item index is wave-uniform; readfirstlane lets the compiler keep it
+    // (and the item and bubble records) in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int nw = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+    for (int k = wave; k < a.n_items; k += nw) {
+        const FoamItem it = a.items[k];
+        const FoamBubble B = a.bub[it.bubble];
+        switch (B.variant) {
+            case 0: foam_item<1, 4, REBASE>(a, it, B, lane); break;
+            case 1: foam_item<1, 8, REBASE>(a, it, B, lane); break;
+            case 2: foam_item<2, 4, REBASE>(a, it, B, lane); break;
+            default: foam_item<2, 8, REBASE>(a, it, B, lane); break;
+        }
+    }
+}
+
+// Writes every bubble's pole rows from the ring partials (one block per bubble).
+__global__ void foam_poles_kernel(float* hi, float* lo, const float2* hpole, const FoamBubble* bub) {
+    const FoamBubble B = bub[blockIdx.x];
+    const float2 t = pole_mean(hpole + B.pole_off, B.nchunks, B.np_f);
+    const float2 b = pole_mean(hpole + B.pole_off + B.nchunks, B.nchunks, B.np_f);
+    for (int j = threadIdx.x; j < B.np; j += blockDim.x) {
+        hi[B.off + j] = t.x;
+        lo[B.off + j] = t.y;
+        const int64_t o = B.off + static_cast<int64_t>(B.nt - 1) * B.np + j;
+        hi[o] = b.x;
+        lo[o] = b.y;
+    }
+}
+
+// Per-bubble FP64 statistics: film volume, CFL lambda, min and max
+// (one block per bubble).
+struct FoamStat {
+    double mass, lam, mn, mx;
+};
+
+__global__ __launch_bounds__(256) void foam_stats_kernel(const float* hi, const float* lo,
+                                                          const FoamBubble* bub, const double* area,
+                                                          const double* lamg, const double* sinab,
+                                                          const double* coef3, FoamStat* out) {
+    const FoamBubble B = bub[blockIdx.x];
+    const int n = B.nt * B.np;
+    double m = 0.0, l = 0.0, mn = INFINITY, mx = -INFINITY;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int i = c / B.np;
+        const double v = static_cast<double>(hi[B.off + c]) + static_cast<double>(lo[B.off + c]);
+        m += v * area[B.row_off + i];
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+        if (i > 0 && i < B.nt - 1) l = fmax(l, lamg[B.row_off + i] + coef3[blockIdx.x] * sinab[B.row_off + i] * v * v);
+    }
+    __shared__ double s[4][4];
+    m = wave_dsum(m);
+    l = wave_dmax(l);
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_down(mn, off));
+        mx = fmax(mx, __shfl_down(mx, off));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s[0][wave] = m; s[1][wave] = l; s[2][wave] = mn; s[3][wave] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        FoamStat r{s[0][0], s[1][0], s[2][0], s[3][0]};
+        for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+            r.mass += s[0][w];
+            r.lam = fmax(r.lam, s[1][w]);
+            r.mn = fmin(r.mn, s[2][w]);
+            r.mx = fmax(r.mx, s[3][w]);
+        }
+        out[blockIdx.x] = r;
+    }
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kFoamPeriod = 16;          // lo/Γ/pole ping-pong x rebase every 8 steps
+
+template <class T>
+T* falloc(size_t count) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
+    if (e != hipSuccess) {
+        set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        throw Error{SF_ERR_NOMEM};
+    }
+    return static_cast<T*>(p);
+}
+
+unsigned cdivu(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+int bucket_of(int np) { return np <= 96 ? 0 : (np <= 192 ? 1 : (np <= 384 ? 2 : 3)); }
+
+}  // namespace
+
+struct Foam {
+    sf_params p;
+    int nb = 0;
+    std::vector<sf_bubble> desc;
+    std::vector<FoamBubble> bub;
+    std::vector<int64_t> offsets;
+    int64_t total = 0;
+    int total_rows = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    int blocks_per_cu = 4;              // resident 256-thread blocks of the step kernel
+
+    float* hi[2] = {nullptr, nullptr};
+    float* lo[2] = {nullptr, nullptr};
+    float* g[2] = {nullptr, nullptr};
+    int cur = 0, cur_hi = 0, cur_pole = 0, since_rebase = 0;
+    bool poles_virtual = false;
+    float2* hpole[2] = {nullptr, nullptr};
+    int pole_slots = 0;
+    FoamRow* rows = nullptr;
+    FoamBubble* d_bub = nullptr;
+    FoamItem* d_items = nullptr;
+    int n_items = 0;
+    int items_cap = 0;
+    double* d_area = nullptr;
+    double* d_lamg = nullptr;
+    double* d_sinab = nullptr;
+    double* d_coef3 = nullptr;
+    FoamStat* d_stat = nullptr;
+    int variant_of_bucket[4] = {1, 1, 3, 3};
+    std::vector<double> time;
+    int64_t steps_taken = 0;
+
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;
+    int graph_base = -1;                  // ping-pong phase the graph was captured at
+    KernelTimer timer;
+
+    ~Foam() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (int k = 0; k < 2; ++k) {
+            (void)hipFree(hi[k]); (void)hipFree(lo[k]); (void)hipFree(g[k]); (void)hipFree(hpole[k]);
+        }
+        (void)hipFree(rows); (void)hipFree(d_bub); (void)hipFree(d_items); (void)hipFree(d_area);
+        (void)hipFree(d_lamg); (void)hipFree(d_sinab); (void)hipFree(d_coef3); (void)hipFree(d_stat);
+        timer.destroy();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Item shapes and the pole-partial layout follow from each bubble's variant.
+    void layout_items() {
+        SF_HIP(hipStreamSynchronize(stream));     // buffers below may still be in use
+        std::vector<FoamItem> items;
+        pole_slots = 0;
+        for (int b = 0; b < nb; ++b) {
+            FoamBubble& B = bub[b];
+            B.variant = variant_of_bucket[bucket_of(B.np)];
+            const int cpl = kVariantCpl[B.variant], rb = kVariantRows[B.variant];
+            B.nchunks = static_cast<int>(cdivu(B.np, 64 * cpl));
+            B.pole_off = pole_slots;
+            pole_slots += 2 * B.nchunks;
+            for (int r0 = 1; r0 < B.nt - 1; r0 += rb)
+                for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, ch * 64 * cpl, ch});
+        }
+        // Largest bubbles first, so that the round-robin deal ends evenly.
+        std::stable_sort(items.begin(), items.end(), [&](const FoamItem& x, const FoamItem& y) {
+            return bub[x.bubble].np > bub[y.bubble].np;
+        });
+        n_items = static_cast<int>(items.size());
+        if (n_items > items_cap) {
+            (void)hipFree(d_items);
+            d_items = falloc<FoamItem>(n_items);
+            items_cap = n_items;
+        }
+        SF_HIP(hipMemcpyAsync(d_items, items.data(), n_items * sizeof(FoamItem), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipMemcpyAsync(d_bub, bub.data(), nb * sizeof(FoamBubble), hipMemcpyHostToDevice, stream));
+        for (int k = 0; k < 2; ++k) {
+            (void)hipFree(hpole[k]);
+            hpole[k] = falloc<float2>(std::max(pole_slots, 1));
+        }
+        SF_HIP(hipStreamSynchronize(stream));
+        drop_graph();
+    }
+
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_base = -1;
+    }
+
+    // Every block of the persistent grid is resident at once (blocks that
+    // waited for a free slot would start their item lists late).
+    unsigned grid_blocks() const {
+        const unsigned resident = static_cast<unsigned>(n_cu) * static_cast<unsigned>(blocks_per_cu);
+        return std::max(1u, std::min(resident, cdivu(static_cast<size_t>(n_items), 4)));
+    }
+
+    FoamArgs args_for(int cur_, int cur_hi_, int cur_pole_, bool pole_mean) const {
+        FoamArgs a;
+        a.hi_in = hi[cur_hi_];
+        a.lo_in = lo[cur_];
+        a.g_in = g[cur_];
+        a.hi_out = hi[1 - cur_hi_];
+        a.lo_out = lo[1 - cur_];
+        a.g_out = g[1 - cur_];
+        a.rows = rows;
+        a.bub = d_bub;
+        a.items = d_items;
+        a.n_items = n_items;
+        a.hpole_in = hpole[cur_pole_];
+        a.hpole_out = hpole[1 - cur_pole_];
+        a.pole_mean = pole_mean ? 1 : 0;
+        return a;
+    }
+
+    void launch_one(bool rebase, const FoamArgs& a) {
+        const dim3 grid(grid_blocks()), blk(256);
+        if (rebase)
+            hipLaunchKernelGGL(foam_step_kernel<true>, grid, blk, 0, stream, a);
+        else
+            hipLaunchKernelGGL(foam_step_kernel<false>, grid, blk, 0, stream, a);
+    }
+
+    // One step with the current buffers; advances the ping-pong state.
+    void step_direct() {
+        const bool rebase = since_rebase + 1 >= 8;
+        const FoamArgs a = args_for(cur, cur_hi, cur_pole, poles_virtual);
+        if (timer.enabled) {
+            auto& ev = timer.next(stream);
+            SF_HIP(hipEventRecord(ev.first, stream));
+            launch_one(rebase, a);
+            SF_HIP(hipEventRecord(ev.second, stream));
+        } else {
+            launch_one(rebase, a);
+        }
+        SF_HIP(hipGetLastError());
+        advance(rebase);
+    }
+
+    void advance(bool rebase) {
+        cur = 1 - cur;
+        cur_pole = 1 - cur_pole;
+        if (rebase) {
+            cur_hi = 1 - cur_hi;
+            since_rebase = 0;
+        } else {
+            ++since_rebase;
+        }
+        poles_virtual = true;
+        ++steps_taken;
+        for (int b = 0; b < nb; ++b) time[b] += static_cast<double>(bub[b].dt);
+    }
+
+    int phase() const { return (cur_hi << 4) | (cur << 3) | since_rebase; }
+
+    // Captures kFoamPeriod steps from the current phase into a graph.
+    void capture() {
+        drop_graph();
+        hipGraph_t gr = nullptr;
+        SF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int c = cur, ch = cur_hi, cp = cur_pole, sr = since_rebase;
+        for (int s = 0; s < kFoamPeriod; ++s) {
+            const bool rebase = sr + 1 >= 8;
+            launch_one(rebase, args_for(c, ch, cp, true));
+            c = 1 - c;
+            cp = 1 - cp;
+            if (rebase) { ch = 1 - ch; sr = 0; } else { ++sr; }
+        }
+        hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (e == hipSuccess) e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+        if (gr) (void)hipGraphDestroy(gr);
+        SF_HIP(e);
+        graph_base = phase();
+    }
+
+    void step(int n) {
+        int done = 0;
+        if (!poles_virtual && done < n) {   // first step reads the stored pole rows
+            step_direct();
+            ++done;
+        }
+        if (use_graph && !timer.enabled) {
+            while (n - done >= kFoamPeriod) {
+                if (graph == nullptr || graph_base != phase()) capture();
+                SF_HIP(hipGraphLaunch(graph, stream));
+                for (int s = 0; s < kFoamPeriod; ++s) advance(since_rebase + 1 >= 8);
+                done += kFoamPeriod;
+            }
+        }
+        while (done < n) {
+            step_direct();
+            ++done;
+        }
+    }
+
+    void materialize() {
+        if (!poles_virtual) return;
+        hipLaunchKernelGGL(foam_poles_kernel, dim3(nb), dim3(256), 0, stream, hi[cur_hi], lo[cur],
+                           hpole[cur_pole], d_bub);
+        SF_HIP(hipGetLastError());
+        poles_virtual = false;
+    }
+
+    std::vector<FoamStat> stats() {
+        materialize();
+        hipLaunchKernelGGL(foam_stats_kernel, dim3(nb), dim3(256), 0, stream, hi[cur_hi], lo[cur], d_bub,
+                           d_area, d_lamg, d_sinab, d_coef3, d_stat);
+        SF_HIP(hipGetLastError());
+        std::vector<FoamStat> s(nb);
+        SF_HIP(hipMemcpyAsync(s.data(), d_stat, nb * sizeof(FoamStat), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        return s;
+    }
+
+    void set_dt(const std::vector<double>& dts) {
+#pragma clang fp contract(off)
+        for (int b = 0; b < nb; ++b) {
+            const sf_bubble& d = desc[b];
+            const double R = d.bubble_radius;
+            const double dt = dts[b];
+            FoamBubble& B = bub[b];
+            B.dt = static_cast<float>(dt);
+            B.dtD = static_cast<float>(dt * p.diffusion_coeff / (R * R));
+            B.dtDs = (p.flags & SF_FLAG_SURFACTANT) ? static_cast<float>(dt * p.surfactant_diffusion / (R * R)) : 0.f;
+            B.dtc = static_cast<float>(dt * p.density * p.gravity / (3.0 * d.viscosity));
+        }
+        SF_HIP(hipMemcpyAsync(d_bub, bub.data(), nb * sizeof(FoamBubble), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        drop_graph();
+    }
+
+    std::vector<double> bubble_dt() const {
+        std::vector<double> r(nb);
+        for (int b = 0; b < nb; ++b) r[b] = static_cast<double>(bub[b].dt);
+        return r;
+    }
+
+    // dt of bubbles whose description asks for the CFL step (dt <= 0) from
+    // the FP64 statistics kernel.
+    void resolve_cfl() {
+        bool any = false;
+        for (const auto& d : desc) any |= !(d.dt > 0.0);
+        std::vector<double> dts(nb);
+        std::vector<FoamStat> st;
+        if (any) st = stats();
+        for (int b = 0; b < nb; ++b)
+            dts[b] = desc[b].dt > 0.0 ? desc[b].dt : p.cfl_safety * (st[b].lam > 0.0 ? 2.0 / st[b].lam : 0.0);
+        for (int b = 0; b < nb; ++b)
+            SF_REQUIRE(dts[b] > 0.0 && std::isfinite(dts[b]), SF_ERR_INVALID, "bubble %d has no finite dt", b);
+        set_dt(dts);
+    }
+
+    void upload(const float* h_hi, const float* h_lo, const float* gam) {
+        for (int k = 0; k < 2; ++k) {
+            if (h_hi) {
+                SF_HIP(hipMemcpyAsync(hi[k], h_hi, total * sizeof(float), hipMemcpyHostToDevice, stream));
+                SF_HIP(hipMemcpyAsync(lo[k], h_lo, total * sizeof(float), hipMemcpyHostToDevice, stream));
+            }
+            if (gam) SF_HIP(hipMemcpyAsync(g[k], gam, total * sizeof(float), hipMemcpyHostToDevice, stream));
+        }
+        SF_HIP(hipStreamSynchronize(stream));
+        cur = cur_hi = cur_pole = since_rebase = 0;
+        poles_virtual = false;
+        drop_graph();
+    }
+
+    // Saves and restores the device state around autotuning trials.
+    struct Snapshot {
+        std::vector<float*> buf;
+    };
+    Snapshot snapshot() {
+        Snapshot s;
+        float* src[6] = {hi[0], hi[1], lo[0], lo[1], g[0], g[1]};
+        for (float* p_ : src) {
+            float* d = falloc<float>(total);
+            SF_HIP(hipMemcpyAsync(d, p_, total * sizeof(float), hipMemcpyDeviceToDevice, stream));
+            s.buf.push_back(d);
+        }
+        SF_HIP(hipStreamSynchronize(stream));
+        return s;
+    }
+    void restore(Snapshot& s) {
+        float* dst[6] = {hi[0], hi[1], lo[0], lo[1], g[0], g[1]};
+        for (int k = 0; k < 6; ++k) {
+            SF_HIP(hipMemcpyAsync(dst[k], s.buf[k], total * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        }
+        SF_HIP(hipStreamSynchronize(stream));
+        for (float* p_ : s.buf) (void)hipFree(p_);
+        s.buf.clear();
+    }
+
+    // Times every item shape on each size bucket and keeps the fastest.
+    void autotune(int trial_steps, int32_t* chosen) {
+        materialize();                  // the state is then whole in the current buffers
+        SF_HIP(hipStreamSynchronize(stream));
+        Snapshot snap = snapshot();
+        const int c0 = cur, ch0 = cur_hi, cp0 = cur_pole, sr0 = since_rebase;
+        const bool pv0 = poles_virtual;
+        const int64_t st0 = steps_taken;
+        const std::vector<double> time0 = time;
+        std::vector<FoamBubble> keep = bub;
+        const bool g0 = use_graph;
+        use_graph = false;
+        timer.drain(stream);            // the caller's own timing survives the trials
+        const double t_ms = timer.ms;
+        const int64_t t_launches = timer.launches;
+        const bool t_enabled = timer.enabled;
+        for (int bk = 0; bk < 4; ++bk) {
+            bool present = false;
+            for (const auto& B : bub) present |= bucket_of(B.np) == bk;
+            if (!present) continue;
+            double best = INFINITY;
+            int best_v = variant_of_bucket[bk];
+            for (int v = 0; v < 4; ++v) {
+                // Only this bucket's bubbles take part in the trial.
+                std::vector<FoamBubble> saved = bub;
+                int save[4];
+                std::copy(variant_of_bucket, variant_of_bucket + 4, save);
+                variant_of_bucket[bk] = v;
+                layout_items_subset(bk);
+                poles_virtual = false;      // first trial step reads the stored pole rows
+                step_direct();              // untimed warm-up
+                timer.enabled = true;
+                timer.ms = 0.0;
+                timer.launches = 0;
+                for (int s = 0; s < trial_steps; ++s) step_direct();
+                timer.drain(stream);
+                const double ms = timer.launches ? timer.ms / timer.launches : INFINITY;
+                timer.enabled = false;
+                if (ms < best) {
+                    best = ms;
+                    best_v = v;
+                }
+                std::copy(save, save + 4, variant_of_bucket);
+                bub = saved;
+            }
+            variant_of_bucket[bk] = best_v;
+        }
+        bub = keep;
+        use_graph = g0;
+        timer.ms = t_ms;
+        timer.launches = t_launches;
+        timer.enabled = t_enabled;
+        restore(snap);
+        cur = c0; cur_hi = ch0; cur_pole = cp0; since_rebase = sr0; poles_virtual = pv0;
+        steps_taken = st0;
+        time = time0;
+        layout_items();
+        if (chosen)
+            for (int k = 0; k < 4; ++k) chosen[k] = variant_of_bucket[k];
+    }
+
+    // Item list restricted to one size bucket (autotuning trials).
+    void layout_items_subset(int bk) {
+        layout_items();
+        std::vector<FoamItem> items;
+        for (int b = 0; b < nb; ++b) {
+            const FoamBubble& B = bub[b];
+            if (bucket_of(B.np) != bk) continue;
+            const int cpl = kVariantCpl[B.variant], rb = kVariantRows[B.variant];
+            for (int r0 = 1; r0 < B.nt - 1; r0 += rb)
+                for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, ch * 64 * cpl, ch});
+        }
+        n_items = static_cast<int>(items.size());
+        SF_HIP(hipMemcpyAsync(d_items, items.data(), n_items * sizeof(FoamItem), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+};
+
+static Foam* F(sf_foam* f) { return reinterpret_cast<Foam*>(f); }
+
+extern "C" int sf_foam_create(const sf_params* params, int32_t n, const sf_bubble* bubbles, sf_foam** out) {
+    return guard([&] {
+#pragma clang fp contract(off)
+        SF_REQUIRE(out && params && bubbles && n > 0, SF_ERR_INVALID, "bad arguments");
+        *out = nullptr;
+        SF_REQUIRE((params->flags & ~SF_FLAG_SURFACTANT) == 0, SF_ERR_INVALID,
+                   "foam bubbles support SF_FLAG_SURFACTANT only (h poles are ring means, Γ poles fixed; "
+                   "no Marangoni term, no fronts); got flags 0x%x", params->flags);
+        SF_REQUIRE(params->sin_guard > 0.0 && params->critical_thickness >= 0.0, SF_ERR_INVALID,
+                   "sin_guard must be positive and critical_thickness non-negative");
+        require_device();
+        Foam* f = new Foam();
+        try {
+            f->p = *params;
+            f->nb = n;
+            f->desc.assign(bubbles, bubbles + n);
+            int dev = 0;
+            SF_HIP(hipGetDevice(&dev));
+            hipDeviceProp_t prop;
+            SF_HIP(hipGetDeviceProperties(&prop, dev));
+            f->n_cu = prop.multiProcessorCount;
+            int occ0 = 0, occ1 = 0;
+            SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, foam_step_kernel<false>, 256, 0));
+            SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, foam_step_kernel<true>, 256, 0));
+            f->blocks_per_cu = std::max(1, std::min(occ0, occ1));
+            SF_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+            f->bub.resize(n);
+            f->offsets.resize(n + 1);
+            int64_t off = 0;
+            int rows_total = 0;
+            std::vector<FoamRow> rows;
+            std::vector<double> area, lamg, sinab, coef3(n);
+            const Float3 hc = split3(params->critical_thickness);
+            for (int b = 0; b < n; ++b) {
+                const sf_bubble& d = bubbles[b];
+                SF_REQUIRE(d.n_theta >= 3 && d.n_phi >= 2 && d.n_phi % 2 == 0, SF_ERR_INVALID,
+                           "bubble %d: need n_theta >= 3 and even n_phi >= 2 (got %dx%d)", b, d.n_theta, d.n_phi);
+                SF_REQUIRE(d.bubble_radius > 0.0 && d.viscosity > 0.0, SF_ERR_INVALID,
+                           "bubble %d: radius and viscosity must be positive", b);
+                FoamBubble& B = f->bub[b];
+                std::memset(&B, 0, sizeof(B));
+                B.off = off;
+                B.nt = d.n_theta;
+                B.np = d.n_phi;
+                B.row_off = rows_total;
+                const double dth = kPi / static_cast<double>(d.n_theta - 1);
+                const double dph = 2.0 * kPi / static_cast<double>(d.n_phi);
+                B.G0 = static_cast<float>(1.0 / (dth * dth));
+                B.np_f = static_cast<float>(d.n_phi);
+                B.hc0 = hc.c0;
+                B.hc1 = hc.c1;
+                B.hc2 = hc.c2;
+                const double R2 = d.bubble_radius * d.bubble_radius;
+                double D = params->diffusion_coeff;
+                if ((params->flags & SF_FLAG_SURFACTANT) && params->surfactant_diffusion > D)
+                    D = params->surfactant_diffusion;
+                coef3[b] = 3.0 * (params->density * params->gravity / (3.0 * d.viscosity));
+                for (int i = 0; i < d.n_theta; ++i) {
+                    const double th = static_cast<double>(i) * dth;
+                    const double s = std::sin(th), c = std::cos(th);
+                    const double ss = std::fabs(s) < params->sin_guard ? std::copysign(params->sin_guard, s) : s;
+                    rows.push_back(FoamRow{static_cast<float>(s), static_cast<float>(c / ss / (2.0 * dth)),
+                                           static_cast<float>(1.0 / (ss * ss * dph * dph)), 0.f});
+                    const double lo_b = i == 0 ? 0.0 : th - 0.5 * dth;
+                    const double hi_b = i == d.n_theta - 1 ? kPi : th + 0.5 * dth;
+                    area.push_back(R2 * (std::cos(lo_b) - std::cos(hi_b)) * dph);
+                    lamg.push_back(4.0 * D * (1.0 / (dth * dth) + 1.0 / (ss * ss * dph * dph)) / R2);
+                    sinab.push_back(std::fabs(s));
+                }
+                rows_total += d.n_theta;
+                f->offsets[b] = off;
+                off += static_cast<int64_t>(d.n_theta) * d.n_phi;
+                off = (off + 3) & ~int64_t(3);      // keep every bubble 16-byte aligned
+            }
+            f->offsets[n] = off;
+            f->total = off;
+            f->total_rows = rows_total;
+            for (int k = 0; k < 2; ++k) {
+                f->hi[k] = falloc<float>(off);
+                f->lo[k] = falloc<float>(off);
+                f->g[k] = falloc<float>(off);
+            }
+            f->rows = falloc<FoamRow>(rows_total);
+            f->d_bub = falloc<FoamBubble>(n);
+            f->d_area = falloc<double>(rows_total);
+            f->d_lamg = falloc<double>(rows_total);
+            f->d_sinab = falloc<double>(rows_total);
+            f->d_coef3 = falloc<double>(n);
+            f->d_stat = falloc<FoamStat>(n);
+            SF_HIP(hipMemcpyAsync(f->rows, rows.data(), rows.size() * sizeof(FoamRow), hipMemcpyHostToDevice, f->stream));
+            SF_HIP(hipMemcpyAsync(f->d_area, area.data(), area.size() * 8, hipMemcpyHostToDevice, f->stream));
+            SF_HIP(hipMemcpyAsync(f->d_lamg, lamg.data(), lamg.size() * 8, hipMemcpyHostToDevice, f->stream));
+            SF_HIP(hipMemcpyAsync(f->d_sinab, sinab.data(), sinab.size() * 8, hipMemcpyHostToDevice, f->stream));
+            SF_HIP(hipMemcpyAsync(f->d_coef3, coef3.data(), coef3.size() * 8, hipMemcpyHostToDevice, f->stream));
+            // uniform initial state from the descriptions
+            std::vector<float> hh(off, 0.f), ll(off, 0.f), gg(off, 0.5f);
+            for (int b = 0; b < n; ++b) {
+                const float h_hi = static_cast<float>(bubbles[b].h0);
+                const float h_lo = static_cast<float>(bubbles[b].h0 - static_cast<double>(h_hi));
+                const int64_t cells = static_cast<int64_t>(bubbles[b].n_theta) * bubbles[b].n_phi;
+                for (int64_t c = 0; c < cells; ++c) {
+                    hh[f->offsets[b] + c] = h_hi;
+                    ll[f->offsets[b] + c] = h_lo;
+                }
+            }
+            f->time.assign(n, 0.0);
+            f->layout_items();
+            f->upload(hh.data(), ll.data(), gg.data());
+            f->resolve_cfl();
+        } catch (...) {
+            delete f;
+            throw;
+        }
+        *out = reinterpret_cast<sf_foam*>(f);
+    });
+}
+
+extern "C" void sf_foam_destroy(sf_foam* f) { delete F(f); }
+
+extern "C" int sf_foam_layout(sf_foam* f, int64_t* offsets) {
+    return guard([&] {
+        SF_REQUIRE(f && offsets, SF_ERR_INVALID, "NULL argument");
+        std::copy(F(f)->offsets.begin(), F(f)->offsets.end(), offsets);
+    });
+}
+
+extern "C" int sf_foam_set_fields_f64(sf_foam* f, const double* h, const double* gamma) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        Foam* fm = F(f);
+        fm->materialize();
+        std::vector<float> hh(fm->total), ll(fm->total), gg(fm->total);
+        if (h) {
+            for (int64_t c = 0; c < fm->total; ++c) {
+                hh[c] = static_cast<float>(h[c]);
+                ll[c] = static_cast<float>(h[c] - static_cast<double>(hh[c]));
+            }
+        } else {
+            SF_HIP(hipMemcpy(hh.data(), fm->hi[fm->cur_hi], fm->total * 4, hipMemcpyDeviceToHost));
+            SF_HIP(hipMemcpy(ll.data(), fm->lo[fm->cur], fm->total * 4, hipMemcpyDeviceToHost));
+        }
+        if (gamma) {
+            for (int64_t c = 0; c < fm->total; ++c) gg[c] = static_cast<float>(gamma[c]);
+        } else {
+            SF_HIP(hipMemcpy(gg.data(), fm->g[fm->cur], fm->total * 4, hipMemcpyDeviceToHost));
+        }
+        fm->upload(hh.data(), ll.data(), gg.data());
+        fm->resolve_cfl();
+    });
+}
+
+extern "C" int sf_foam_set_fields(sf_foam* f, const float* h, const float* gamma) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && h != nullptr, SF_ERR_INVALID, "NULL argument");
+        Foam* fm = F(f);
+        std::vector<float> zeros(fm->total, 0.f), gg;
+        if (!gamma) {
+            gg.assign(fm->total, 0.5f);
+            gamma = gg.data();
+        }
+        fm->upload(h, zeros.data(), gamma);
+        fm->resolve_cfl();
+    });
+}
+
+extern "C" int sf_foam_copy_fields_f64(sf_foam* f, double* h, double* gamma) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        Foam* fm = F(f);
+        fm->materialize();
+        std::vector<float> hh(fm->total), ll(fm->total), gg(fm->total);
+        SF_HIP(hipMemcpyAsync(hh.data(), fm->hi[fm->cur_hi], fm->total * 4, hipMemcpyDeviceToHost, fm->stream));
+        SF_HIP(hipMemcpyAsync(ll.data(), fm->lo[fm->cur], fm->total * 4, hipMemcpyDeviceToHost, fm->stream));
+        SF_HIP(hipMemcpyAsync(gg.data(), fm->g[fm->cur], fm->total * 4, hipMemcpyDeviceToHost, fm->stream));
+        SF_HIP(hipStreamSynchronize(fm->stream));
+        for (int64_t c = 0; c < fm->total; ++c) {
+            if (h) h[c] = static_cast<double>(hh[c]) + static_cast<double>(ll[c]);
+            if (gamma) gamma[c] = gg[c];
+        }
+    });
+}
+
+extern "C" int sf_foam_step(sf_foam* f, int32_t n_steps) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && n_steps >= 0, SF_ERR_INVALID, "bad arguments");
+        F(f)->step(n_steps);
+    });
+}
+
+extern "C" int sf_foam_synchronize(sf_foam* f) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        SF_HIP(hipStreamSynchronize(F(f)->stream));
+    });
+}
+
+extern "C" int sf_foam_use_graph(sf_foam* f, int32_t enable) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        F(f)->use_graph = enable != 0;
+        if (!enable) F(f)->drop_graph();
+    });
+}
+
+extern "C" int sf_foam_autotune(sf_foam* f, int32_t trial_steps, int32_t* chosen) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && trial_steps > 0, SF_ERR_INVALID, "bad arguments");
+        F(f)->autotune(trial_steps, chosen);
+    });
+}
+
+extern "C" int sf_foam_set_tile_shapes(sf_foam* f, const int32_t* shapes) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && shapes != nullptr, SF_ERR_INVALID, "NULL argument");
+        for (int k = 0; k < 4; ++k)
+            SF_REQUIRE(shapes[k] >= 0 && shapes[k] < 4, SF_ERR_INVALID, "tile shape %d out of range", shapes[k]);
+        Foam* fm = F(f);
+        fm->materialize();              // pole partials change layout with the shapes
+        std::copy(shapes, shapes + 4, fm->variant_of_bucket);
+        fm->layout_items();
+    });
+}
+
+extern "C" int sf_foam_bubble_dt(sf_foam* f, double* dt) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && dt != nullptr, SF_ERR_INVALID, "NULL argument");
+        const auto v = F(f)->bubble_dt();
+        std::copy(v.begin(), v.end(), dt);
+    });
+}
+
+extern "C" int sf_foam_stats(sf_foam* f, double* mass, double* dt_cfl, double* hmin, double* hmax) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        const auto s = F(f)->stats();
+        for (size_t b = 0; b < s.size(); ++b) {
+            if (mass) mass[b] = s[b].mass;
+            if (dt_cfl) dt_cfl[b] = s[b].lam > 0.0 ? 2.0 / s[b].lam : INFINITY;
+            if (hmin) hmin[b] = s[b].mn;
+            if (hmax) hmax[b] = s[b].mx;
+        }
+    });
+}
+
+extern "C" int sf_foam_kernel_timing(sf_foam* f, int32_t enable, double* avg_ms, int64_t* launches) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        Foam* fm = F(f);
+        fm->timer.drain(fm->stream);
+        if (avg_ms) *avg_ms = fm->timer.launches ? fm->timer.ms / static_cast<double>(fm->timer.launches) : 0.0;
+        if (launches) *launches = fm->timer.launches;
+        fm->timer.ms = 0.0;
+        fm->timer.launches = 0;
+        fm->timer.enabled = enable != 0;
+    });
+}
+
+extern "C" int sf_foam_info(sf_foam* f, int64_t* total_cells, int32_t* n_items, int32_t* variants) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        Foam* fm = F(f);
+        int64_t cells = 0;
+        for (const auto& d : fm->desc) cells += static_cast<int64_t>(d.n_theta) * d.n_phi;
+        if (total_cells) *total_cells = cells;
+        if (n_items) *n_items = fm->n_items;
+        if (variants)
+            for (int k = 0; k < 4; ++k) variants[k] = fm->variant_of_bucket[k];
+    });
+}

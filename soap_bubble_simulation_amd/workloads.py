@@ -104,3 +104,33 @@ def spectral_cmf(seed: int) -> np.ndarray:
 def camera(radius: float) -> Tuple[float, float, float]:
     """Fixed synthetic camera, 4 bubble radii out, above the equator."""
     return (2.2 * radius, 1.4 * radius, 3.1 * radius)
+
+
+def foam_batch(n_bubbles: int, seed: int, lat=(32, 512), diameter=(5e-3, 50e-3),
+               h=(200e-9, 1000e-9), viscosity=(1e-3, 5e-3)):
+    """Bubble descriptions of a batched foam: n_theta ~ logU[lat] (rounded),
+    n_phi = 2 n_theta, diameter ~ logU, base thickness and viscosity ~ U.
+    Returns a list of dicts with n_theta, n_phi, bubble_radius, viscosity, h0."""
+    rng = np.random.default_rng(seed)
+    lat_v = np.exp(rng.uniform(math.log(lat[0]), math.log(lat[1]), n_bubbles))
+    nt = np.clip(np.rint(lat_v).astype(np.int64), lat[0], lat[1])
+    d = np.exp(rng.uniform(math.log(diameter[0]), math.log(diameter[1]), n_bubbles))
+    h0 = rng.uniform(h[0], h[1], n_bubbles)
+    eta = rng.uniform(viscosity[0], viscosity[1], n_bubbles)
+    return [dict(n_theta=int(nt[k]), n_phi=int(2 * nt[k]), bubble_radius=float(d[k] / 2.0),
+                 viscosity=float(eta[k]), h0=float(h0[k])) for k in range(n_bubbles)]
+
+
+def foam_noise(offsets: np.ndarray, sizes, h0, seed: int, rel: float = 0.05,
+               dtype=np.float64) -> np.ndarray:
+    """Concatenated thickness fields h0_b (1 + rel U[-1, 1]) per cell; padding
+    cells between bubbles hold h0 of the bubble before them."""
+    rng = np.random.default_rng(seed)
+    total = int(offsets[-1])
+    out = np.empty(total, dtype=dtype)
+    for k in range(len(sizes)):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        n = int(sizes[k])
+        out[a:a + n] = h0[k] * (1.0 + rel * rng.uniform(-1.0, 1.0, n))
+        out[a + n:b] = h0[k]
+    return out

@@ -92,3 +92,20 @@ def test_cpp_api_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
                         "-I", os.path.join(ROOT, "include"), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_foam_rejects_unsupported_flags():
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))      # carries SF_FLAG_TFE
+    b = (L.Bubble * 1)(L.Bubble(16, 32, 0.01, 1e-3, 500e-9, 0.0))
+    h = C.c_void_p()
+    assert L.lib().sf_foam_create(p, 1, b, C.byref(h)) == L.SF_ERR_INVALID
+    assert "SF_FLAG_SURFACTANT" in L.last_error()
+    p.flags = L.FLAG_SURFACTANT
+    b[0].n_phi = 31
+    assert L.lib().sf_foam_create(p, 1, b, C.byref(h)) in (L.SF_ERR_INVALID, L.SF_ERR_DEVICE)
+
+
+def test_bubble_struct_layout():
+    assert C.sizeof(L.Bubble) == 40
+    assert L.Bubble.bubble_radius.offset == 8

@@ -1,0 +1,175 @@
+"""GPU parity of the batched foam (sf_foam_*) against the FP64 oracle.
+
+Each foam bubble is checked against an oracle simulation of the same grid,
+radius, viscosity and time step with the foam's physics (Γ transport on,
+h poles as ring means, Γ poles fixed, no Marangoni term, no fronts), started
+from bit-identical fields.  Bars as for one bubble: rupture exact, h rel-L2
+<= 1e-6 over the trajectory, Γ within 1e-6 per step.
+"""
+import numpy as np
+import pytest
+
+from soap_bubble_simulation_amd import _lib as L
+from soap_bubble_simulation_amd import workloads as W
+from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_params
+
+pytestmark = pytest.mark.gpu
+
+# Grids chosen to hit every code path: n_phi below one wave (4, 6, 40),
+# ragged last chunks for both lane widths (130, 200), several chunks (520),
+# a bubble with a single interior row (3 x 4), each size bucket.
+SHAPES = [(3, 4), (9, 6), (20, 40), (33, 66), (65, 130), (100, 200), (64, 128), (260, 520)]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make_foam(shapes, seed, safety=0.5, crit=10e-9):
+    rng = np.random.default_rng(seed)
+    bubbles = [FoamBubble(nt, npf, bubble_radius=float(rng.uniform(2.5e-3, 25e-3)),
+                          viscosity=float(rng.uniform(1e-3, 5e-3)), h0=float(rng.uniform(200e-9, 1e-6)))
+               for nt, npf in shapes]
+    p = foam_params(critical_thickness=crit, cfl_safety=safety)
+    return FoamDrainage(p, bubbles), p
+
+
+def oracle_for(O, p, b, dt):
+    q = O.Params()
+    for name, _ in O.Params._fields_:
+        setattr(q, name, getattr(p, name))
+    q.bubble_radius = b.bubble_radius
+    q.viscosity = b.viscosity
+    q.dt = dt
+    return O.Sim(q, b.n_theta, b.n_phi)
+
+
+def seeded_fields(fm, seed, rel=0.05):
+    sizes = [b.n_theta * b.n_phi for b in fm.bubbles]
+    h = W.ff_exact(W.foam_noise(fm.offsets, sizes, [b.h0 for b in fm.bubbles], seed, rel))
+    rng = np.random.default_rng(seed + 1)
+    g = (0.5 + 0.2 * rng.uniform(-1, 1, fm.length)).astype(np.float32).astype(np.float64)
+    return h, g
+
+
+@pytest.mark.parametrize("shape_idx", [0, 1, 2, 3])
+def test_foam_trajectory_matches_oracle(oracle, shape_idx):
+    fm, p = make_foam(SHAPES, seed=10 + shape_idx)
+    fm.set_tile_shapes([shape_idx] * 4)
+    h, g = seeded_fields(fm, seed=20 + shape_idx)
+    fm.set_fields(h, g)
+    dts = fm.bubble_dt()
+    n_steps = 37                      # crosses several rebases and a graph replay
+    fm.step(n_steps)
+    hg, gg = fm.fields()
+    for k, b in enumerate(fm.bubbles):
+        orc = oracle_for(oracle, p, b, dts[k])
+        h0 = fm.split(h)[k]
+        orc.set_fields(h0, fm.split(g)[k])
+        orc.step(dts[k], n_steps)
+        ho, go = orc.fields()
+        hk, gk = fm.split(hg)[k], fm.split(gg)[k]
+        err, inc = rel_l2(hk, ho), rel_l2(hk - h0, ho - h0)
+        print("foam shape %d bubble %d (%dx%d): rel-L2 h %.3e, increment %.3e"
+              % (shape_idx, k, b.n_theta, b.n_phi, err, inc))
+        assert err <= 1e-6, (k, b, err)
+        assert inc <= 1e-5, (k, b, inc)
+        assert np.max(np.abs(gk - go)) <= 1e-6 * n_steps, (k, b)
+        # pole rows are ring means of the GPU's own field
+        assert abs(hk[0].mean() - hk[1].mean()) <= 1e-12 * abs(hk[1].mean()) + 1e-22
+        assert np.all(hk[0] == hk[0, 0]) and np.all(hk[-1] == hk[-1, 0])
+
+
+def test_foam_dt_and_mass_match_oracle(oracle):
+    fm, p = make_foam(SHAPES, seed=3, safety=0.9)
+    h, g = seeded_fields(fm, seed=4)
+    fm.set_fields(h, g)
+    dts = fm.bubble_dt()
+    st = fm.stats()
+    for k, b in enumerate(fm.bubbles):
+        orc = oracle_for(oracle, p, b, 0.0)
+        orc.set_fields(fm.split(h)[k], fm.split(g)[k])
+        assert st["dt_cfl"][k] == pytest.approx(orc.cfl_dt(), rel=1e-12)
+        assert dts[k] == pytest.approx(np.float32(0.9 * orc.cfl_dt()), rel=1e-7)
+        assert st["mass"][k] == pytest.approx(orc.mass(), rel=1e-12)
+        assert st["min"][k] == orc.min_thickness() and st["max"][k] == orc.max_thickness()
+
+
+def test_foam_graph_is_bit_identical_to_direct_launches():
+    h = None
+    outs = []
+    for graph in (True, False):
+        fm, _ = make_foam(SHAPES, seed=5)
+        if h is None:
+            h, g = seeded_fields(fm, seed=6)
+        fm.set_fields(h, g)
+        fm.use_graph(graph)
+        fm.step(1 + 16 * 3 + 5)
+        outs.append(fm.fields())
+        fm.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_foam_autotune_keeps_state_and_shapes_agree():
+    fm, _ = make_foam(SHAPES, seed=7)
+    h, g = seeded_fields(fm, seed=8)
+    fm.set_fields(h, g)
+    fm.step(5)
+    before = fm.fields()
+    chosen = fm.autotune(trial_steps=3)
+    assert all(0 <= c < 4 for c in chosen)
+    after = fm.fields()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert fm.info()["variants"] == chosen
+    fm.step(20)
+    tuned = fm.fields()
+    # Every shape computes the same cell formula; the order of the pole-ring
+    # sums and the compiler's FMA contraction may differ between shapes, an
+    # FP32 rounding of the per-step increment (~1e-9 of h per step here).
+    ref, _ = make_foam(SHAPES, seed=7)
+    ref.set_fields(h, g)
+    ref.set_tile_shapes([0, 0, 0, 0])
+    ref.step(25)
+    plain = ref.fields()
+    assert rel_l2(tuned[0], plain[0]) <= 1e-9
+    assert rel_l2(tuned[0] - h, plain[0] - h) <= 1e-6
+    assert np.max(np.abs(tuned[1] - plain[1])) <= 1e-6
+
+
+def test_foam_uniform_rows_follow_closed_form():
+    """D = 0 and uniform h: each row is forward Euler on h' = -c h^3 sin(theta)."""
+    shapes = [(17, 34), (40, 80), (129, 258)]
+    rng = np.random.default_rng(11)
+    bubbles = [FoamBubble(nt, npf, bubble_radius=0.01, viscosity=float(rng.uniform(1e-3, 5e-3)),
+                          h0=600e-9, dt=50.0) for nt, npf in shapes]
+    p = foam_params()
+    p.diffusion_coeff = 0.0
+    p.surfactant_diffusion = 0.0
+    fm = FoamDrainage(p, bubbles)
+    n = 40
+    fm.step(n)
+    hg, _ = fm.fields()
+    for k, b in enumerate(bubbles):
+        hk = fm.split(hg)[k]
+        th = np.arange(b.n_theta) * np.pi / (b.n_theta - 1)
+        c = p.density * p.gravity / (3.0 * b.viscosity)
+        y = np.full(b.n_theta, b.h0)
+        for _ in range(n):
+            y[1:-1] = np.maximum(y[1:-1] + b.dt * (-c * y[1:-1] ** 3 * np.sin(th[1:-1])), 0.0)
+            y[0], y[-1] = y[1], y[-2]
+        assert np.all(hk == hk[:, :1]), "rows must stay uniform in phi"
+        assert np.max(np.abs(hk[:, 0] - y) / y) <= 2e-6
+
+
+def test_foam_rupture_cells_frozen():
+    fm, p = make_foam([(33, 66), (65, 130)], seed=12, crit=300e-9)
+    h, g = seeded_fields(fm, seed=13, rel=0.5)
+    fm.set_fields(h, g)
+    fm.step(1)
+    hg, gg = fm.fields()
+    for k in range(2):
+        h0, g0 = fm.split(h)[k], fm.split(g)[k].astype(np.float32)
+        m = h0[1:-1] < 300e-9
+        assert m.any() and (~m).any()
+        assert np.array_equal(fm.split(hg)[k][1:-1][m], h0[1:-1][m])
+        assert np.array_equal(fm.split(gg)[k][1:-1][m], g0[1:-1][m].astype(np.float64))
