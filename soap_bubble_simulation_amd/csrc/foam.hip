@@ -36,8 +36,10 @@ struct FoamBubble {
     int nt, np;
     int row_off;          // first row in the row-coefficient table
     int pole_off;         // first ring partial: [nchunks top][nchunks bottom]
-    int nchunks;          // column chunks of the bubble's item shape
-    int variant;          // item shape: 0 (1,4) 1 (1,8) 2 (2,4) 3 (2,8) as (CPL, RB)
+    int nchunks;          // column chunks of the bubble's tile shape
+    int variant;          // tile shape index (kShapeCpl, kShapeBand)
+    int path;             // 0: 1 column per lane, 1: 2, 2: 4 (float4), 3: 4 (two float2)
+    int pad_;
     float dt;             // s
     float dtD;            // dt D_h / R^2
     float dtDs;           // dt D_s / R^2
@@ -47,8 +49,9 @@ struct FoamBubble {
     float hc0, hc1, hc2;  // critical thickness, three-float expansion
 };
 
+// One wave's work: rows [r0, r1) of the column chunk starting at c0.
 struct FoamItem {
-    int bubble, r0, c0, chunk;
+    int bubble, r0, r1, c0, chunk, pad_[3];
 };
 
 // Per-row coefficients of one bubble: sin(theta), cot/(2 dtheta),
@@ -66,38 +69,70 @@ struct FoamArgs {
     float* g_out;
     const FoamRow* rows;
     const FoamBubble* bub;
-    const FoamItem* items;
-    int n_items;
     const float2* hpole_in;
     float2* hpole_out;
     int pole_mean;        // ring neighbours of the poles from hpole_in
 };
 
-constexpr int kVariantCpl[4] = {1, 1, 2, 2};
-constexpr int kVariantRows[4] = {4, 8, 4, 8};
+// Tile shapes: columns per lane x rows per item (a band of rows is streamed
+// top to bottom by one wave, so taller bands re-read fewer halo rows).
+constexpr int kShapes = 8;
+constexpr int kShapeCpl[kShapes] = {1, 1, 2, 2, 2, 4, 4, 4};
+constexpr int kShapeBand[kShapes] = {16, 32, 16, 32, 64, 16, 32, 64};
+constexpr int kPaths = 4;     // 1, 2, 4 (float4) and 4 (two float2) columns per lane
 
+// One row of a lane's columns plus its west and east neighbours.
 template <int CPL>
 struct FRow {
     float h[CPL], l[CPL], g[CPL];
     float hw, lw, gw, he, le, ge;
 };
 
-template <int CPL>
-__device__ inline void foam_load(const FoamArgs& a, int64_t off, int np, int nt, int r, int cc,
+// Lane columns: CPL values from column cc.  With four columns per lane and
+// n_phi = 2 mod 4 (V4 false) the two halves are separate float2 accesses and
+// the second half starts at column cc2, which for the row's last lane wraps
+// to column 0: its columns 2 and 3 then hold the east neighbour of column 1
+// and are never stored.
+template <int CPL, bool V4>
+__device__ inline void load_vec(const float* row, int cc, int cc2, float (&v)[CPL]) {
+    if constexpr (CPL == 4 && V4) {
+        const float4 t = *reinterpret_cast<const float4*>(row + cc);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else if constexpr (CPL == 4) {
+        const float2 t0 = *reinterpret_cast<const float2*>(row + cc);
+        const float2 t1 = *reinterpret_cast<const float2*>(row + cc2);
+        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
+    } else if constexpr (CPL == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(row + cc);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        v[0] = row[cc];
+    }
+}
+
+template <int CPL, bool V4>
+__device__ inline void store_vec(float* row, int cc, bool full, const float (&v)[CPL]) {
+    if constexpr (CPL == 4 && V4) {
+        *reinterpret_cast<float4*>(row + cc) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (CPL == 4) {
+        *reinterpret_cast<float2*>(row + cc) = make_float2(v[0], v[1]);
+        if (full) *reinterpret_cast<float2*>(row + cc + 2) = make_float2(v[2], v[3]);
+    } else if constexpr (CPL == 2) {
+        *reinterpret_cast<float2*>(row + cc) = make_float2(v[0], v[1]);
+    } else {
+        row[cc] = v[0];
+    }
+}
+
+// Loads row r (clamped to the last row) of the lane's columns.  Every lane
+// loads, valid or not, so the wave never diverges around memory accesses.
+template <int CPL, bool V4>
+__device__ inline void foam_load(const FoamArgs& a, int64_t off, int np, int nt, int r, int cc, int cc2,
                                  int jw, int je, FRow<CPL>& R) {
     const int64_t base = off + static_cast<int64_t>(min(r, nt - 1)) * np;
-    if (CPL == 2) {
-        const float2 h = *reinterpret_cast<const float2*>(a.hi_in + base + cc);
-        const float2 l = *reinterpret_cast<const float2*>(a.lo_in + base + cc);
-        const float2 g = *reinterpret_cast<const float2*>(a.g_in + base + cc);
-        R.h[0] = h.x; R.h[CPL - 1] = h.y;
-        R.l[0] = l.x; R.l[CPL - 1] = l.y;
-        R.g[0] = g.x; R.g[CPL - 1] = g.y;
-    } else {
-        R.h[0] = a.hi_in[base + cc];
-        R.l[0] = a.lo_in[base + cc];
-        R.g[0] = a.g_in[base + cc];
-    }
+    load_vec<CPL, V4>(a.hi_in + base, cc, cc2, R.h);
+    load_vec<CPL, V4>(a.lo_in + base, cc, cc2, R.l);
+    load_vec<CPL, V4>(a.g_in + base, cc, cc2, R.g);
     R.hw = a.hi_in[base + jw];
     R.lw = a.lo_in[base + jw];
     R.gw = a.g_in[base + jw];
@@ -117,153 +152,162 @@ __device__ inline void foam_pole_row(FRow<CPL>& R, float2 P) {
     R.lw = R.le = P.y;
 }
 
-template <int CPL, int RB, bool REBASE>
-__device__ void foam_item(const FoamArgs& a, const FoamItem it, const FoamBubble& B, int lane) {
-    constexpr int PF = 2;
+// Wave-wide float-float sum of the lane's new ring values; lane 0 stores it.
+template <int CPL>
+__device__ inline void ring_partial(const float (&h)[CPL], const float (&l)[CPL], int nvalid, int lane,
+                                    float2* out) {
+    float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int v = 0; v < CPL; ++v) {
+        if (v < nvalid) {
+            float a_, e_;
+            two_sum(h[v], l[v], a_, e_);
+            s = ff_add(s, make_float2(a_, e_));
+        }
+    }
+    s = wave_ff_sum(s);
+    if (lane == 0) *out = s;
+}
+
+// Per-lane constants of one work item.
+struct BandLane {
+    int cc, cc2, jw, je;      // first column, second-half column, west and east neighbours
+    int nvalid;               // columns this lane stores (0 for idle lanes)
+    bool valid, full;
+    float2 ptop, pbot;        // pole means (virtual poles)
+    FoamRow rl;               // coefficients of row r0 + lane (read back with readlane)
+};
+
+__device__ inline float lane_value(float v, int t) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
+}
+
+// Computes and stores row i from its north, centre and south rows.  The
+// pole rows are replaced in place: row 0 is reloaded after row 1, and row
+// nt - 2 is the last row any band computes.
+template <int CPL, bool V4, bool REBASE>
+__device__ inline void band_row(const FoamArgs& a, const FoamItem& it, const FoamBubble& B, const BandLane& L,
+                                int lane, int i, FRow<CPL>& N, const FRow<CPL>& C, FRow<CPL>& Sx) {
     const int np = B.np, nt = B.nt;
-    const int c = it.c0 + CPL * lane;
-    const bool valid = c < np;
-    const int cc = valid ? c : np - CPL;
-    const int jw = cc == 0 ? np - 1 : cc - 1;
-    const int je = cc + CPL >= np ? cc + CPL - np : cc + CPL;
-    const int r0 = it.r0;
-    const int r1 = min(r0 + RB, nt - 1);
-    const bool touches_top = r0 == 1, touches_bot = r1 == nt - 1;
+    if (a.pole_mean && i == 1) foam_pole_row<CPL>(N, L.ptop);
+    if (a.pole_mean && i == nt - 2) foam_pole_row<CPL>(Sx, L.pbot);
+    const int t = i - it.r0;                   // wave-uniform, < 64
+    const float G1 = lane_value(L.rl.G1, t), G2 = lane_value(L.rl.G2, t);
+    const float cS = B.dtc * lane_value(L.rl.sin_t, t);
 
-    // Lane t < RB holds the coefficients of row r0 + t; rows read them with
-    // readlane, so the loop never waits on a vector load of its own.
-    const FoamRow rl = a.rows[B.row_off + min(r0 + (lane % RB), nt - 1)];
-
-    float2 ptop = make_float2(0.f, 0.f), pbot = make_float2(0.f, 0.f);
-    if (a.pole_mean) {
-        if (touches_top) ptop = pole_mean(a.hpole_in + B.pole_off, B.nchunks, B.np_f);
-        if (touches_bot) pbot = pole_mean(a.hpole_in + B.pole_off + B.nchunks, B.nchunks, B.np_f);
+    float wh[CPL], wl[CPL], wg[CPL], eh[CPL], el[CPL], eg[CPL];
+    wh[0] = C.hw; wl[0] = C.lw; wg[0] = C.gw;
+    eh[CPL - 1] = C.he; el[CPL - 1] = C.le; eg[CPL - 1] = C.ge;
+#pragma unroll
+    for (int v = 1; v < CPL; ++v) {
+        wh[v] = C.h[v - 1]; wl[v] = C.l[v - 1]; wg[v] = C.g[v - 1];
+        eh[v - 1] = C.h[v]; el[v - 1] = C.l[v]; eg[v - 1] = C.g[v];
     }
-
-    FRow<CPL> S[RB + 2];
+    float lo_new[CPL], g_new[CPL];
 #pragma unroll
-    for (int k = 0; k < PF + 2 && k < RB + 2; ++k) foam_load<CPL>(a, B.off, np, nt, r0 - 1 + k, cc, jw, je, S[k]);
-
-    float top_h[CPL], top_l[CPL], bot_h[CPL], bot_l[CPL];
-#pragma unroll
-    for (int v = 0; v < CPL; ++v) top_h[v] = top_l[v] = bot_h[v] = bot_l[v] = 0.f;
-
-#pragma unroll
-    for (int t = 0; t < RB; ++t) {
-        if (t + PF + 2 < RB + 2) foam_load<CPL>(a, B.off, np, nt, r0 + t + PF + 1, cc, jw, je, S[t + PF + 2]);
-        const int i = r0 + t;
-        if (i < r1) {
-            FRow<CPL> N = S[t];
-            const FRow<CPL> C = S[t + 1];
-            FRow<CPL> Sx = S[t + 2];
-            if (a.pole_mean && i == 1) foam_pole_row<CPL>(N, ptop);
-            if (a.pole_mean && i == nt - 2) foam_pole_row<CPL>(Sx, pbot);
-            const float sin_t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl.sin_t), t));
-            const float G1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl.G1), t));
-            const float G2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl.G2), t));
-            const float cS = B.dtc * sin_t;
-
-            float wh[CPL], wl[CPL], wg[CPL], eh[CPL], el[CPL], eg[CPL];
-            wh[0] = C.hw; wl[0] = C.lw; wg[0] = C.gw;
-            eh[CPL - 1] = C.he; el[CPL - 1] = C.le; eg[CPL - 1] = C.ge;
-            if (CPL == 2) {
-                wh[CPL - 1] = C.h[0]; wl[CPL - 1] = C.l[0]; wg[CPL - 1] = C.g[0];
-                eh[0] = C.h[CPL - 1]; el[0] = C.l[CPL - 1]; eg[0] = C.g[CPL - 1];
-            }
-            float lo_new[CPL], g_new[CPL];
-#pragma unroll
-            for (int v = 0; v < CPL; ++v) {
-                const float ch = C.h[v], cl = C.l[v], cg = C.g[v];
-                const bool rupt = ff_less(ch, cl, B.hc0, B.hc1, B.hc2);
-                const float dtp = (Sx.h[v] - ch) + (Sx.l[v] - cl);
-                const float dtm = (ch - N.h[v]) + (cl - N.l[v]);
-                const float dpp = (eh[v] - ch) + (el[v] - cl);
-                const float dpm = (ch - wh[v]) + (cl - wl[v]);
-                const float lap = (dtp - dtm) * B.G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
-                const float H = ch + cl;
-                const float ln = fmaxf(cl + fmaf(B.dtD, lap, -cS * (H * H * H)), -ch);
-                lo_new[v] = rupt ? cl : ln;
-                const float gtp = Sx.g[v] - cg, gtm = cg - N.g[v];
-                const float gpp = eg[v] - cg, gpm = cg - wg[v];
-                const float glap = (gtp - gtm) * B.G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
-                const float gn = fminf(fmaxf(fmaf(B.dtDs, glap, cg), 0.f), 1.f);
-                g_new[v] = rupt ? cg : gn;
-            }
-            if (i == 1) {
-#pragma unroll
-                for (int v = 0; v < CPL; ++v) { top_h[v] = C.h[v]; top_l[v] = lo_new[v]; }
-            }
-            if (i == nt - 2) {
-#pragma unroll
-                for (int v = 0; v < CPL; ++v) { bot_h[v] = C.h[v]; bot_l[v] = lo_new[v]; }
-            }
-            if (valid) {
-                const int64_t o = B.off + static_cast<int64_t>(i) * np + cc;
-                if (CPL == 2) {
-                    if (REBASE) {
-                        float2 hh, ll;
-                        two_sum(C.h[0], lo_new[0], hh.x, ll.x);
-                        two_sum(C.h[CPL - 1], lo_new[CPL - 1], hh.y, ll.y);
-                        *reinterpret_cast<float2*>(a.hi_out + o) = hh;
-                        *reinterpret_cast<float2*>(a.lo_out + o) = ll;
-                    } else {
-                        *reinterpret_cast<float2*>(a.lo_out + o) = make_float2(lo_new[0], lo_new[CPL - 1]);
-                    }
-                    *reinterpret_cast<float2*>(a.g_out + o) = make_float2(g_new[0], g_new[CPL - 1]);
-                } else {
-                    if (REBASE) {
-                        float hh, ll;
-                        two_sum(C.h[0], lo_new[0], hh, ll);
-                        a.hi_out[o] = hh;
-                        a.lo_out[o] = ll;
-                    } else {
-                        a.lo_out[o] = lo_new[0];
-                    }
-                    a.g_out[o] = g_new[0];
-                }
-            }
-        }
+    for (int v = 0; v < CPL; ++v) {
+        const float ch = C.h[v], cl = C.l[v], cg = C.g[v];
+        // Rupture mask on the input state (drainage.rs:331-333).
+        const bool rupt = ff_less(ch, cl, B.hc0, B.hc1, B.hc2);
+        const float dtp = (Sx.h[v] - ch) + (Sx.l[v] - cl);
+        const float dtm = (ch - N.h[v]) + (cl - N.l[v]);
+        const float dpp = (eh[v] - ch) + (el[v] - cl);
+        const float dpm = (ch - wh[v]) + (cl - wl[v]);
+        const float lap = (dtp - dtm) * B.G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
+        const float H = ch + cl;
+        const float ln = fmaxf(cl + fmaf(B.dtD, lap, -cS * (H * H * H)), -ch);   // h >= 0
+        lo_new[v] = rupt ? cl : ln;
+        const float gtp = Sx.g[v] - cg, gtm = cg - N.g[v];
+        const float gpp = eg[v] - cg, gpm = cg - wg[v];
+        const float glap = (gtp - gtm) * B.G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
+        const float gn = fminf(fmaxf(fmaf(B.dtDs, glap, cg), 0.f), 1.f);  // drainage.wgsl:198-201
+        g_new[v] = rupt ? cg : gn;
     }
+    // Ring partials of this chunk for the pole means of the next step
+    // (drainage.rs:480-503), from the new ring values.
+    if (i == 1) ring_partial<CPL>(C.h, lo_new, L.nvalid, lane, a.hpole_out + B.pole_off + it.chunk);
+    if (i == nt - 2) ring_partial<CPL>(C.h, lo_new, L.nvalid, lane, a.hpole_out + B.pole_off + B.nchunks + it.chunk);
 
-    // Ring partials of this chunk for the pole means of the next step.
-    if (touches_top || touches_bot) {
-        float2 st = make_float2(0.f, 0.f), sb = make_float2(0.f, 0.f);
-        if (valid) {
+    if (L.valid) {
+        const int64_t o = B.off + static_cast<int64_t>(i) * np;
+        if (REBASE) {
+            float hh[CPL], ll[CPL];
 #pragma unroll
-            for (int v = 0; v < CPL; ++v) {
-                float s_, e_;
-                two_sum(top_h[v], top_l[v], s_, e_);
-                st = ff_add(st, make_float2(s_, e_));
-                two_sum(bot_h[v], bot_l[v], s_, e_);
-                sb = ff_add(sb, make_float2(s_, e_));
-            }
+            for (int v = 0; v < CPL; ++v) two_sum(C.h[v], lo_new[v], hh[v], ll[v]);
+            store_vec<CPL, V4>(a.hi_out + o, L.cc, L.full, hh);
+            store_vec<CPL, V4>(a.lo_out + o, L.cc, L.full, ll);
+        } else {
+            store_vec<CPL, V4>(a.lo_out + o, L.cc, L.full, lo_new);
         }
-        st = wave_ff_sum(st);
-        sb = wave_ff_sum(sb);
-        if (lane == 0) {
-            if (touches_top) a.hpole_out[B.pole_off + it.chunk] = st;
-            if (touches_bot) a.hpole_out[B.pole_off + B.nchunks + it.chunk] = sb;
-        }
+        store_vec<CPL, V4>(a.g_out + o, L.cc, L.full, g_new);
     }
 }
 
-template <bool REBASE>
-__global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a) {
+// Streams rows [r0, r1) of one column chunk of one bubble through a window
+// of five row buffers per lane: north, centre, south and two rows in
+// flight.  The loop is unrolled over the five buffer rotations so that no
+// buffer is ever copied (a copy would wait for the loads in flight).
+template <int CPL, bool V4, bool REBASE>
+__device__ void foam_band(const FoamArgs& a, const FoamItem& it, const FoamBubble& B, int lane) {
+    const int np = B.np, nt = B.nt;
+    BandLane L;
+    const int c = it.c0 + CPL * lane;
+    L.valid = c < np;
+    L.cc = L.valid ? c : 0;                            // idle lanes shadow column 0
+    L.jw = L.cc == 0 ? np - 1 : L.cc - 1;
+    L.je = (L.cc + CPL) % np;
+    L.full = L.cc + CPL <= np;                         // false only for a 2-column tail lane
+    L.cc2 = L.full ? L.cc + 2 : L.cc + 2 - np;
+    L.nvalid = L.valid ? (L.full ? CPL : 2) : 0;
+    const int r0 = it.r0, r1 = it.r1;
+    L.rl = a.rows[B.row_off + min(r0 + lane, nt - 1)];   // bands hold at most 64 rows
+    L.ptop = L.pbot = make_float2(0.f, 0.f);
+    if (a.pole_mean) {
+        if (r0 == 1) L.ptop = pole_mean(a.hpole_in + B.pole_off, B.nchunks, B.np_f);
+        if (r1 == nt - 1) L.pbot = pole_mean(a.hpole_in + B.pole_off + B.nchunks, B.nchunks, B.np_f);
+    }
+
+    FRow<CPL> P0, P1, P2, P3, P4;
+    foam_load<CPL, V4>(a, B.off, np, nt, r0 - 1, L.cc, L.cc2, L.jw, L.je, P0);
+    foam_load<CPL, V4>(a, B.off, np, nt, r0, L.cc, L.cc2, L.jw, L.je, P1);
+    foam_load<CPL, V4>(a, B.off, np, nt, r0 + 1, L.cc, L.cc2, L.jw, L.je, P2);
+    foam_load<CPL, V4>(a, B.off, np, nt, r0 + 2, L.cc, L.cc2, L.jw, L.je, P3);
+    foam_load<CPL, V4>(a, B.off, np, nt, r0 + 3, L.cc, L.cc2, L.jw, L.je, P4);
+
+    // Row i uses (N, C, S); the N buffer then receives row i + 4.
+#define SF_FOAM_ROW(Nb, Cb, Sb)                                                                    \
+    {                                                                                              \
+        if (i >= r1) break;                                                                        \
+        band_row<CPL, V4, REBASE>(a, it, B, L, lane, i, Nb, Cb, Sb);                               \
+        if (i + 4 <= r1) foam_load<CPL, V4>(a, B.off, np, nt, i + 4, L.cc, L.cc2, L.jw, L.je, Nb); \
+        ++i;                                                                                       \
+    }
+    for (int i = r0;;) {
+        SF_FOAM_ROW(P0, P1, P2)
+        SF_FOAM_ROW(P1, P2, P3)
+        SF_FOAM_ROW(P2, P3, P4)
+        SF_FOAM_ROW(P3, P4, P0)
+        SF_FOAM_ROW(P4, P0, P1)
+    }
+#undef SF_FOAM_ROW
+}
+
+// One work item per wave.  Items are grouped by code path (lane width and
+// vector access) and every path is its own kernel, so that each runs at the
+// occupancy its own register count allows.
+template <int PATH, bool REBASE>
+__global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a, const FoamItem* items, int n) {
     const int lane = threadIdx.x & 63;
     // The item index is wave-uniform; readfirstlane lets the compiler keep it
     // (and the item and bubble records) in scalar registers.
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-    const int nw = static_cast<int>((gridDim.x * blockDim.x) >> 6);
-    for (int k = wave; k < a.n_items; k += nw) {
-        const FoamItem it = a.items[k];
-        const FoamBubble B = a.bub[it.bubble];
-        switch (B.variant) {
-            case 0: foam_item<1, 4, REBASE>(a, it, B, lane); break;
-            case 1: foam_item<1, 8, REBASE>(a, it, B, lane); break;
-            case 2: foam_item<2, 4, REBASE>(a, it, B, lane); break;
-            default: foam_item<2, 8, REBASE>(a, it, B, lane); break;
-        }
-    }
+    const int k = __builtin_amdgcn_readfirstlane(static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (k >= n) return;
+    const FoamItem it = items[k];
+    const FoamBubble B = a.bub[it.bubble];
+    if constexpr (PATH == 0) foam_band<1, false, REBASE>(a, it, B, lane);
+    else if constexpr (PATH == 1) foam_band<2, false, REBASE>(a, it, B, lane);
+    else if constexpr (PATH == 2) foam_band<4, true, REBASE>(a, it, B, lane);
+    else foam_band<4, false, REBASE>(a, it, B, lane);
 }
 
 // Writes every bubble's pole rows from the ring partials (one block per bubble).
@@ -362,7 +406,6 @@ struct Foam {
     int total_rows = 0;
     hipStream_t stream = nullptr;
     int n_cu = 256;
-    int blocks_per_cu = 4;              // resident 256-thread blocks of the step kernel
 
     float* hi[2] = {nullptr, nullptr};
     float* lo[2] = {nullptr, nullptr};
@@ -376,12 +419,14 @@ struct Foam {
     FoamItem* d_items = nullptr;
     int n_items = 0;
     int items_cap = 0;
+    int pole_cap = 0;
+    int path_begin[kPaths + 1] = {0, 0, 0, 0, 0};
     double* d_area = nullptr;
     double* d_lamg = nullptr;
     double* d_sinab = nullptr;
     double* d_coef3 = nullptr;
     FoamStat* d_stat = nullptr;
-    int variant_of_bucket[4] = {1, 1, 3, 3};
+    int variant_of_bucket[4] = {1, 3, 6, 6};
     std::vector<double> time;
     int64_t steps_taken = 0;
 
@@ -402,25 +447,50 @@ struct Foam {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // Item shapes and the pole-partial layout follow from each bubble's variant.
-    void layout_items() {
+    // Tile shape, chunking and pole-partial slots of bubble b.
+    void shape_bubble(int b) {
+        FoamBubble& B = bub[b];
+        B.variant = variant_of_bucket[bucket_of(B.np)];
+        const int cpl = kShapeCpl[B.variant];
+        B.path = cpl == 1 ? 0 : (cpl == 2 ? 1 : (B.np % 4 == 0 ? 2 : 3));
+        B.nchunks = static_cast<int>(cdivu(B.np, 64 * cpl));
+    }
+
+    // Work items of bubble b: column chunks x bands of at most kShapeBand
+    // rows, the interior rows split as evenly as the band count allows.
+    void append_items(int b, std::vector<FoamItem>& items) const {
+        const FoamBubble& B = bub[b];
+        const int cpl = kShapeCpl[B.variant];
+        const int rows = B.nt - 2;
+        const int nbands = (rows + kShapeBand[B.variant] - 1) / kShapeBand[B.variant];
+        for (int k = 0; k < nbands; ++k) {
+            const int r0 = 1 + static_cast<int>(static_cast<int64_t>(rows) * k / nbands);
+            const int r1 = 1 + static_cast<int>(static_cast<int64_t>(rows) * (k + 1) / nbands);
+            for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, r1, ch * 64 * cpl, ch, {0, 0, 0}});
+        }
+    }
+
+    // Work items, grouped by code path; within a path bubbles, bands and
+    // chunks keep their natural order, so that the blocks running at the
+    // same time stream neighbouring rows and share their halo rows and edge
+    // columns in cache.  only_bucket >= 0 keeps the bubbles of one size
+    // bucket (autotuning trials); pole slots are laid out for all bubbles.
+    void layout_items(int only_bucket = -1) {
         SF_HIP(hipStreamSynchronize(stream));     // buffers below may still be in use
-        std::vector<FoamItem> items;
+        std::vector<FoamItem> per_path[kPaths];
         pole_slots = 0;
         for (int b = 0; b < nb; ++b) {
-            FoamBubble& B = bub[b];
-            B.variant = variant_of_bucket[bucket_of(B.np)];
-            const int cpl = kVariantCpl[B.variant], rb = kVariantRows[B.variant];
-            B.nchunks = static_cast<int>(cdivu(B.np, 64 * cpl));
-            B.pole_off = pole_slots;
-            pole_slots += 2 * B.nchunks;
-            for (int r0 = 1; r0 < B.nt - 1; r0 += rb)
-                for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, ch * 64 * cpl, ch});
+            shape_bubble(b);
+            bub[b].pole_off = pole_slots;
+            pole_slots += 2 * bub[b].nchunks;
+            if (only_bucket < 0 || bucket_of(bub[b].np) == only_bucket) append_items(b, per_path[bub[b].path]);
         }
-        // Largest bubbles first, so that the round-robin deal ends evenly.
-        std::stable_sort(items.begin(), items.end(), [&](const FoamItem& x, const FoamItem& y) {
-            return bub[x.bubble].np > bub[y.bubble].np;
-        });
+        std::vector<FoamItem> items;
+        for (int p = 0; p < kPaths; ++p) {
+            path_begin[p] = static_cast<int>(items.size());
+            items.insert(items.end(), per_path[p].begin(), per_path[p].end());
+        }
+        path_begin[kPaths] = static_cast<int>(items.size());
         n_items = static_cast<int>(items.size());
         if (n_items > items_cap) {
             (void)hipFree(d_items);
@@ -429,9 +499,12 @@ struct Foam {
         }
         SF_HIP(hipMemcpyAsync(d_items, items.data(), n_items * sizeof(FoamItem), hipMemcpyHostToDevice, stream));
         SF_HIP(hipMemcpyAsync(d_bub, bub.data(), nb * sizeof(FoamBubble), hipMemcpyHostToDevice, stream));
-        for (int k = 0; k < 2; ++k) {
-            (void)hipFree(hpole[k]);
-            hpole[k] = falloc<float2>(std::max(pole_slots, 1));
+        if (pole_slots > pole_cap) {
+            for (int k = 0; k < 2; ++k) {
+                (void)hipFree(hpole[k]);
+                hpole[k] = falloc<float2>(pole_slots);
+            }
+            pole_cap = pole_slots;
         }
         SF_HIP(hipStreamSynchronize(stream));
         drop_graph();
@@ -441,13 +514,6 @@ struct Foam {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         graph_base = -1;
-    }
-
-    // Every block of the persistent grid is resident at once (blocks that
-    // waited for a free slot would start their item lists late).
-    unsigned grid_blocks() const {
-        const unsigned resident = static_cast<unsigned>(n_cu) * static_cast<unsigned>(blocks_per_cu);
-        return std::max(1u, std::min(resident, cdivu(static_cast<size_t>(n_items), 4)));
     }
 
     FoamArgs args_for(int cur_, int cur_hi_, int cur_pole_, bool pole_mean) const {
@@ -460,20 +526,30 @@ struct Foam {
         a.g_out = g[1 - cur_];
         a.rows = rows;
         a.bub = d_bub;
-        a.items = d_items;
-        a.n_items = n_items;
         a.hpole_in = hpole[cur_pole_];
         a.hpole_out = hpole[1 - cur_pole_];
         a.pole_mean = pole_mean ? 1 : 0;
         return a;
     }
 
-    void launch_one(bool rebase, const FoamArgs& a) {
-        const dim3 grid(grid_blocks()), blk(256);
+    template <int PATH>
+    void launch_path(bool rebase, const FoamArgs& a) {
+        const int n = path_begin[PATH + 1] - path_begin[PATH];
+        if (n == 0) return;
+        const dim3 grid(cdivu(static_cast<size_t>(n), 4)), blk(256);
+        const FoamItem* items = d_items + path_begin[PATH];
         if (rebase)
-            hipLaunchKernelGGL(foam_step_kernel<true>, grid, blk, 0, stream, a);
+            hipLaunchKernelGGL((foam_step_kernel<PATH, true>), grid, blk, 0, stream, a, items, n);
         else
-            hipLaunchKernelGGL(foam_step_kernel<false>, grid, blk, 0, stream, a);
+            hipLaunchKernelGGL((foam_step_kernel<PATH, false>), grid, blk, 0, stream, a, items, n);
+    }
+
+    // One step: a launch per code path in use.
+    void launch_one(bool rebase, const FoamArgs& a) {
+        launch_path<0>(rebase, a);
+        launch_path<1>(rebase, a);
+        launch_path<2>(rebase, a);
+        launch_path<3>(rebase, a);
     }
 
     // One step with the current buffers; advances the ping-pong state.
@@ -645,7 +721,9 @@ struct Foam {
     }
 
     // Times every item shape on each size bucket and keeps the fastest.
-    void autotune(int trial_steps, int32_t* chosen) {
+    void autotune(int trial_steps, int32_t* chosen, double* trial_ms) {
+        if (trial_ms)
+            for (int k = 0; k < 4 * kShapes; ++k) trial_ms[k] = -1.0;
         materialize();                  // the state is then whole in the current buffers
         SF_HIP(hipStreamSynchronize(stream));
         Snapshot snap = snapshot();
@@ -666,13 +744,13 @@ struct Foam {
             if (!present) continue;
             double best = INFINITY;
             int best_v = variant_of_bucket[bk];
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < kShapes; ++v) {
                 // Only this bucket's bubbles take part in the trial.
                 std::vector<FoamBubble> saved = bub;
                 int save[4];
                 std::copy(variant_of_bucket, variant_of_bucket + 4, save);
                 variant_of_bucket[bk] = v;
-                layout_items_subset(bk);
+                layout_items(bk);
                 poles_virtual = false;      // first trial step reads the stored pole rows
                 step_direct();              // untimed warm-up
                 timer.enabled = true;
@@ -682,6 +760,7 @@ struct Foam {
                 timer.drain(stream);
                 const double ms = timer.launches ? timer.ms / timer.launches : INFINITY;
                 timer.enabled = false;
+                if (trial_ms) trial_ms[bk * kShapes + v] = ms;
                 if (ms < best) {
                     best = ms;
                     best_v = v;
@@ -705,21 +784,6 @@ struct Foam {
             for (int k = 0; k < 4; ++k) chosen[k] = variant_of_bucket[k];
     }
 
-    // Item list restricted to one size bucket (autotuning trials).
-    void layout_items_subset(int bk) {
-        layout_items();
-        std::vector<FoamItem> items;
-        for (int b = 0; b < nb; ++b) {
-            const FoamBubble& B = bub[b];
-            if (bucket_of(B.np) != bk) continue;
-            const int cpl = kVariantCpl[B.variant], rb = kVariantRows[B.variant];
-            for (int r0 = 1; r0 < B.nt - 1; r0 += rb)
-                for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, ch * 64 * cpl, ch});
-        }
-        n_items = static_cast<int>(items.size());
-        SF_HIP(hipMemcpyAsync(d_items, items.data(), n_items * sizeof(FoamItem), hipMemcpyHostToDevice, stream));
-        SF_HIP(hipStreamSynchronize(stream));
-    }
 };
 
 static Foam* F(sf_foam* f) { return reinterpret_cast<Foam*>(f); }
@@ -745,10 +809,6 @@ extern "C" int sf_foam_create(const sf_params* params, int32_t n, const sf_bubbl
             hipDeviceProp_t prop;
             SF_HIP(hipGetDeviceProperties(&prop, dev));
             f->n_cu = prop.multiProcessorCount;
-            int occ0 = 0, occ1 = 0;
-            SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, foam_step_kernel<false>, 256, 0));
-            SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, foam_step_kernel<true>, 256, 0));
-            f->blocks_per_cu = std::max(1, std::min(occ0, occ1));
             SF_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
             f->bub.resize(n);
             f->offsets.resize(n + 1);
@@ -928,10 +988,10 @@ extern "C" int sf_foam_use_graph(sf_foam* f, int32_t enable) {
     });
 }
 
-extern "C" int sf_foam_autotune(sf_foam* f, int32_t trial_steps, int32_t* chosen) {
+extern "C" int sf_foam_autotune(sf_foam* f, int32_t trial_steps, int32_t* chosen, double* trial_ms) {
     return guard([&] {
         SF_REQUIRE(f != nullptr && trial_steps > 0, SF_ERR_INVALID, "bad arguments");
-        F(f)->autotune(trial_steps, chosen);
+        F(f)->autotune(trial_steps, chosen, trial_ms);
     });
 }
 
@@ -939,7 +999,8 @@ extern "C" int sf_foam_set_tile_shapes(sf_foam* f, const int32_t* shapes) {
     return guard([&] {
         SF_REQUIRE(f != nullptr && shapes != nullptr, SF_ERR_INVALID, "NULL argument");
         for (int k = 0; k < 4; ++k)
-            SF_REQUIRE(shapes[k] >= 0 && shapes[k] < 4, SF_ERR_INVALID, "tile shape %d out of range", shapes[k]);
+            SF_REQUIRE(shapes[k] >= 0 && shapes[k] < kShapes, SF_ERR_INVALID, "tile shape %d out of range",
+                       shapes[k]);
         Foam* fm = F(f);
         fm->materialize();              // pole partials change layout with the shapes
         std::copy(shapes, shapes + 4, fm->variant_of_bucket);
