@@ -36,7 +36,11 @@ sys.path.insert(0, ROOT)
 METRIC = "drainage cell-steps/s plain vs surrogate at ≤1% rel-L2 h; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STEP = 16.0       # h and surfactant, FP32, read + write
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json")
+PMC_SUMMARY = {
+    "c2": os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json"),
+    "c2-drainage": os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json"),
+    "c3": os.path.join(ROOT, "profiles", "foam_step_4096_bubbles_pmc.json"),
+}
 
 
 def parse():
@@ -47,6 +51,8 @@ def parse():
     ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3"])
     ap.add_argument("--bubbles", type=int, default=4096, help="foam size of the c3 workload")
     ap.add_argument("--no-autotune", action="store_true", help="c3: keep the default tile shapes")
+    ap.add_argument("--tile-shapes", default=None,
+                    help="c3: tile shape index per size bucket, e.g. 0,0,0,0 (skips autotuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -63,11 +69,11 @@ def copy_probe(L, nbytes=256 << 20, iters=20):
     return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
-def pmc_traffic():
-    """HBM bytes per drainage launch from the committed rocprofv3 --pmc
+def pmc_traffic(workload):
+    """HBM bytes per step-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE, WRITE_SIZE per dispatch, scaled by the copy probe)."""
     try:
-        with open(PMC_SUMMARY) as f:
+        with open(PMC_SUMMARY[workload]) as f:
             d = json.load(f)
         return float(d["traffic_bytes_per_launch"]), d
     except (OSError, ValueError, KeyError):
@@ -150,7 +156,11 @@ def bench_foam(args, L, W):
     sizes = [b.n_theta * b.n_phi for b in bubbles]
     h = W.foam_noise(fm.offsets, sizes, [b.h0 for b in bubbles], seed=6, dtype=np.float32)
     fm.set_fields_f32(h)
-    chosen = None if args.no_autotune else fm.autotune(trial_steps=5)
+    chosen, trial = None, None
+    if args.tile_shapes:
+        fm.set_tile_shapes([int(v) for v in args.tile_shapes.split(",")])
+    elif not args.no_autotune:
+        chosen, trial = fm.autotune(trial_steps=5, return_times=True)
     fm.step(max(1, args.warmup))
     fm.synchronize()
     fm.kernel_timing(True)
@@ -175,9 +185,12 @@ def bench_foam(args, L, W):
                                "U[1,5] mPa s, per-bubble CFL dt, Γ transport, 16-step HIP graphs"
                                % len(bubbles),
                    "bubbles": len(bubbles), "cells": cells, "items_per_step": info["items"],
-                   "tile_shapes": info["variants"], "autotuned": chosen is not None},
+                   "tile_shapes": info["variants"], "autotuned": chosen is not None,
+                   "autotune_trial_ms": None if trial is None else [[round(float(x), 4) for x in row]
+                                                                     for row in trial]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "foam_step_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("c3")[0],
+                     "kernel": "foam_step_kernel",
                      "avg_launch_ms": k_ms, "launches_timed": k_n,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells},
         "state": {"mass_m3_total": float(st["mass"].sum()), "min_h": float(st["min"].min()),
@@ -280,7 +293,7 @@ def main():
     value = cells * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
     achieved = BYTES_PER_CELL_STEP * cells / (drain_ms * 1e-3) / 1e9
-    traffic, pmc = pmc_traffic()
+    traffic, pmc = pmc_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "drainage_step_kernel", "avg_launch_ms": drain_ms,

@@ -256,8 +256,8 @@ int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
 
 /* ---- batched foam --------------------------------------------------------
  * Many bubbles, each with its own grid, radius, viscosity and time step,
- * advanced together in one launch per step (a 16-step HIP graph is replayed
- * for long runs).  The reference keeps a scalar thickness per foam bubble
+ * advanced together: one kernel launch per code path in use per step (a
+ * 16-step HIP graph is replayed for long runs).  The reference keeps a scalar thickness per foam bubble
  * (foam.rs:32-49); a foam context gives every bubble the drainage field of
  * DrainageSimulator::step (drainage.rs:298-392) with the surfactant
  * transport of drainage.wgsl:178-201: h poles are ring means
@@ -299,13 +299,17 @@ int sf_foam_step(sf_foam* foam, int32_t n_steps);
 int sf_foam_synchronize(sf_foam* foam);
 /* Replays 16-step HIP graphs for long step counts (default on). */
 int sf_foam_use_graph(sf_foam* foam, int32_t enable);
-/* Times every tile shape (1 or 2 columns per lane x 4 or 8 rows per wave
- * item) on each bubble-size bucket (n_phi <= 96, <= 192, <= 384, > 384) for
- * trial_steps steps on a copy of the state, keeps the fastest, and restores
- * the state.  chosen[4] (may be NULL) receives the shape index per bucket:
- * 0 = (1,4), 1 = (1,8), 2 = (2,4), 3 = (2,8). */
-int sf_foam_autotune(sf_foam* foam, int32_t trial_steps, int32_t* chosen);
-/* Sets the tile shape index (0-3, as above) of each of the 4 buckets. */
+/* Tile shapes: a wave streams a band of rows of one column chunk.
+ * index:            0   1   2   3   4   5   6   7
+ * columns per lane: 1   1   2   2   2   4   4   4
+ * rows per band:    16  32  16  32  64  16  32  64
+ * Times every shape on each bubble-size bucket (n_phi <= 96, <= 192, <= 384,
+ * > 384) for trial_steps steps on a copy of the state, keeps the fastest per
+ * bucket and restores the state.  chosen[4] receives the shape per bucket,
+ * trial_ms[4][8] the average step-kernel time of each trial (-1 for buckets
+ * without bubbles); either may be NULL. */
+int sf_foam_autotune(sf_foam* foam, int32_t trial_steps, int32_t* chosen, double* trial_ms);
+/* Sets the tile shape index (0-7, as above) of each of the 4 buckets. */
 int sf_foam_set_tile_shapes(sf_foam* foam, const int32_t* shapes);
 /* dt[n_bubbles]: each bubble's time step. */
 int sf_foam_bubble_dt(sf_foam* foam, double* dt);

@@ -92,17 +92,24 @@ __global__ __launch_bounds__(256) void stats_partial_kernel(const float* hi, con
     }
 }
 
-__global__ void stats_final_kernel(const StatsPart* part, int nparts, StatsPart* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave folds the per-block partials.
+__global__ __launch_bounds__(64) void stats_final_kernel(const StatsPart* part, int nparts, StatsPart* out) {
     StatsPart r{INFINITY, -INFINITY, 0.0, 0.0, 0, 0};
-    for (int b = 0; b < nparts; ++b) {
+    for (int b = threadIdx.x; b < nparts; b += 64) {
         r.mn = fmin(r.mn, part[b].mn);
         r.mx = fmax(r.mx, part[b].mx);
         r.mass += part[b].mass;
         r.lam = fmax(r.lam, part[b].lam);
         r.nrupt += part[b].nrupt;
     }
-    *out = r;
+    for (int off = 32; off > 0; off >>= 1) {
+        r.mn = fmin(r.mn, __shfl_down(r.mn, off));
+        r.mx = fmax(r.mx, __shfl_down(r.mx, off));
+        r.mass += __shfl_down(r.mass, off);
+        r.lam = fmax(r.lam, __shfl_down(r.lam, off));
+        r.nrupt += __shfl_down(r.nrupt, off);
+    }
+    if (threadIdx.x == 0) *out = r;
 }
 
 __global__ void gather_kernel(const float* hi, const float* lo, const int64_t* idx, int n,

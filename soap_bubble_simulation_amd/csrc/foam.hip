@@ -2,25 +2,28 @@
 //
 // Every bubble is its own latitude-longitude grid (n_theta x n_phi, any
 // even n_phi) with its own radius, viscosity and time step; all of them
-// advance together in one persistent launch per step.  The state of all
-// bubbles lives in shared concatenated arrays (same float-float h + FP32 Γ
-// layout as a single bubble, see drainage_kernels.hpp), so one launch
-// streams the whole foam.
+// advance together, one step of every bubble per launch sequence.  The
+// state of all bubbles lives in shared concatenated arrays (the same
+// float-float h + FP32 Γ layout as a single bubble, see
+// drainage_kernels.hpp), so a step streams the whole foam once.
 //
-// Work is cut into wave-sized items: one item is (bubble, band of RB rows,
-// chunk of 64 * CPL columns) and one wave processes it.  The item shape per
-// bubble (CPL in {1, 2} columns per lane, RB in {4, 8} rows) is chosen per
-// size bucket by the autotuner in this file.  Items are ordered largest
-// first and dealt round-robin to a grid sized to the GPU, so every wave
-// loops over its items and leaves when they are done.  A run of 16 steps
-// (one full period of the ping-pong and rebase buffers) is captured in a
-// HIP graph and replayed.
+// Work is cut into wave items: one item is a band of up to 16-64 rows of a
+// chunk of 64 x CPL columns of one bubble, streamed top to bottom by one
+// wave through a five-row register window (north, centre, south, two rows in
+// flight).  The tile shape (CPL in {1, 2, 4} columns per lane, band height)
+// is chosen per bubble-size bucket by the autotuner below.  Items are
+// grouped by code path (lane width, vector access) and each path is its own
+// kernel, so that each runs at the occupancy of its own register count;
+// within a path items keep their natural order (bubble, band, chunk) and
+// blocks are renumbered per XCD, so that neighbouring items share halo rows
+// and edge columns in one L2.  A run of 16 steps (one period of the
+// ping-pong and rebase buffers) is captured in a HIP graph and replayed.
 //
 // Physics per cell is the drainage step of drainage.rs:298-392 with the
 // surfactant transport of drainage.wgsl:178-201: h poles are ring means, Γ
-// poles are copied through, no marginal-regeneration fronts (foam bubbles
-// have no per-bubble front schedule in the reference: foam.rs:32-49 keeps a
-// scalar thickness per bubble).
+// poles are copied through, no marginal-regeneration fronts (the reference
+// keeps a scalar thickness per foam bubble, foam.rs:32-49, so there is no
+// per-bubble front schedule to follow).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -300,7 +303,12 @@ __global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a, const 
     const int lane = threadIdx.x & 63;
     // The item index is wave-uniform; readfirstlane lets the compiler keep it
     // (and the item and bubble records) in scalar registers.
-    const int k = __builtin_amdgcn_readfirstlane(static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    // Blocks are dealt to the 8 XCDs round-robin; renumbering them gives each
+    // XCD a contiguous run of items, so neighbouring bands and chunks (their
+    // halo rows and edge columns) meet in the same L2.
+    const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    const int v = (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
+    const int k = __builtin_amdgcn_readfirstlane(v * 4 + static_cast<int>(threadIdx.x >> 6));
     if (k >= n) return;
     const FoamItem it = items[k];
     const FoamBubble B = a.bub[it.bubble];
@@ -536,7 +544,7 @@ struct Foam {
     void launch_path(bool rebase, const FoamArgs& a) {
         const int n = path_begin[PATH + 1] - path_begin[PATH];
         if (n == 0) return;
-        const dim3 grid(cdivu(static_cast<size_t>(n), 4)), blk(256);
+        const dim3 grid((cdivu(static_cast<size_t>(n), 4) + kXcds - 1) / kXcds * kXcds), blk(256);
         const FoamItem* items = d_items + path_begin[PATH];
         if (rebase)
             hipLaunchKernelGGL((foam_step_kernel<PATH, true>), grid, blk, 0, stream, a, items, n);

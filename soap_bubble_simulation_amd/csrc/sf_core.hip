@@ -34,11 +34,22 @@ namespace sf {
 
 // Plain float4 copy: the reference kernel for achievable HBM bandwidth and
 // for scaling the FETCH_SIZE / WRITE_SIZE counters of the other kernels.
+// Streaming copy used to calibrate bandwidth and the HBM counters: every
+// thread keeps four independent 16-byte loads in flight per iteration.
+constexpr int kCopyUnroll = 4;
+
 __global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src,
                                                           float4* __restrict__ dst, size_t n4) {
-    for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n4;
-         k += static_cast<size_t>(gridDim.x) * blockDim.x)
-        dst[k] = src[k];
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    for (; k + (kCopyUnroll - 1) * stride < n4; k += kCopyUnroll * stride) {
+        float4 v[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) v[u] = src[k + u * stride];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) dst[k + u * stride] = v[u];
+    }
+    for (; k < n4; k += stride) dst[k] = src[k];
 }
 
 }  // namespace sf
