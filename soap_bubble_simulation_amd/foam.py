@@ -18,7 +18,8 @@ import numpy as np
 
 from . import _lib as L
 
-VARIANTS = ((1, 4), (1, 8), (2, 4), (2, 8))   # (columns per lane, rows per wave item)
+# Tile shapes (columns per lane, rows per band) by index, as in sf_foam_autotune.
+TILE_SHAPES = ((1, 16), (1, 32), (2, 16), (2, 32), (2, 64), (4, 16), (4, 32), (4, 64))
 
 
 @dataclass
@@ -122,9 +123,14 @@ class FoamDrainage:
     def use_graph(self, enable: bool) -> None:
         L.check(self._lib.sf_foam_use_graph(self._h, 1 if enable else 0))
 
-    def autotune(self, trial_steps: int = 8) -> List[int]:
+    def autotune(self, trial_steps: int = 8, return_times: bool = False):
+        """Picks the fastest tile shape per size bucket; optionally also
+        returns the trial kernel times [bucket][shape] in ms (-1: empty)."""
         chosen = (C.c_int32 * 4)()
-        L.check(self._lib.sf_foam_autotune(self._h, int(trial_steps), chosen))
+        times = np.empty(4 * len(TILE_SHAPES), np.float64)
+        L.check(self._lib.sf_foam_autotune(self._h, int(trial_steps), chosen, L.dptr(times)))
+        if return_times:
+            return list(chosen), times.reshape(4, len(TILE_SHAPES))
         return list(chosen)
 
     def set_tile_shapes(self, shapes: Sequence[int]) -> None:

@@ -15,10 +15,12 @@ from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_param
 
 pytestmark = pytest.mark.gpu
 
-# Grids chosen to hit every code path: n_phi below one wave (4, 6, 40),
-# ragged last chunks for both lane widths (130, 200), several chunks (520),
-# a bubble with a single interior row (3 x 4), each size bucket.
-SHAPES = [(3, 4), (9, 6), (20, 40), (33, 66), (65, 130), (100, 200), (64, 128), (260, 520)]
+# Grids chosen to hit every code path: n_phi below one wave (2, 4, 6, 40),
+# ragged last chunks for every lane width (130, 200), several chunks (520),
+# n_phi = 2 mod 4 (two-float2 path of 4-column lanes: 6, 66, 130), several
+# bands (98 and 258 interior rows), a bubble with a single interior row
+# (3 x 4), and every size bucket.
+SHAPES = [(3, 4), (5, 2), (9, 6), (20, 40), (33, 66), (65, 130), (100, 200), (64, 128), (260, 520)]
 
 
 def rel_l2(a, b):
@@ -52,7 +54,7 @@ def seeded_fields(fm, seed, rel=0.05):
     return h, g
 
 
-@pytest.mark.parametrize("shape_idx", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape_idx", range(8))
 def test_foam_trajectory_matches_oracle(oracle, shape_idx):
     fm, p = make_foam(SHAPES, seed=10 + shape_idx)
     fm.set_tile_shapes([shape_idx] * 4)
@@ -117,7 +119,7 @@ def test_foam_autotune_keeps_state_and_shapes_agree():
     fm.step(5)
     before = fm.fields()
     chosen = fm.autotune(trial_steps=3)
-    assert all(0 <= c < 4 for c in chosen)
+    assert all(0 <= c < 8 for c in chosen)
     after = fm.fields()
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     assert fm.info()["variants"] == chosen
