@@ -8,6 +8,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "drainage.hpp"
 
 namespace sf {
@@ -171,16 +173,18 @@ static void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&);
+// Step launchers; t0/t1 (may be null) are events the runtime records at the
+// kernel's own start and end (hipExtLaunchKernelGGL), for kernel timing.
+using StepFn = void (*)(dim3, dim3, hipStream_t, hipEvent_t, hipEvent_t, const StepArgs&);
 
 template <bool RB, int GM, bool M, bool D>
-static void launch_step(dim3 grid, dim3 block, hipStream_t s, const StepArgs& a) {
-    hipLaunchKernelGGL((drainage_step_kernel<RB, GM, M, D>), grid, block, 0, s, a);
+static void launch_step(dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const StepArgs& a) {
+    hipExtLaunchKernelGGL((drainage_step_kernel<RB, GM, M, D>), grid, block, 0, s, t0, t1, 0, a);
 }
 
 template <bool RB, int GM, bool M, bool D, int ROWS>
-static void launch_vec4(dim3 grid, dim3 block, hipStream_t s, const StepArgs& a) {
-    hipLaunchKernelGGL((drainage_vec4_kernel<RB, GM, M, D, ROWS, kPrefetchRows>), grid, block, 0, s, a);
+static void launch_vec4(dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const StepArgs& a) {
+    hipExtLaunchKernelGGL((drainage_vec4_kernel<RB, GM, M, D, ROWS, kPrefetchRows>), grid, block, 0, s, t0, t1, 0, a);
 }
 
 // Kernel table indexed by [rebase][gmode][marangoni][diag].  The scalar
@@ -559,11 +563,9 @@ void Drainage::step(int n_steps, double dt, bool flush) {
         const StepFn fn = step_fn(rebase, gmode, marangoni, diag, vec4, rows_per_block);
         if (step_timer.enabled) {
             auto& ev = step_timer.next(stream);
-            SF_HIP(hipEventRecord(ev.first, stream));
-            fn(grid, blk, stream, a);
-            SF_HIP(hipEventRecord(ev.second, stream));
+            fn(grid, blk, stream, ev.first, ev.second, a);
         } else {
-            fn(grid, blk, stream, a);
+            fn(grid, blk, stream, nullptr, nullptr, a);
         }
         SF_HIP(hipGetLastError());
         cur = 1 - cur;

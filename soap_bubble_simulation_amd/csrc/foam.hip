@@ -30,6 +30,8 @@
 #include <numeric>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "drainage.hpp"
 
 namespace sf {
@@ -540,24 +542,32 @@ struct Foam {
         return a;
     }
 
+    // t0 / t1 (may be null): events recorded at the kernel's own start / end.
     template <int PATH>
-    void launch_path(bool rebase, const FoamArgs& a) {
+    void launch_path(bool rebase, const FoamArgs& a, hipEvent_t t0, hipEvent_t t1) {
         const int n = path_begin[PATH + 1] - path_begin[PATH];
         if (n == 0) return;
         const dim3 grid((cdivu(static_cast<size_t>(n), 4) + kXcds - 1) / kXcds * kXcds), blk(256);
         const FoamItem* items = d_items + path_begin[PATH];
         if (rebase)
-            hipLaunchKernelGGL((foam_step_kernel<PATH, true>), grid, blk, 0, stream, a, items, n);
+            hipExtLaunchKernelGGL((foam_step_kernel<PATH, true>), grid, blk, 0, stream, t0, t1, 0, a, items, n);
         else
-            hipLaunchKernelGGL((foam_step_kernel<PATH, false>), grid, blk, 0, stream, a, items, n);
+            hipExtLaunchKernelGGL((foam_step_kernel<PATH, false>), grid, blk, 0, stream, t0, t1, 0, a, items, n);
     }
 
-    // One step: a launch per code path in use.
-    void launch_one(bool rebase, const FoamArgs& a) {
-        launch_path<0>(rebase, a);
-        launch_path<1>(rebase, a);
-        launch_path<2>(rebase, a);
-        launch_path<3>(rebase, a);
+    // One step: a launch per code path in use; t0 is recorded at the start
+    // of the first kernel and t1 at the end of the last.
+    void launch_one(bool rebase, const FoamArgs& a, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+        int first = -1, last = -1;
+        for (int p = 0; p < kPaths; ++p)
+            if (path_begin[p + 1] > path_begin[p]) {
+                if (first < 0) first = p;
+                last = p;
+            }
+        launch_path<0>(rebase, a, first == 0 ? t0 : nullptr, last == 0 ? t1 : nullptr);
+        launch_path<1>(rebase, a, first == 1 ? t0 : nullptr, last == 1 ? t1 : nullptr);
+        launch_path<2>(rebase, a, first == 2 ? t0 : nullptr, last == 2 ? t1 : nullptr);
+        launch_path<3>(rebase, a, first == 3 ? t0 : nullptr, last == 3 ? t1 : nullptr);
     }
 
     // One step with the current buffers; advances the ping-pong state.
@@ -566,9 +576,7 @@ struct Foam {
         const FoamArgs a = args_for(cur, cur_hi, cur_pole, poles_virtual);
         if (timer.enabled) {
             auto& ev = timer.next(stream);
-            SF_HIP(hipEventRecord(ev.first, stream));
-            launch_one(rebase, a);
-            SF_HIP(hipEventRecord(ev.second, stream));
+            launch_one(rebase, a, ev.first, ev.second);
         } else {
             launch_one(rebase, a);
         }

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "drainage.hpp"
 
 namespace sf {
@@ -646,9 +648,7 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 20));
     if (d->spec_timer.enabled) {
         auto& ev = d->spec_timer.next(d->stream);
-        SF_HIP(hipEventRecord(ev.first, d->stream));
-        hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, a);
-        SF_HIP(hipEventRecord(ev.second, d->stream));
+        hipExtLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, ev.first, ev.second, 0, a);
     } else {
         hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, a);
     }
