@@ -321,6 +321,98 @@ int sf_foam_stats(sf_foam* foam, double* mass, double* dt_cfl, double* hmin, dou
 int sf_foam_kernel_timing(sf_foam* foam, int32_t enable, double* avg_ms, int64_t* launches);
 /* Total cells, wave work items per step, and the tile shape per bucket. */
 int sf_foam_info(sf_foam* foam, int64_t* total_cells, int32_t* n_items, int32_t* variants);
+/* Only bubbles with mask[b] != 0 are stepped from now on (NULL: all); the
+ * others keep their state and time until they are stepped again. */
+int sf_foam_set_active(sf_foam* foam, const uint8_t* mask);
+/* Replaces the state from float-float h (hi + lo) and Γ arrays of the layout
+ * length, in host or device memory; Γ NULL keeps it.  Time steps are kept. */
+int sf_foam_set_state(sf_foam* foam, const float* hi, const float* lo, const float* gamma);
+/* Device pointers of the current state (pole rows written); valid until the
+ * next call that steps or changes the foam.  NULL outputs are skipped. */
+int sf_foam_device_state(sf_foam* foam, const float** hi, const float** lo, const float** gamma);
+/* out[n_bubbles]: the bubble descriptions with their resolved time steps. */
+int sf_foam_bubbles(sf_foam* foam, sf_bubble* out);
+/* The foam's HIP stream (hipStream_t). */
+int sf_foam_stream(sf_foam* foam, void** stream);
+/* Number of bubbles of the foam. */
+int sf_foam_num_bubbles(sf_foam* foam, int32_t* n);
+
+/* ---- learned surrogate (Fourier neural operator) --------------------------
+ * No reference counterpart: a network that advances a bubble's film
+ * thickness by `window` drainage steps, trained on trajectories of the plain
+ * solver (a foam context) with this library's own HIP forward and backward
+ * kernels (BF16 MFMA products, FP32 master weights, Adam), and checked every
+ * window by one plain step that acts as judge:
+ *   the network predicts h(t+K-1) and h(t+K); h_J = one plain step from
+ *   h(t+K-1); a bubble's window is accepted when
+ *   ||h_J - h(t+K)||_2 <= tol ||h(t+K)||_2 and its state becomes h_J,
+ *   otherwise it falls back to K plain steps from h(t).
+ * Input channels per cell: relative thickness perturbation, surfactant,
+ * sin(theta), the window's gravity-sink number K dt rho g h_ref^2 / (3 eta)
+ * and diffusion number K dt D / (R^2 dtheta^2) (h_ref = the bubble's mean h).
+ * Outputs: (h(t+j) - h(t)) / (h_ref s_y) for j = K-1, K.  Tensors are
+ * channel-last [batch][n_theta][n_phi][channels], FP32. */
+typedef struct sf_fno_config {
+    int32_t n_theta, n_phi;   /* grid of every bubble; n_theta*n_phi % 4096 == 0 */
+    int32_t width;            /* channels (divides 256, <= 64)                  */
+    int32_t modes_theta;      /* theta modes k in [-modes_theta, modes_theta)   */
+    int32_t modes_phi;        /* phi modes m in [0, modes_phi)                  */
+    int32_t layers;           /* Fourier layers                                 */
+    int32_t window;           /* K, steps per inference                         */
+    int32_t max_batch;        /* bubbles per forward/backward pass              */
+    uint64_t seed;            /* weight initialisation                          */
+} sf_fno_config;
+
+typedef struct sf_fno_stats {
+    int64_t windows, bubble_windows, accepted, fallbacks;
+    double max_residual;            /* max over bubble-windows of the judge residual   */
+    double sum_residual;
+    double max_increment_residual;  /* same error relative to ||h(t+K) - h(t)||        */
+    double surrogate_ms, judge_ms, fallback_ms;   /* host wall clock per phase (sums) */
+} sf_fno_stats;
+
+typedef struct sf_fno sf_fno;
+
+/* 256 x 512 grid, width 32, 16 x 16 modes, 4 layers, K = 16, batch 16. */
+int sf_fno_config_default(sf_fno_config* config);
+int sf_fno_create(const sf_fno_config* config, sf_fno** out);
+void sf_fno_destroy(sf_fno* model);
+int sf_fno_num_params(sf_fno* model, int64_t* n);
+/* Flat FP32 parameters: lift W[5][width], b[width]; per layer W[width][width],
+ * b[width], R[2 modes_theta][modes_phi][2][width][width] (re, im); projection
+ * Q[width][2], q[2]. */
+int sf_fno_get_params(sf_fno* model, float* params);
+int sf_fno_set_params(sf_fno* model, const float* params);
+/* scales[2] = {s_x, s_y}: input perturbation and target scales (set from the
+ * first training window when unset). */
+int sf_fno_get_scales(sf_fno* model, double* scales);
+int sf_fno_set_scales(sf_fno* model, const double* scales);
+/* Host-array forward pass: in[batch][N][5] -> out[batch][N][2]. */
+int sf_fno_forward(sf_fno* model, int32_t batch, const float* in, float* out);
+/* Loss mean((out - target)^2) and its gradient w.r.t. every parameter. */
+int sf_fno_gradient(sf_fno* model, int32_t batch, const float* in, const float* target, double* loss,
+                    float* grad);
+/* One Adam step (lr, beta 0.9 / 0.999) on a host batch; returns the loss. */
+int sf_fno_train_step(sf_fno* model, int32_t batch, const float* in, const float* target, double lr,
+                      double* loss);
+/* `windows` training windows generated online by the foam's plain solver
+ * (every bubble n_theta x n_phi); losses[windows] (may be NULL). */
+int sf_fno_train_foam(sf_fno* model, sf_foam* foam, const sf_params* params, int32_t windows, double lr,
+                      double* losses);
+/* Predictors of sf_fno_advance_foam: the network, or the no-network
+ * baselines judged the same way: h(t) itself, or the previous window's
+ * increment carried on linearly (identity on a call's first window). */
+#define SF_FNO_PREDICT_NETWORK      0
+#define SF_FNO_PREDICT_IDENTITY     1
+#define SF_FNO_PREDICT_EXTRAPOLATE  2
+/* `windows` surrogate windows with judge and fallback on every bubble;
+ * stats (may be NULL) are accumulated. */
+int sf_fno_advance_foam(sf_fno* model, sf_foam* foam, const sf_params* params, int32_t windows, double tol,
+                        int32_t predictor, sf_fno_stats* stats);
+/* Input features of bubbles [first, first + count) of the foam's current
+ * state into in_host[count][N][5]. */
+int sf_fno_features(sf_fno* model, sf_foam* foam, const sf_params* params, int32_t first, int32_t count,
+                    float* in_host);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,10 @@ FLAG_POLES_COPY_H = 1 << 3
 FLAG_POLES_AVG_GAMMA = 1 << 4
 FLAG_DIAG = 1 << 5
 
+PREDICT_NETWORK = 0
+PREDICT_IDENTITY = 1
+PREDICT_EXTRAPOLATE = 2
+
 
 class SoapFilmError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -89,6 +93,25 @@ class Bubble(C.Structure):
         ("viscosity", C.c_double),
         ("h0", C.c_double),
         ("dt", C.c_double),
+    ]
+
+
+class FnoConfig(C.Structure):
+    """Mirror of ``sf_fno_config``."""
+    _fields_ = [
+        ("n_theta", C.c_int32), ("n_phi", C.c_int32), ("width", C.c_int32),
+        ("modes_theta", C.c_int32), ("modes_phi", C.c_int32), ("layers", C.c_int32),
+        ("window", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64),
+    ]
+
+
+class FnoStats(C.Structure):
+    """Mirror of ``sf_fno_stats``."""
+    _fields_ = [
+        ("windows", C.c_int64), ("bubble_windows", C.c_int64), ("accepted", C.c_int64),
+        ("fallbacks", C.c_int64), ("max_residual", C.c_double), ("sum_residual", C.c_double),
+        ("max_increment_residual", C.c_double), ("surrogate_ms", C.c_double),
+        ("judge_ms", C.c_double), ("fallback_ms", C.c_double),
     ]
 
 
@@ -160,6 +183,27 @@ SIGNATURES = {
     "sf_foam_stats": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "sf_foam_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
     "sf_foam_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "sf_foam_set_active": (C.c_int, [_vp, _u8p]),
+    "sf_foam_set_state": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "sf_foam_device_state": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "sf_foam_bubbles": (C.c_int, [_vp, C.POINTER(Bubble)]),
+    "sf_foam_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sf_foam_num_bubbles": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "sf_fno_config_default": (C.c_int, [C.POINTER(FnoConfig)]),
+    "sf_fno_create": (C.c_int, [C.POINTER(FnoConfig), C.POINTER(_vp)]),
+    "sf_fno_destroy": (None, [_vp]),
+    "sf_fno_num_params": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "sf_fno_get_params": (C.c_int, [_vp, _fp]),
+    "sf_fno_set_params": (C.c_int, [_vp, _fp]),
+    "sf_fno_get_scales": (C.c_int, [_vp, _dp]),
+    "sf_fno_set_scales": (C.c_int, [_vp, _dp]),
+    "sf_fno_forward": (C.c_int, [_vp, C.c_int32, _fp, _fp]),
+    "sf_fno_gradient": (C.c_int, [_vp, C.c_int32, _fp, _fp, _dp, _fp]),
+    "sf_fno_train_step": (C.c_int, [_vp, C.c_int32, _fp, _fp, C.c_double, _dp]),
+    "sf_fno_train_foam": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_double, _dp]),
+    "sf_fno_advance_foam": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_double, C.c_int32,
+                                      C.POINTER(FnoStats)]),
+    "sf_fno_features": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_int32, _fp]),
 }
 
 _lib = None

@@ -1,0 +1,333 @@
+// fno_kernels.hpp -- device kernels of the Fourier neural operator surrogate.
+//
+// Tensors are FP32 in HBM, channel-last: activations [B][n_theta][n_phi][C].
+// Every contraction (truncated DFTs, inverse DFTs, pointwise channel mixing,
+// and the transposed products of the backward pass) runs through one
+// batched GEMM on BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32
+// accumulation; operands are rounded to BF16 (nearest even) as they are
+// staged into LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sf_common.hpp"
+
+namespace sf {
+namespace fno {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (inputs are finite)
+    return static_cast<unsigned short>(u >> 16);
+}
+
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major storage;
+// op(A)[m][k] = ta ? A[k * lda + m] : A[m * lda + k], likewise for B.
+struct GemmArgs {
+    const float* A;
+    const float* B;
+    float* C;
+    int M, N, K;
+    int lda, ldb, ldc;
+    long long sA, sB, sC;   // batch strides in elements
+    int ta, tb;
+    float alpha, beta;
+};
+
+constexpr int kTM = 64, kTN = 64, kTK = 16, kPad = 4;
+
+// Block: 4 waves, 64 x 64 tile of C; wave w owns the 32 x 32 quadrant
+// (w / 2, w % 2) as 2 x 2 MFMA tiles of 16 x 16.  Grid: (M tiles, N tiles,
+// batch).  MFMA lane layout (16x16x16, bf16): lane l supplies A[l % 16][4 (l / 16) + v]
+// and B[4 (l / 16) + v][l % 16] and receives D[4 (l / 16) + v][l % 16].
+__global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs g) {
+    __shared__ unsigned short As[kTM][kTK + kPad];   // [m][k]
+    __shared__ unsigned short Bs[kTN][kTK + kPad];   // [n][k]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
+    const long long bz = blockIdx.z;
+    const float* A = g.A + bz * g.sA;
+    const float* B = g.B + bz * g.sB;
+    float* C = g.C + bz * g.sC;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        if (!g.ta) {
+            const int r = tid >> 2, kq = (tid & 3) * 4, m = m0 + r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + kq + j;
+                const float v = (m < g.M && k < g.K) ? A[static_cast<long long>(m) * g.lda + k] : 0.f;
+                As[r][kq + j] = to_bf16(v);
+            }
+        } else {
+            const int kr = tid >> 4, mq = (tid & 15) * 4, k = k0 + kr;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = m0 + mq + j;
+                const float v = (m < g.M && k < g.K) ? A[static_cast<long long>(k) * g.lda + m] : 0.f;
+                As[mq + j][kr] = to_bf16(v);
+            }
+        }
+        if (!g.tb) {
+            const int kr = tid >> 4, nq = (tid & 15) * 4, k = k0 + kr;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + nq + j;
+                const float v = (n < g.N && k < g.K) ? B[static_cast<long long>(k) * g.ldb + n] : 0.f;
+                Bs[nq + j][kr] = to_bf16(v);
+            }
+        } else {
+            const int r = tid >> 2, kq = (tid & 3) * 4, n = n0 + r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + kq + j;
+                const float v = (n < g.N && k < g.K) ? B[static_cast<long long>(n) * g.ldb + k] : 0.f;
+                Bs[r][kq + j] = to_bf16(v);
+            }
+        }
+        __syncthreads();
+        const int kl = 4 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + i * 16 + (lane & 15)][kl]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + j * 16 + (lane & 15)][kl]);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                if (m < g.M && n < g.N) {
+                    float r = g.alpha * acc[i][j][v];
+                    const long long o = static_cast<long long>(m) * g.ldc + n;
+                    if (g.beta != 0.f) r += g.beta * C[o];
+                    C[o] = r;
+                }
+            }
+}
+
+// ---- elementwise -------------------------------------------------------------
+
+__device__ inline float gelu(float z) {      // tanh form
+    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+    return 0.5f * z * (1.f + tanhf(k0 * (z + k1 * z * z * z)));
+}
+__device__ inline float gelu_grad(float z) {
+    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+    const float u = k0 * (z + k1 * z * z * z);
+    const float t = tanhf(u);
+    return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * k0 * (1.f + 3.f * k1 * z * z);
+}
+
+// x_out = act(z), z = z_in + bias[c] (+ y), over n rows of C channels.
+__global__ void bias_act_kernel(float* z, const float* y, const float* bias, float* x_out, long long total, int C,
+                                int act) {
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < total;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        float v = z[k] + bias[k % C];
+        if (y) v += y[k];
+        z[k] = v;
+        x_out[k] = act ? gelu(v) : v;
+    }
+}
+
+// dz = dx * act'(z) (in place on dx when act, else a copy).
+__global__ void act_grad_kernel(const float* z, const float* dx, float* dz, long long total, int act) {
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < total;
+         k += static_cast<long long>(gridDim.x) * blockDim.x)
+        dz[k] = act ? dx[k] * gelu_grad(z[k]) : dx[k];
+}
+
+// out[j] = sum over rows of part[row][j] (j < cols), FP64 accumulation; one
+// thread per column, rows strided (small column counts).
+__global__ void column_sum_kernel(const float* part, long long rows, int cols, float* out, int accumulate) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    double s = 0.0;
+    for (long long r = 0; r < rows; ++r) s += part[r * cols + j];
+    out[j] = accumulate ? out[j] + static_cast<float>(s) : static_cast<float>(s);
+}
+
+// Per-block column sums of a [rows][C] tensor: partial[block][C].
+__global__ __launch_bounds__(256) void column_partial_kernel(const float* x, long long rows, int C, float* partial) {
+    // 256 threads: thread t sums column (t % C) over rows congruent to t / C.
+    const int per = 256 / C;
+    const int c = threadIdx.x % C, lane_row = threadIdx.x / C;
+    double s = 0.0;
+    if (lane_row < per) {
+        for (long long r = blockIdx.x * static_cast<long long>(per) + lane_row; r < rows;
+             r += static_cast<long long>(gridDim.x) * per)
+            s += x[r * C + c];
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        double t = 0.0;
+        for (int q = 0; q < per; ++q) t += red[q * C + threadIdx.x];
+        partial[blockIdx.x * C + threadIdx.x] = static_cast<float>(t);
+    }
+}
+
+// Complex combine after the theta DFT.  G[b] = [Er; Ei] (2Mt x T) * X1[b]
+// (T x (2 Mp C)), laid out [2][Mt][2][Mp][C] as (E part, k, re/im of X1, m, c).
+// Xs[b][k][m][2][c] = (Er Xr - Ei Xi, Er Xi + Ei Xr).
+__global__ void theta_combine_kernel(const float* G, float* Xs, int Mt, int Mp, int C, long long total) {
+    for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
+         q += static_cast<long long>(gridDim.x) * blockDim.x) {
+        // q indexes [b][k][m][c]
+        const int c = static_cast<int>(q % C);
+        long long r = q / C;
+        const int m = static_cast<int>(r % Mp);
+        r /= Mp;
+        const int k = static_cast<int>(r % Mt);
+        const long long b = r / Mt;
+        const long long base = b * (2LL * Mt * 2 * Mp * C);
+        auto at = [&](int part, int ri) { return G[base + ((static_cast<long long>(part) * Mt + k) * 2 + ri) * Mp * C + static_cast<long long>(m) * C + c]; };
+        const float re = at(0, 0) - at(1, 1);
+        const float im = at(0, 1) + at(1, 0);
+        const long long o = (((b * Mt + k) * Mp + m) * 2) * C + c;
+        Xs[o] = re;
+        Xs[o + C] = im;
+    }
+}
+
+// Adjoint of theta_combine_kernel: dG from dXs.
+__global__ void theta_combine_adjoint_kernel(const float* dXs, float* dG, int Mt, int Mp, int C, long long total) {
+    for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
+         q += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int c = static_cast<int>(q % C);
+        long long r = q / C;
+        const int m = static_cast<int>(r % Mp);
+        r /= Mp;
+        const int k = static_cast<int>(r % Mt);
+        const long long b = r / Mt;
+        const long long o = (((b * Mt + k) * Mp + m) * 2) * C + c;
+        const float dre = dXs[o], dim = dXs[o + C];
+        const long long base = b * (2LL * Mt * 2 * Mp * C);
+        auto at = [&](int part, int ri) -> float& { return dG[base + ((static_cast<long long>(part) * Mt + k) * 2 + ri) * Mp * C + static_cast<long long>(m) * C + c]; };
+        at(0, 0) = dre;
+        at(1, 1) = -dre;
+        at(0, 1) = dim;
+        at(1, 0) = dim;
+    }
+}
+
+// Spectral mixing per retained mode (k, m): Y[b][k][m][o] = sum_c Xs[b][k][m][c] R[k][m][c][o]
+// (complex).  Xs: [B][Mt][Mp][2][C]; R: [Mt][Mp][2][C][C]; Y written in the
+// layout of the inverse theta GEMM: [B][2][Mt][Mp][C].  One block per mode.
+__global__ void mode_mix_kernel(const float* Xs, const float* R, float* Y, int Bn, int Mt, int Mp, int C) {
+    const int mode = blockIdx.x;               // k * Mp + m
+    const float* Rm = R + static_cast<long long>(mode) * 2 * C * C;
+    for (int q = threadIdx.x; q < Bn * C; q += blockDim.x) {
+        const int b = q / C, o = q % C;
+        const float* x = Xs + (static_cast<long long>(b) * Mt * Mp + mode) * 2 * C;
+        float re = 0.f, im = 0.f;
+        for (int c = 0; c < C; ++c) {
+            const float xr = x[c], xi = x[C + c];
+            const float rr = Rm[c * C + o], ri = Rm[C * C + c * C + o];
+            re += xr * rr - xi * ri;
+            im += xr * ri + xi * rr;
+        }
+        const long long plane = static_cast<long long>(Mt) * Mp * C;
+        const long long o_re = static_cast<long long>(b) * 2 * plane + static_cast<long long>(mode) * C + o;
+        Y[o_re] = re;
+        Y[o_re + plane] = im;
+    }
+}
+
+// Backward of mode_mix_kernel: dXs (overwritten) and dR (accumulated).
+__global__ void mode_mix_grad_kernel(const float* Xs, const float* R, const float* dY, float* dXs, float* dR, int Bn,
+                                     int Mt, int Mp, int C) {
+    const int mode = blockIdx.x;
+    const float* Rm = R + static_cast<long long>(mode) * 2 * C * C;
+    float* dRm = dR + static_cast<long long>(mode) * 2 * C * C;
+    const long long plane = static_cast<long long>(Mt) * Mp * C;
+    // dX[b][c] = sum_o (dYr R_re + dYi R_im, -dYr R_im + dYi R_re)
+    for (int q = threadIdx.x; q < Bn * C; q += blockDim.x) {
+        const int b = q / C, c = q % C;
+        const float* dyr = dY + static_cast<long long>(b) * 2 * plane + static_cast<long long>(mode) * C;
+        const float* dyi = dyr + plane;
+        float xr = 0.f, xi = 0.f;
+        for (int o = 0; o < C; ++o) {
+            const float rr = Rm[c * C + o], ri = Rm[C * C + c * C + o];
+            xr += dyr[o] * rr + dyi[o] * ri;
+            xi += -dyr[o] * ri + dyi[o] * rr;
+        }
+        float* dx = dXs + (static_cast<long long>(b) * Mt * Mp + mode) * 2 * C;
+        dx[c] = xr;
+        dx[C + c] = xi;
+    }
+    // dR[c][o] += sum_b (dYr Xr + dYi Xi, -dYr Xi + dYi Xr)
+    for (int q = threadIdx.x; q < C * C; q += blockDim.x) {
+        const int c = q / C, o = q % C;
+        float gr = 0.f, gi = 0.f;
+        for (int b = 0; b < Bn; ++b) {
+            const float* x = Xs + (static_cast<long long>(b) * Mt * Mp + mode) * 2 * C;
+            const float dyr = dY[static_cast<long long>(b) * 2 * plane + static_cast<long long>(mode) * C + o];
+            const float dyi = dY[static_cast<long long>(b) * 2 * plane + plane + static_cast<long long>(mode) * C + o];
+            gr += dyr * x[c] + dyi * x[C + c];
+            gi += -dyr * x[C + c] + dyi * x[c];
+        }
+        dRm[c * C + o] += gr;
+        dRm[C * C + c * C + o] += gi;
+    }
+}
+
+// Adam on FP32 master weights (bias-corrected step size passed in).
+__global__ void adam_kernel(float* w, const float* g, float* m, float* v, long long n, float lr_t, float b1, float b2,
+                            float eps) {
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const float gk = g[k];
+        const float mk = b1 * m[k] + (1.f - b1) * gk;
+        const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+        m[k] = mk;
+        v[k] = vk;
+        w[k] -= lr_t * mk / (sqrtf(vk) + eps);
+    }
+}
+
+// Loss L = mean((out - target)^2) pieces: dout = 2 (out - target) / count and
+// per-block partial sums of the squared error (FP64).
+__global__ __launch_bounds__(256) void mse_kernel(const float* out, const float* target, float* dout, long long n,
+                                                  double inv_count, double* partial) {
+    double s = 0.0;
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const double d = static_cast<double>(out[k]) - static_cast<double>(target[k]);
+        s += d * d;
+        dout[k] = static_cast<float>(2.0 * d * inv_count);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+}  // namespace fno
+}  // namespace sf

@@ -44,7 +44,7 @@ struct FoamBubble {
     int nchunks;          // column chunks of the bubble's tile shape
     int variant;          // tile shape index (kShapeCpl, kShapeBand)
     int path;             // 0: 1 column per lane, 1: 2, 2: 4 (float4), 3: 4 (two float2)
-    int pad_;
+    int active;           // stepped (1) or held (0), see Foam::set_active
     float dt;             // s
     float dtD;            // dt D_h / R^2
     float dtDs;           // dt D_s / R^2
@@ -323,6 +323,7 @@ __global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a, const 
 // Writes every bubble's pole rows from the ring partials (one block per bubble).
 __global__ void foam_poles_kernel(float* hi, float* lo, const float2* hpole, const FoamBubble* bub) {
     const FoamBubble B = bub[blockIdx.x];
+    if (!B.active) return;      // held bubbles have no fresh ring partials
     const float2 t = pole_mean(hpole + B.pole_off, B.nchunks, B.np_f);
     const float2 b = pole_mean(hpole + B.pole_off + B.nchunks, B.nchunks, B.np_f);
     for (int j = threadIdx.x; j < B.np; j += blockDim.x) {
@@ -409,6 +410,7 @@ int bucket_of(int np) { return np <= 96 ? 0 : (np <= 192 ? 1 : (np <= 384 ? 2 : 
 struct Foam {
     sf_params p;
     int nb = 0;
+    std::vector<uint8_t> active;        // per bubble: stepped (1) or held (0)
     std::vector<sf_bubble> desc;
     std::vector<FoamBubble> bub;
     std::vector<int64_t> offsets;
@@ -493,7 +495,9 @@ struct Foam {
             shape_bubble(b);
             bub[b].pole_off = pole_slots;
             pole_slots += 2 * bub[b].nchunks;
-            if (only_bucket < 0 || bucket_of(bub[b].np) == only_bucket) append_items(b, per_path[bub[b].path]);
+            bub[b].active = active[b];
+            if (active[b] && (only_bucket < 0 || bucket_of(bub[b].np) == only_bucket))
+                append_items(b, per_path[bub[b].path]);
         }
         std::vector<FoamItem> items;
         for (int p = 0; p < kPaths; ++p) {
@@ -595,7 +599,8 @@ struct Foam {
         }
         poles_virtual = true;
         ++steps_taken;
-        for (int b = 0; b < nb; ++b) time[b] += static_cast<double>(bub[b].dt);
+        for (int b = 0; b < nb; ++b)
+            if (active[b]) time[b] += static_cast<double>(bub[b].dt);
     }
 
     int phase() const { return (cur_hi << 4) | (cur << 3) | since_rebase; }
@@ -697,13 +702,14 @@ struct Foam {
         set_dt(dts);
     }
 
+    // Sources may be host or device memory.
     void upload(const float* h_hi, const float* h_lo, const float* gam) {
         for (int k = 0; k < 2; ++k) {
             if (h_hi) {
-                SF_HIP(hipMemcpyAsync(hi[k], h_hi, total * sizeof(float), hipMemcpyHostToDevice, stream));
-                SF_HIP(hipMemcpyAsync(lo[k], h_lo, total * sizeof(float), hipMemcpyHostToDevice, stream));
+                SF_HIP(hipMemcpyAsync(hi[k], h_hi, total * sizeof(float), hipMemcpyDefault, stream));
+                SF_HIP(hipMemcpyAsync(lo[k], h_lo, total * sizeof(float), hipMemcpyDefault, stream));
             }
-            if (gam) SF_HIP(hipMemcpyAsync(g[k], gam, total * sizeof(float), hipMemcpyHostToDevice, stream));
+            if (gam) SF_HIP(hipMemcpyAsync(g[k], gam, total * sizeof(float), hipMemcpyDefault, stream));
         }
         SF_HIP(hipStreamSynchronize(stream));
         cur = cur_hi = cur_pole = since_rebase = 0;
@@ -734,6 +740,24 @@ struct Foam {
         SF_HIP(hipStreamSynchronize(stream));
         for (float* p_ : s.buf) (void)hipFree(p_);
         s.buf.clear();
+    }
+
+    // Makes the whole state current in both ping-pong buffers (pole rows
+    // written), so that a bubble that is not stepped reads the same values
+    // whichever buffer is current.
+    void unify_buffers() {
+        materialize();
+        SF_HIP(hipMemcpyAsync(hi[1 - cur_hi], hi[cur_hi], total * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(lo[1 - cur], lo[cur], total * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(g[1 - cur], g[cur], total * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    }
+
+    // Only bubbles with mask[b] != 0 (all when mask is null) are stepped;
+    // the others keep their state and time until they are stepped again.
+    void set_active(const uint8_t* mask) {
+        unify_buffers();
+        for (int b = 0; b < nb; ++b) active[b] = mask ? (mask[b] != 0) : 1;
+        layout_items();
     }
 
     // Times every item shape on each size bucket and keeps the fastest.
@@ -906,6 +930,7 @@ extern "C" int sf_foam_create(const sf_params* params, int32_t n, const sf_bubbl
                 }
             }
             f->time.assign(n, 0.0);
+            f->active.assign(n, 1);
             f->layout_items();
             f->upload(hh.data(), ll.data(), gg.data());
             f->resolve_cfl();
@@ -1068,5 +1093,58 @@ extern "C" int sf_foam_info(sf_foam* f, int64_t* total_cells, int32_t* n_items, 
         if (n_items) *n_items = fm->n_items;
         if (variants)
             for (int k = 0; k < 4; ++k) variants[k] = fm->variant_of_bucket[k];
+    });
+}
+
+extern "C" int sf_foam_set_active(sf_foam* f, const uint8_t* mask) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        F(f)->set_active(mask);
+    });
+}
+
+extern "C" int sf_foam_set_state(sf_foam* f, const float* hi, const float* lo, const float* gamma) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && hi != nullptr && lo != nullptr, SF_ERR_INVALID, "NULL argument");
+        Foam* fm = F(f);
+        if (gamma == nullptr) fm->unify_buffers();   // keep Γ: current in both buffers
+        fm->upload(hi, lo, gamma);
+    });
+}
+
+extern "C" int sf_foam_device_state(sf_foam* f, const float** hi, const float** lo, const float** gamma) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        Foam* fm = F(f);
+        fm->materialize();
+        SF_HIP(hipStreamSynchronize(fm->stream));
+        if (hi) *hi = fm->hi[fm->cur_hi];
+        if (lo) *lo = fm->lo[fm->cur];
+        if (gamma) *gamma = fm->g[fm->cur];
+    });
+}
+
+extern "C" int sf_foam_bubbles(sf_foam* f, sf_bubble* out) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && out != nullptr, SF_ERR_INVALID, "NULL argument");
+        Foam* fm = F(f);
+        for (int b = 0; b < fm->nb; ++b) {
+            out[b] = fm->desc[b];
+            out[b].dt = static_cast<double>(fm->bub[b].dt);
+        }
+    });
+}
+
+extern "C" int sf_foam_stream(sf_foam* f, void** stream) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && stream != nullptr, SF_ERR_INVALID, "NULL argument");
+        *stream = F(f)->stream;
+    });
+}
+
+extern "C" int sf_foam_num_bubbles(sf_foam* f, int32_t* n) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && n != nullptr, SF_ERR_INVALID, "NULL argument");
+        *n = F(f)->nb;
     });
 }

@@ -154,6 +154,24 @@ class FoamDrainage:
         L.check(self._lib.sf_foam_kernel_timing(self._h, 1 if enable else 0, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def set_active(self, mask: Optional[np.ndarray]) -> None:
+        """Steps only bubbles with mask[b] != 0 (None: all)."""
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        L.check(self._lib.sf_foam_set_active(self._h, L.u8ptr(m)))
+
+    def set_state(self, hi: np.ndarray, lo: np.ndarray, gamma: Optional[np.ndarray] = None) -> None:
+        """Float-float state (host arrays of the layout length)."""
+        hi = np.ascontiguousarray(hi, np.float32)
+        lo = np.ascontiguousarray(lo, np.float32)
+        g = None if gamma is None else np.ascontiguousarray(gamma, np.float32)
+        L.check(self._lib.sf_foam_set_state(self._h, hi.ctypes.data, lo.ctypes.data,
+                                             None if g is None else g.ctypes.data))
+
+    def resolved_bubbles(self) -> List[FoamBubble]:
+        arr = (L.Bubble * len(self.bubbles))()
+        L.check(self._lib.sf_foam_bubbles(self._h, arr))
+        return [FoamBubble(b.n_theta, b.n_phi, b.bubble_radius, b.viscosity, b.h0, b.dt) for b in arr]
+
     def info(self):
         cells = C.c_int64()
         items = C.c_int32()

@@ -1,0 +1,117 @@
+"""Learned surrogate of the drainage step (Fourier neural operator).
+
+A thin wrapper of the ``sf_fno_*`` entry points of include/soapfilm.h: the
+network, its training (HIP forward/backward kernels, BF16 MFMA products,
+FP32 master weights, Adam) on windows generated online by a
+:class:`~soap_bubble_simulation_amd.foam.FoamDrainage` context, and surrogate
+stepping with a plain-solver judge step and fallback per window.  The
+reference has no counterpart; it sits beside the drainage step of
+drainage.rs:298-392 as an accelerator of it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .foam import FoamDrainage
+
+N_IN = 5
+N_OUT = 2
+
+
+def default_config(**overrides) -> L.FnoConfig:
+    c = L.FnoConfig()
+    L.check(L.lib().sf_fno_config_default(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class FnoSurrogate:
+    """Owns one ``sf_fno`` model."""
+
+    def __init__(self, config: Optional[L.FnoConfig] = None, **overrides):
+        self._lib = L.lib()
+        self.config = config if config is not None else default_config(**overrides)
+        h = C.c_void_p()
+        L.check(self._lib.sf_fno_create(self.config, C.byref(h)))
+        self._h = h
+        n = C.c_int64()
+        L.check(self._lib.sf_fno_num_params(self._h, C.byref(n)))
+        self.n_params = n.value
+        self.cells = self.config.n_theta * self.config.n_phi
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sf_fno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- parameters -----------------------------------------------------------
+    def params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        L.check(self._lib.sf_fno_get_params(self._h, L.fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        L.check(self._lib.sf_fno_set_params(self._h, L.fptr(p)))
+
+    def scales(self):
+        s = np.zeros(2, np.float64)
+        L.check(self._lib.sf_fno_get_scales(self._h, L.dptr(s)))
+        return float(s[0]), float(s[1])
+
+    def set_scales(self, sx: float, sy: float) -> None:
+        s = np.array([sx, sy], np.float64)
+        L.check(self._lib.sf_fno_set_scales(self._h, L.dptr(s)))
+
+    # ---- host-array passes ----------------------------------------------------
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        b = x.shape[0]
+        out = np.empty((b, self.config.n_theta, self.config.n_phi, N_OUT), np.float32)
+        L.check(self._lib.sf_fno_forward(self._h, b, L.fptr(x), L.fptr(out)))
+        return out
+
+    def gradient(self, x: np.ndarray, target: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        L.check(self._lib.sf_fno_gradient(self._h, x.shape[0], L.fptr(x), L.fptr(t), C.byref(loss), L.fptr(g)))
+        return loss.value, g
+
+    def train_step(self, x: np.ndarray, target: np.ndarray, lr: float) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_double()
+        L.check(self._lib.sf_fno_train_step(self._h, x.shape[0], L.fptr(x), L.fptr(t), float(lr), C.byref(loss)))
+        return loss.value
+
+    # ---- foam-driven ------------------------------------------------------------
+    def train_foam(self, foam: FoamDrainage, windows: int, lr: float) -> np.ndarray:
+        losses = np.zeros(windows, np.float64)
+        L.check(self._lib.sf_fno_train_foam(self._h, foam._h, foam.params, int(windows), float(lr),
+                                            L.dptr(losses)))
+        return losses
+
+    def advance_foam(self, foam: FoamDrainage, windows: int, tol: float = 1e-3,
+                     predictor: int = L.PREDICT_NETWORK, stats: Optional[L.FnoStats] = None) -> L.FnoStats:
+        """Surrogate windows with judge and fallback; predictor selects the
+        network or a no-network baseline (identity, linear extrapolation)."""
+        st = stats if stats is not None else L.FnoStats()
+        L.check(self._lib.sf_fno_advance_foam(self._h, foam._h, foam.params, int(windows), float(tol),
+                                              int(predictor), C.byref(st)))
+        return st
+
+    def features(self, foam: FoamDrainage, first: int, count: int) -> np.ndarray:
+        out = np.empty((count, self.config.n_theta, self.config.n_phi, N_IN), np.float32)
+        L.check(self._lib.sf_fno_features(self._h, foam._h, foam.params, int(first), int(count), L.fptr(out)))
+        return out

@@ -1,0 +1,251 @@
+"""Learned surrogate (FNO): the FP64 numpy reference is pinned on the CPU
+(finite differences, brute-force DFTs), then the HIP forward and backward
+kernels are checked against it and the trained network against the plain
+solver and the no-network baselines (GPU tests).
+
+Bars: HIP forward / gradient vs FP64 reference within BF16 product
+precision (relative L2 <= 2e-2 per parameter group); numpy gradient vs
+central differences <= 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import fno_ref as R
+
+
+def tiny_shapes():
+    return R.Shapes(n_theta=8, n_phi=16, width=3, modes_theta=2, modes_phi=3, layers=2)
+
+
+def rand_problem(s, batch, seed, scale=0.5):
+    rng = np.random.default_rng(seed)
+    flat = rng.normal(0, scale, s.n_params())
+    x = rng.normal(0, 1, (batch, s.T, s.P, R.N_IN))
+    t = rng.normal(0, 1, (batch, s.T, s.P, R.N_OUT))
+    return flat, x, t
+
+
+def brute_spectral(x, Rc, s):
+    """Spectral convolution by explicit double sums (no FFT)."""
+    B, T, P, C = x.shape
+    tt = np.arange(T)[:, None]
+    pp = np.arange(P)[None, :]
+    y = np.zeros_like(x)
+    ks = [k if k < s.Mt // 2 else k - s.Mt for k in range(s.Mt)]
+    for kk, k in enumerate(ks):
+        for m in range(s.Mp):
+            e = np.exp(-2j * np.pi * (k * tt / T + m * pp / P))
+            X = np.einsum("btpc,tp->bc", x, e)
+            Y = X @ Rc[kk, m]
+            w = (1.0 if m == 0 else 2.0) / (T * P)
+            y += w * np.real(np.einsum("bo,tp->btpo", Y, np.conj(e)))
+    return y
+
+
+def test_reference_spectral_matches_brute_force():
+    s = tiny_shapes()
+    rng = np.random.default_rng(1)
+    x = rng.normal(0, 1, (2, s.T, s.P, s.C))
+    Rc = rng.normal(0, 1, (s.Mt, s.Mp, s.C, s.C)) + 1j * rng.normal(0, 1, (s.Mt, s.Mp, s.C, s.C))
+    y, _ = R.spectral(x, Rc, s)
+    assert np.allclose(y, brute_spectral(x, Rc, s), rtol=1e-12, atol=1e-12)
+
+
+def test_reference_gradient_finite_differences():
+    s = tiny_shapes()
+    flat, x, t = rand_problem(s, 2, seed=2)
+    loss, g, _ = R.loss_and_grad(flat, x, t, s)
+    rng = np.random.default_rng(3)
+    idx = rng.choice(flat.size, 40, replace=False)
+    # include the first and last entries of every group boundary region
+    idx = np.unique(np.concatenate([idx, [0, flat.size - 1, flat.size - 3]]))
+    eps = 1e-6
+    for i in idx:
+        fp, fm = flat.copy(), flat.copy()
+        fp[i] += eps
+        fm[i] -= eps
+        num = (R.loss_and_grad(fp, x, t, s)[0] - R.loss_and_grad(fm, x, t, s)[0]) / (2 * eps)
+        assert abs(num - g[i]) <= 1e-6 * max(1e-3, abs(num)) + 1e-9, (i, num, g[i])
+
+
+def test_reference_gelu_grad():
+    z = np.linspace(-4, 4, 101)
+    num = (R.gelu(z + 1e-6) - R.gelu(z - 1e-6)) / 2e-6
+    assert np.allclose(num, R.gelu_grad(z), atol=1e-8)
+
+
+# ---- GPU ------------------------------------------------------------------------
+
+GPU_CFG = dict(n_theta=32, n_phi=128, width=8, modes_theta=4, modes_phi=4, layers=2, window=4, max_batch=2)
+
+
+def group_slices(s):
+    C, L = s.C, s.L
+    out, o = [], 0
+    def add(name, n):
+        nonlocal o
+        out.append((name, slice(o, o + n)))
+        o += n
+    add("lift W", R.N_IN * C)
+    add("lift b", C)
+    for l in range(L):
+        add("W%d" % l, C * C)
+        add("b%d" % l, C)
+        add("R%d" % l, s.Mt * s.Mp * 2 * C * C)
+    add("Q", C * R.N_OUT)
+    add("q", R.N_OUT)
+    return out
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.fixture
+def model():
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+    m = FnoSurrogate(**GPU_CFG)
+    yield m
+    m.close()
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_reference(model):
+    s = R.Shapes.of(model.config)
+    flat, x, _ = rand_problem(s, 2, seed=5, scale=0.3)
+    model.set_params(flat.astype(np.float32))
+    out = model.forward(x.astype(np.float32))
+    ref, _, _ = R.forward(flat.astype(np.float32).astype(np.float64), x.astype(np.float32).astype(np.float64), s)
+    assert rel(out, ref) <= 2e-2, rel(out, ref)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_reference(model):
+    s = R.Shapes.of(model.config)
+    flat, x, t = rand_problem(s, 2, seed=6, scale=0.3)
+    f32 = flat.astype(np.float32)
+    model.set_params(f32)
+    loss, g = model.gradient(x.astype(np.float32), t.astype(np.float32))
+    rl, rg, _ = R.loss_and_grad(f32.astype(np.float64), x.astype(np.float32).astype(np.float64),
+                                t.astype(np.float32).astype(np.float64), s)
+    assert abs(loss - rl) <= 1e-2 * rl
+    for name, sl in group_slices(s):
+        assert rel(g[sl], rg[sl]) <= 2e-2, (name, rel(g[sl], rg[sl]))
+
+
+@pytest.mark.gpu
+def test_gpu_train_step_reduces_loss(model):
+    s = R.Shapes.of(model.config)
+    rng = np.random.default_rng(7)
+    x = rng.normal(0, 1, (2, s.T, s.P, R.N_IN)).astype(np.float32)
+    # a smooth target the network can fit: a fixed linear map of the inputs
+    t = np.stack([0.5 * x[..., 0] - 0.2 * x[..., 2], 0.3 * x[..., 1] + 0.1], axis=-1).astype(np.float32)
+    losses = [model.train_step(x, t, lr=3e-3) for _ in range(60)]
+    assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
+
+
+# ---- surrogate on a foam: online training, judge, fallback, baselines ------------
+
+SMALL = dict(n_theta=64, n_phi=128, width=16, modes_theta=8, modes_phi=8, layers=4, window=8, max_batch=4)
+
+
+def small_foam(nb, seed, **grid):
+    from soap_bubble_simulation_amd import workloads as W
+    from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_params
+    T, P = grid.get("n_theta", 64), grid.get("n_phi", 128)
+    d = W.foam_batch(nb, seed=seed, lat=(T, T))
+    bubbles = [FoamBubble(**{**x, "n_phi": P}) for x in d]
+    fm = FoamDrainage(foam_params(), bubbles)
+    h = W.foam_noise(fm.offsets, [T * P] * nb, [b.h0 for b in bubbles], seed=seed + 1)
+    fm.set_fields(W.ff_exact(h))
+    return fm
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A small surrogate trained online on plain-solver windows (module-wide)."""
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+    m = FnoSurrogate(**SMALL)
+    losses = []
+    for traj in range(6):
+        fm = small_foam(4, 100 + 7 * traj)
+        losses.extend(m.train_foam(fm, 24, 1e-3))
+        fm.close()
+    yield m, np.array(losses)
+    m.close()
+
+
+@pytest.mark.gpu
+def test_surrogate_features_match_state(trained):
+    m, _ = trained
+    fm = small_foam(2, 31)
+    x = m.features(fm, 0, 2)
+    h, g = fm.fields()
+    sx, _ = m.scales()
+    for k in range(2):
+        hk = fm.split(h)[k]
+        assert np.allclose(x[k, ..., 0], (hk / hk.mean() - 1.0) / sx, rtol=1e-4, atol=1e-4)
+        th = np.arange(64) * np.pi / 63
+        assert np.allclose(x[k, :, 0, 2], np.sin(th), atol=1e-6)
+        assert np.all(x[k, ..., 3] == x[k, 0, 0, 3]) and np.all(x[k, ..., 4] == x[k, 0, 0, 4])
+
+
+@pytest.mark.gpu
+def test_surrogate_training_converges(trained):
+    _, losses = trained
+    # every trajectory starts with a large transient window; compare steady windows
+    steady = losses.reshape(6, 24)[:, 4:]
+    assert np.isfinite(losses).all()
+    assert steady[-1].mean() < 0.5 * steady[0, :4].mean() or steady[-1].mean() < 0.05
+
+
+@pytest.mark.gpu
+def test_network_beats_identity_on_held_out_increments(trained):
+    """What the trained network gains over the method without it: its
+    K-step increment against the plain solver's, on bubbles it never saw."""
+    m, _ = trained
+    _, sy = m.scales()
+    fm = small_foam(4, 999)
+    K = SMALL["window"]
+    err_net, err_id = [], []
+    for w in range(8):
+        h0, _ = fm.fields()
+        out = m.forward(m.features(fm, 0, 4))
+        fm.step(K)
+        h1, _ = fm.fields()
+        pred = np.array([out[k, ..., 1] * fm.split(h0)[k].mean() * sy for k in range(4)])
+        true = np.array([fm.split(h1)[k] - fm.split(h0)[k] for k in range(4)])
+        err_net.append(np.linalg.norm(pred - true) / np.linalg.norm(true))
+        err_id.append(1.0)      # identity predicts no change
+    print("held-out increment rel-L2 per window, network:", np.round(err_net, 3))
+    assert np.mean(err_net) < 0.9 * np.mean(err_id)
+
+
+@pytest.mark.gpu
+def test_judge_fallback_is_exact_and_counted(trained):
+    """With a tolerance no prediction meets, every window falls back to K
+    plain steps from h(t): the state equals the plain solver's bit for bit."""
+    from soap_bubble_simulation_amd import _lib as L
+    m, _ = trained
+    fm, ref = small_foam(4, 555), small_foam(4, 555)
+    st = m.advance_foam(fm, 3, tol=1e-30, predictor=L.PREDICT_NETWORK)
+    ref.step(3 * SMALL["window"])
+    assert st.windows == 3 and st.bubble_windows == 12 and st.fallbacks == 12 and st.accepted == 0
+    assert np.array_equal(fm.fields()[0], ref.fields()[0])
+
+
+@pytest.mark.gpu
+def test_judge_accepts_and_tracks_plain_solver(trained):
+    from soap_bubble_simulation_amd import _lib as L
+    m, _ = trained
+    fm, ref = small_foam(4, 556), small_foam(4, 556)
+    h_start = fm.fields()[0]
+    st = m.advance_foam(fm, 10, tol=1e-3, predictor=L.PREDICT_NETWORK)
+    ref.step(10 * SMALL["window"])
+    assert st.accepted + st.fallbacks == st.bubble_windows == 40
+    assert st.max_residual >= 0.0
+    h_s, h_r = fm.fields()[0], ref.fields()[0]
+    err = np.linalg.norm(h_s - h_r) / np.linalg.norm(h_r)
+    print("accepted %d / %d, rel-L2 h %.3e, increment %.3e" % (
+        st.accepted, st.bubble_windows, err, np.linalg.norm(h_s - h_r) / np.linalg.norm(h_r - h_start)))
+    assert err <= 1e-2          # the north-star bar on h

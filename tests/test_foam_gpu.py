@@ -175,3 +175,41 @@ def test_foam_rupture_cells_frozen():
         assert m.any() and (~m).any()
         assert np.array_equal(fm.split(hg)[k][1:-1][m], h0[1:-1][m])
         assert np.array_equal(fm.split(gg)[k][1:-1][m], g0[1:-1][m].astype(np.float64))
+
+
+def test_foam_active_mask_holds_bubbles():
+    shapes = [(33, 66), (65, 130), (40, 80)]
+    fm, p = make_foam(shapes, seed=21)
+    h, g = seeded_fields(fm, seed=22)
+    fm.set_fields(h, g)
+    fm.set_active(np.array([1, 0, 1], np.uint8))
+    fm.step(20)
+    held = fm.fields()
+    assert np.array_equal(fm.split(held[0])[1], fm.split(h)[1])      # bubble 1 untouched
+    fm.set_active(None)
+    fm.step(10)
+    out = fm.fields()
+    for k, n in [(0, 30), (1, 10), (2, 30)]:
+        ref, _ = make_foam(shapes, seed=21)
+        ref.set_fields(h, g)
+        ref.step(n)
+        hr = ref.fields()[0]
+        # the held bubble resumes at another phase of the every-8-steps rebase
+        # of the float-float state, which moves FP32 roundings (~1e-10 here)
+        tol = 1e-12 if k != 1 else 1e-9
+        assert rel_l2(fm.split(out[0])[k], ref.split(hr)[k]) <= tol, (k, n)
+
+
+def test_foam_set_state_roundtrip():
+    fm, _ = make_foam([(33, 66), (20, 40)], seed=23)
+    h, g = seeded_fields(fm, seed=24)
+    hi = h.astype(np.float32)
+    lo = (h - hi.astype(np.float64)).astype(np.float32)
+    dt0 = fm.bubble_dt().copy()
+    fm.set_state(hi, lo, g.astype(np.float32))
+    hh, gg = fm.fields()
+    assert np.array_equal(hh, hi.astype(np.float64) + lo.astype(np.float64))
+    assert np.array_equal(gg, g.astype(np.float32).astype(np.float64))
+    assert np.array_equal(fm.bubble_dt(), dt0)
+    rb = fm.resolved_bubbles()
+    assert [b.dt for b in rb] == list(dt0)
