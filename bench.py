@@ -48,8 +48,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3"])
-    ap.add_argument("--bubbles", type=int, default=4096, help="foam size of the c3 workload")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3", "c4", "c5"])
+    ap.add_argument("--train-windows", type=int, default=300,
+                    help="c4/c5: online training windows before the timed surrogate run")
+    ap.add_argument("--tol", type=float, default=1e-3, help="c4/c5: judge tolerance")
+    ap.add_argument("--bubbles", type=int, default=None,
+                    help="bubbles of the c3 (default 4096) or c5 (default 1024) workload")
     ap.add_argument("--no-autotune", action="store_true", help="c3: keep the default tile shapes")
     ap.add_argument("--tile-shapes", default=None,
                     help="c3: tile shape index per size bucket, e.g. 0,0,0,0 (skips autotuning)")
@@ -149,7 +153,7 @@ def bench_foam(args, L, W):
     """Batched foam: every bubble advances one step of its own dt per step."""
     from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_params
 
-    desc = W.foam_batch(args.bubbles, seed=5)
+    desc = W.foam_batch(args.bubbles or 4096, seed=5)
     bubbles = [FoamBubble(**d) for d in desc]
     p = foam_params()
     fm = FoamDrainage(p, bubbles)
@@ -203,6 +207,110 @@ def bench_foam(args, L, W):
     print(json.dumps(out))
 
 
+def fno_window_flops(cfg, cells):
+    """Algorithmic FLOPs of one network inference over `cells` cells (GEMM
+    products: truncated DFTs and their inverses, pointwise mixing, lift and
+    projection)."""
+    T, P, C, Mt, Mp = cfg.n_theta, cfg.n_phi, cfg.width, 2 * cfg.modes_theta, cfg.modes_phi
+    per_cell_layer = (2 * 2 * Mp * C        # phi DFT
+                      + 2 * 2 * Mp * C      # inverse phi DFT
+                      + 2 * C * C           # pointwise W
+                      + 2 * 2 * Mt * 2 * Mp * C / P * 2)   # theta DFT + inverse, per cell
+    return cells * (cfg.layers * per_cell_layer + 2 * 5 * C + 2 * C * 2)
+
+
+def bench_surrogate(args, L, W, n_bubbles):
+    """Surrogate + judge (c4: one bubble, c5: a batch of bubbles) against the
+    plain solver on identical seeded inputs, with the no-network baselines."""
+    from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_params
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+
+    T, P = 256, 512
+    model = FnoSurrogate(n_theta=T, n_phi=P, max_batch=16)
+    K = model.config.window
+
+    def make(nb, seed):
+        d = W.foam_batch(nb, seed=seed, lat=(T, T))
+        bubbles = [FoamBubble(**{**x, "n_phi": P}) for x in d]
+        fm = FoamDrainage(foam_params(), bubbles)
+        h = W.foam_noise(fm.offsets, [T * P] * nb, [b.h0 for b in bubbles], seed=seed + 1)
+        fm.set_fields(W.ff_exact(h))
+        return fm
+
+    # online training on plain-solver trajectories of 16 random bubbles; a new
+    # trajectory (new bubbles and initial films) every 62 windows (~1000 steps)
+    t0 = time.perf_counter()
+    losses = []
+    seed = 1000
+    train = None
+    for w in range(args.train_windows):
+        if w % 62 == 0:
+            if train is not None:
+                train.close()
+            train = make(16, seed)
+            seed += 11
+        lr = 1e-3 if w < 0.75 * args.train_windows else 2e-4
+        losses.append(float(model.train_foam(train, 1, lr)[0]))
+    train_s = time.perf_counter() - t0
+    windows = max(1, args.steps // K)
+    steps = windows * K
+
+    ref = make(n_bubbles, 77)
+    h0, _ = ref.fields()
+    ref.synchronize()
+    t1 = time.perf_counter()
+    ref.step(steps)
+    ref.synchronize()
+    plain_s = time.perf_counter() - t1
+    h_ref, _ = ref.fields()
+    cells = n_bubbles * T * P
+    inc_norm = float(np.linalg.norm(h_ref - h0))
+
+    runs = {}
+    for name, pred in [("network", L.PREDICT_NETWORK), ("identity", L.PREDICT_IDENTITY),
+                       ("extrapolate", L.PREDICT_EXTRAPOLATE)]:
+        fm = make(n_bubbles, 77)
+        fm.synchronize()
+        t2 = time.perf_counter()
+        st = model.advance_foam(fm, windows, tol=args.tol, predictor=pred)
+        fm.synchronize()
+        wall = time.perf_counter() - t2
+        h, _ = fm.fields()
+        runs[name] = {
+            "cell_steps_per_s": cells * steps / wall, "wall_s": wall,
+            "speedup_vs_plain": plain_s / wall,
+            "accepted": int(st.accepted), "fallbacks": int(st.fallbacks),
+            "max_residual": st.max_residual,
+            "rel_l2_h": float(np.linalg.norm(h - h_ref) / np.linalg.norm(h_ref)),
+            "rel_l2_increment": float(np.linalg.norm(h - h_ref) / max(inc_norm, 1e-300)),
+            "network_ms": st.surrogate_ms, "judge_ms": st.judge_ms, "fallback_ms": st.fallback_ms,
+        }
+        fm.close()
+    net = runs["network"]
+    flops = fno_window_flops(model.config, cells) * windows
+    achieved = flops / (net["network_ms"] * 1e-3) / 1e12 if net["network_ms"] > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": net["cell_steps_per_s"], "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": 0, "ms_per_step": net["wall_s"] * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "%s: FNO surrogate (4 Fourier layers, width 32, 16x16 modes, K = 16) with a "
+                               "plain-step judge per window (tolerance %g) and fallback, %d bubble(s) at 256x512, "
+                               "%d steps; trained online for %d windows on plain-solver trajectories"
+                               % (args.workload, args.tol, n_bubbles, steps, args.train_windows),
+                   "bubbles": n_bubbles, "cells": cells, "window": K, "windows": windows},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": achieved / 2500.0, "traffic": None, "kernel": "gemm_kernel (network inference phase)",
+                     "flops_per_window": fno_window_flops(model.config, cells)},
+        "plain": {"cell_steps_per_s": cells * steps / plain_s, "wall_s": plain_s},
+        "surrogate_vs_plain": net["cell_steps_per_s"] / (cells * steps / plain_s),
+        "runs": runs,
+        "training": {"windows": args.train_windows, "seconds": train_s, "first_loss": losses[0] if losses else None,
+                     "last_losses_mean": float(np.mean(losses[-20:])) if losses else None,
+                     "scales": model.scales()},
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -224,6 +332,9 @@ def main():
 
     if args.workload == "c3":
         bench_foam(args, L, W)
+        return
+    if args.workload in ("c4", "c5"):
+        bench_surrogate(args, L, W, 1 if args.workload == "c4" else (args.bubbles or 1024))
         return
     spectral = args.workload == "c2"
     if args.workload == "c1":

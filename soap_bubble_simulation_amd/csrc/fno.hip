@@ -445,7 +445,34 @@ struct Fno {
         // phi DFT: per (b, t): [2 Mp x P] x [P x C]
         gemm(Ephi, x, X1, 2 * Mp, C, P, P, C, C, 0, static_cast<long long>(P) * C, static_cast<long long>(MpC2),
              B * T, 0, 0, 1.f, 0.f);
-        // theta DFT: per b: [2 Mt x T] x [T x 2 Mp C]
+        // theta DFT, mode mixing, inverse theta DFT
+        theta_stage(l, B);
+        // inverse phi: per (b, t): [P x 2Mp] x [2Mp x C]
+        gemm(Iphi, Z, ybuf, P, C, 2 * Mp, 2 * Mp, C, C, 0, static_cast<long long>(MpC2), static_cast<long long>(P) * C,
+             B * T, 0, 0, 1.f, 0.f);
+        SF_HIP(hipGetLastError());
+    }
+
+    using RowKernel = void (*)(RowArgs);
+
+    // Fused layer kernel for this shape, or null (then the unfused GEMM path
+    // runs): n_phi 512 (8 waves x 4 row tiles) or 128 (4 waves x 2), width
+    // 16 or 32, 2 modes_phi 16 or 32.  row_threads receives the block size.
+    RowKernel row_kernel(int* row_threads) const {
+        const int CT = C / 16, JT = (2 * Mp) / 16;
+        if (C % 16 || (2 * Mp) % 16) return nullptr;
+#define SF_ROWK(p, r, w, c, j) \
+    if (P == p && CT == c && JT == j) { *row_threads = 64 * w; return &fno_row_kernel<r, c, j, w>; }
+        SF_ROWK(512, 4, 8, 2, 2) SF_ROWK(512, 4, 8, 1, 1) SF_ROWK(512, 4, 8, 2, 1) SF_ROWK(512, 4, 8, 1, 2)
+        SF_ROWK(128, 2, 4, 2, 2) SF_ROWK(128, 2, 4, 1, 1) SF_ROWK(128, 2, 4, 2, 1) SF_ROWK(128, 2, 4, 1, 2)
+#undef SF_ROWK
+        return nullptr;
+    }
+
+    // theta stage of layer l: X1 (phi DFT of x_l) -> Z (inverse theta DFT of
+    // the mixed spectrum), spectrum kept in spec[l]
+    void theta_stage(int l, int B) {
+        const int MpC2 = 2 * Mp * C;
         gemm(Eth, X1, G, 2 * Mt, MpC2, T, T, MpC2, MpC2, 0, static_cast<long long>(T) * MpC2,
              static_cast<long long>(2 * Mt) * MpC2, B, 0, 0, 1.f, 0.f);
         const long long tot = static_cast<long long>(B) * Mt * Mp * C;
@@ -453,26 +480,37 @@ struct Fno {
                            tot);
         hipLaunchKernelGGL(mode_mix_kernel, dim3(Mt * Mp), dim3(256), 0, stream, spec[l], w + o_r[l], Y, B, Mt, Mp,
                            C);
-        // inverse theta: per b: [2T x 2Mt] x [2Mt x Mp C]
         gemm(Ath, Y, Z, 2 * T, Mp * C, 2 * Mt, 2 * Mt, Mp * C, Mp * C, 0, static_cast<long long>(2 * Mt) * Mp * C,
              static_cast<long long>(2 * T) * Mp * C, B, 0, 0, 1.f, 0.f);
-        // inverse phi: per (b, t): [P x 2Mp] x [2Mp x C]
-        gemm(Iphi, Z, ybuf, P, C, 2 * Mp, 2 * Mp, C, C, 0, static_cast<long long>(MpC2), static_cast<long long>(P) * C,
-             B * T, 0, 0, 1.f, 0.f);
-        SF_HIP(hipGetLastError());
     }
 
-    void forward(int B) {
+    // training: keep the pre-activations for the backward pass
+    void forward(int B, bool training) {
         const long long R = static_cast<long long>(B) * N;
         // lift
         gemm(in, w + o_wl, xs[0], static_cast<int>(R), C, kCin, kCin, C, C, 0, 0, 0, 1, 0, 0, 1.f, 0.f);
         hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(R * C, 256, 16384)), dim3(256), 0, stream, xs[0],
                            static_cast<const float*>(nullptr), w + o_bl, xs[0], R * C, C, 0);
-        for (int l = 0; l < L; ++l) {
-            spectral_forward(l, B, xs[l]);
-            gemm(xs[l], w + o_w[l], zs[l], static_cast<int>(R), C, C, C, C, C, 0, 0, 0, 1, 0, 0, 1.f, 0.f);
-            hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(R * C, 256, 16384)), dim3(256), 0, stream, zs[l],
-                               ybuf, w + o_b[l], xs[l + 1], R * C, C, l < L - 1 ? 1 : 0);
+        int row_threads = 256;
+        const RowKernel rk = row_kernel(&row_threads);
+        if (rk) {
+            const int MpC2 = 2 * Mp * C;
+            gemm(Ephi, xs[0], X1, 2 * Mp, C, P, P, C, C, 0, static_cast<long long>(P) * C,
+                 static_cast<long long>(MpC2), B * T, 0, 0, 1.f, 0.f);
+            for (int l = 0; l < L; ++l) {
+                theta_stage(l, B);
+                RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi, Ephi, xs[l + 1], training ? zs[l] : nullptr,
+                           l < L - 1 ? X1 : nullptr, l < L - 1 ? 1 : 0};
+                hipLaunchKernelGGL(rk, dim3(static_cast<unsigned>(B * T)), dim3(row_threads), 0, stream, ra);
+                SF_HIP(hipGetLastError());
+            }
+        } else {
+            for (int l = 0; l < L; ++l) {
+                spectral_forward(l, B, xs[l]);
+                gemm(xs[l], w + o_w[l], zs[l], static_cast<int>(R), C, C, C, C, C, 0, 0, 0, 1, 0, 0, 1.f, 0.f);
+                hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(R * C, 256, 16384)), dim3(256), 0, stream, zs[l],
+                                   ybuf, w + o_b[l], xs[l + 1], R * C, C, l < L - 1 ? 1 : 0);
+            }
         }
         gemm(xs[L], w + o_q, out, static_cast<int>(R), kCout, C, C, kCout, kCout, 0, 0, 0, 1, 0, 0, 1.f, 0.f);
         hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(R * kCout, 256, 16384)), dim3(256), 0, stream, out,
@@ -707,7 +745,7 @@ struct Fno {
                                dim3(256), 0, stream, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo, d_win, dscratch, first,
                                nb, N, static_cast<float>(1.0 / sy), target);
             SF_HIP(hipGetLastError());
-            forward(nb);
+            forward(nb, true);
             total_loss += loss(nb);
             backward(nb);
             adam(lr);
@@ -744,7 +782,7 @@ struct Fno {
                 for (int first = 0; first < fi.nb; first += Bm) {
                     const int nb = std::min(Bm, fi.nb - first);
                     build_features(fi, href, first, nb, ht_hi, ht_lo, g);
-                    forward(nb);
+                    forward(nb, false);
                     hipLaunchKernelGGL(predict_states_kernel,
                                        dim3(blocks_for(static_cast<long long>(nb) * N, 256, 16384)), dim3(256), 0,
                                        stream, ht_hi, ht_lo, out, d_win, dscratch, first, nb, N,
@@ -907,7 +945,7 @@ extern "C" int sf_fno_forward(sf_fno* m, int32_t batch, const float* in, float* 
         SF_REQUIRE(f && in && out && batch >= 1 && batch <= f->Bm, SF_ERR_INVALID, "bad arguments");
         const long long R = static_cast<long long>(batch) * f->N;
         SF_HIP(hipMemcpyAsync(f->in, in, R * kCin * 4, hipMemcpyHostToDevice, f->stream));
-        f->forward(batch);
+        f->forward(batch, false);
         SF_HIP(hipMemcpyAsync(out, f->out, R * kCout * 4, hipMemcpyDeviceToHost, f->stream));
         SF_HIP(hipStreamSynchronize(f->stream));
     });
@@ -921,7 +959,7 @@ extern "C" int sf_fno_gradient(sf_fno* m, int32_t batch, const float* in, const 
         const long long R = static_cast<long long>(batch) * f->N;
         SF_HIP(hipMemcpyAsync(f->in, in, R * kCin * 4, hipMemcpyHostToDevice, f->stream));
         SF_HIP(hipMemcpyAsync(f->target, target, R * kCout * 4, hipMemcpyHostToDevice, f->stream));
-        f->forward(batch);
+        f->forward(batch, true);
         const double l = f->loss(batch);
         f->backward(batch);
         if (loss) *loss = l;
@@ -938,7 +976,7 @@ extern "C" int sf_fno_train_step(sf_fno* m, int32_t batch, const float* in, cons
         const long long R = static_cast<long long>(batch) * f->N;
         SF_HIP(hipMemcpyAsync(f->in, in, R * kCin * 4, hipMemcpyHostToDevice, f->stream));
         SF_HIP(hipMemcpyAsync(f->target, target, R * kCout * 4, hipMemcpyHostToDevice, f->stream));
-        f->forward(batch);
+        f->forward(batch, true);
         const double l = f->loss(batch);
         f->backward(batch);
         f->adam(lr);

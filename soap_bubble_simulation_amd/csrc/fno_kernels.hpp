@@ -132,6 +132,139 @@ __device__ inline float gelu(float z) {      // tanh form
     const float k0 = 0.7978845608028654f, k1 = 0.044715f;
     return 0.5f * z * (1.f + tanhf(k0 * (z + k1 * z * z * z)));
 }
+
+// ---- fused Fourier layer, one block per (bubble, theta row) ------------------
+//
+// For one grid row t of bubble b (P cells x C channels):
+//   z  = x W + Iphi Z[b][t] + bias      (pointwise mixing + inverse phi DFT,
+//                                        one MFMA product with K = C + J)
+//   x' = act(z)                          (stored; z stored when training)
+//   X1'[b][t] = Ephi x'                  (phi DFT of the next layer, fed from
+//                                        the accumulators: the MFMA result
+//                                        layout D[4(l/16)+v][l%16] is the
+//                                        B-operand layout B[4(l/16)+v][l%16])
+// Block: WAVES waves, wave w owns rows p in [w P/WAVES, (w+1) P/WAVES).
+// RT = P / (16 WAVES) row tiles per wave, CT = C / 16 channel tiles, JT =
+// J / 16 tiles of the 2 Mp real/imaginary phi modes.
+struct RowArgs {
+    const float* x;       // [rows][P][C]
+    const float* Z;       // [rows][J][C]
+    const float* W;       // [C][C]
+    const float* bias;    // [C]
+    const float* Iphi;    // [P][J]
+    const float* Ephi;    // [J][P]
+    float* x_out;         // [rows][P][C]
+    float* z_out;         // [rows][P][C] or null
+    float* X1_out;        // [rows][J][C] or null
+    int act;
+};
+
+template <int RT, int CT, int JT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
+    constexpr int C = 16 * CT, J = 16 * JT, K = C + J, P = 16 * RT * WAVES, NT = 64 * WAVES;
+    __shared__ unsigned short Bs[C][K + 4];       // [out channel][k]
+    __shared__ float red[WAVES][J][C];
+    const long long row = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mr = lane & 15, kq = 4 * (lane >> 4);
+    for (int q = threadIdx.x; q < K * C; q += NT) {
+        const int k = q / C, n = q % C;
+        const float v = k < C ? a.W[k * C + n] : a.Z[row * J * C + static_cast<long long>(k - C) * C + n];
+        Bs[n][k] = to_bf16(v);
+    }
+    __syncthreads();
+    const float* x = a.x + row * P * C;
+    f32x4 acc[RT][CT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int p = wave * RT * 16 + rt * 16 + mr;
+#pragma unroll
+        for (int ks = 0; ks < CT; ++ks) {
+            const float4 v = *reinterpret_cast<const float4*>(x + static_cast<long long>(p) * C + ks * 16 + kq);
+            const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
+                                    static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const bf16x4 B = *reinterpret_cast<const bf16x4*>(&Bs[ct * 16 + mr][ks * 16 + kq]);
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc[rt][ct], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int js = 0; js < JT; ++js) {
+            const float4 v = *reinterpret_cast<const float4*>(a.Iphi + static_cast<long long>(p) * J + js * 16 + kq);
+            const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
+                                    static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const bf16x4 B = *reinterpret_cast<const bf16x4*>(&Bs[ct * 16 + mr][C + js * 16 + kq]);
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc[rt][ct], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: bias, activation, stores
+    float* xo = a.x_out + row * P * C;
+    float* zo = a.z_out ? a.z_out + row * P * C : nullptr;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int o = ct * 16 + mr;
+            const float bo = a.bias[o];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int p = wave * RT * 16 + rt * 16 + kq + v;
+                const float z = acc[rt][ct][v] + bo;
+                const float xv = a.act ? gelu(z) : z;
+                xo[static_cast<long long>(p) * C + o] = xv;
+                if (zo) zo[static_cast<long long>(p) * C + o] = z;
+                acc[rt][ct][v] = xv;
+            }
+        }
+    if (a.X1_out == nullptr) return;
+    // phi DFT of the new activations: X1'[j][o] = sum_p Ephi[j][p] x'[p][o]
+    f32x4 acc2[JT][CT];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc2[jt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int p0 = wave * RT * 16 + rt * 16;
+        bf16x4 Bx[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+            Bx[ct] = bf16x4{static_cast<short>(to_bf16(acc[rt][ct][0])), static_cast<short>(to_bf16(acc[rt][ct][1])),
+                            static_cast<short>(to_bf16(acc[rt][ct][2])), static_cast<short>(to_bf16(acc[rt][ct][3]))};
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) {
+            const float4 v = *reinterpret_cast<const float4*>(a.Ephi + static_cast<long long>(jt * 16 + mr) * P + p0 + kq);
+            const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
+                                    static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                acc2[jt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bx[ct], acc2[jt][ct], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[wave][jt * 16 + kq + v][ct * 16 + mr] = acc2[jt][ct][v];
+    __syncthreads();
+    float* X1 = a.X1_out + row * J * C;
+    for (int q = threadIdx.x; q < J * C; q += NT) {
+        const int j = q / C, o = q % C;
+        float sum = 0.f;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) sum += red[w][j][o];
+        X1[q] = sum;
+    }
+}
 __device__ inline float gelu_grad(float z) {
     const float k0 = 0.7978845608028654f, k1 = 0.044715f;
     const float u = k0 * (z + k1 * z * z * z);

@@ -133,6 +133,30 @@ def test_gpu_gradient_matches_reference(model):
         assert rel(g[sl], rg[sl]) <= 2e-2, (name, rel(g[sl], rg[sl]))
 
 
+# shapes served by the fused layer kernel (P = 64 RT, C = 16 CT, 2 Mp = 16 JT)
+FUSED_CFG = dict(n_theta=32, n_phi=128, width=16, modes_theta=4, modes_phi=8, layers=3, window=4, max_batch=2)
+
+
+@pytest.mark.gpu
+def test_gpu_fused_layers_match_reference():
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+    m = FnoSurrogate(**FUSED_CFG)
+    s = R.Shapes.of(m.config)
+    flat, x, t = rand_problem(s, 2, seed=8, scale=0.25)
+    f32 = flat.astype(np.float32)
+    m.set_params(f32)
+    x32, t32 = x.astype(np.float32), t.astype(np.float32)
+    out = m.forward(x32)
+    ref, _, _ = R.forward(f32.astype(np.float64), x32.astype(np.float64), s)
+    assert rel(out, ref) <= 2e-2, rel(out, ref)
+    loss, g = m.gradient(x32, t32)
+    rl, rg, _ = R.loss_and_grad(f32.astype(np.float64), x32.astype(np.float64), t32.astype(np.float64), s)
+    assert abs(loss - rl) <= 1e-2 * rl
+    for name, sl in group_slices(s):
+        assert rel(g[sl], rg[sl]) <= 2e-2, (name, rel(g[sl], rg[sl]))
+    m.close()
+
+
 @pytest.mark.gpu
 def test_gpu_train_step_reduces_loss(model):
     s = R.Shapes.of(model.config)
