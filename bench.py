@@ -299,7 +299,8 @@ def bench_surrogate(args, L, W, n_bubbles):
                                % (args.workload, args.tol, n_bubbles, steps, args.train_windows),
                    "bubbles": n_bubbles, "cells": cells, "window": K, "windows": windows},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": achieved / 2500.0, "traffic": None, "kernel": "gemm_kernel (network inference phase)",
+                     "frac": achieved / 2500.0, "traffic": None,
+                     "kernel": "network inference phase (fno_row_kernel + gemm_kernel, host-timed)",
                      "flops_per_window": fno_window_flops(model.config, cells)},
         "plain": {"cell_steps_per_s": cells * steps / plain_s, "wall_s": plain_s},
         "surrogate_vs_plain": net["cell_steps_per_s"] / (cells * steps / plain_s),

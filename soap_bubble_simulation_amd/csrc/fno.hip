@@ -311,14 +311,14 @@ struct Fno {
                  Nn, Nn, static_cast<long long>(kSplitRows) * Mm, static_cast<long long>(kSplitRows) * Nn,
                  static_cast<long long>(Mm) * Nn, nb, 1, 0, 1.f, 0.f);
         }
-        hipLaunchKernelGGL(column_sum_kernel, dim3((Mm * Nn + 255) / 256), dim3(256), 0, stream, part, chunks,
+        hipLaunchKernelGGL(column_sum_kernel, dim3((Mm * Nn + 31) / 32), dim3(256), 0, stream, part, chunks,
                            Mm * Nn, dst, 1);
         SF_HIP(hipGetLastError());
     }
 
     // dst[c] += sum over rows of x[row][c].
     void bias_grad(const float* x, long long nrows, int Cc, float* dst) {
-        const unsigned nblk = blocks_for(nrows, 256 / Cc * 64, 4096);
+        const unsigned nblk = blocks_for(nrows, 256 / Cc * 64, 512);
         const long long need = static_cast<long long>(nblk) * Cc;
         if (need > part_cap) {
             (void)hipFree(part);
@@ -326,7 +326,7 @@ struct Fno {
             part_cap = need;
         }
         hipLaunchKernelGGL(column_partial_kernel, dim3(nblk), dim3(256), 0, stream, x, nrows, Cc, part);
-        hipLaunchKernelGGL(column_sum_kernel, dim3((Cc + 255) / 256), dim3(256), 0, stream, part,
+        hipLaunchKernelGGL(column_sum_kernel, dim3((Cc + 31) / 32), dim3(256), 0, stream, part,
                            static_cast<long long>(nblk), Cc, dst, 1);
         SF_HIP(hipGetLastError());
     }

@@ -291,14 +291,23 @@ __global__ void act_grad_kernel(const float* z, const float* dx, float* dz, long
         dz[k] = act ? dx[k] * gelu_grad(z[k]) : dx[k];
 }
 
-// out[j] = sum over rows of part[row][j] (j < cols), FP64 accumulation; one
-// thread per column, rows strided (small column counts).
-__global__ void column_sum_kernel(const float* part, long long rows, int cols, float* out, int accumulate) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
+// out[j] (+)= sum over rows of part[row][j], FP64 accumulation.  Block of
+// 256 threads: 32 consecutive columns x 8 row lanes, folded in LDS.
+__global__ __launch_bounds__(256) void column_sum_kernel(const float* part, long long rows, int cols, float* out,
+                                                         int accumulate) {
+    const int tx = threadIdx.x & 31, rl = threadIdx.x >> 5;
+    const int j = blockIdx.x * 32 + tx;
     double s = 0.0;
-    for (long long r = 0; r < rows; ++r) s += part[r * cols + j];
-    out[j] = accumulate ? out[j] + static_cast<float>(s) : static_cast<float>(s);
+    if (j < cols)
+        for (long long r = rl; r < rows; r += 8) s += part[r * cols + j];
+    __shared__ double red[8][32];
+    red[rl][tx] = s;
+    __syncthreads();
+    if (rl == 0 && j < cols) {
+        double t = 0.0;
+        for (int q = 0; q < 8; ++q) t += red[q][tx];
+        out[j] = accumulate ? out[j] + static_cast<float>(t) : static_cast<float>(t);
+    }
 }
 
 // Per-block column sums of a [rows][C] tensor: partial[block][C].
