@@ -247,12 +247,48 @@ int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, double dt, cons
                               double refractive_index, const float* cmf);
 /* Blocks until all work enqueued on ctx has finished. */
 int sf_drainage_synchronize(sf_drainage* ctx);
+/* The thickness field rendered through the runtime LUT, one pixel per grid
+ * point (image width n_phi, height n_theta): the fragment colour of
+ * bubble.wgsl:425-439 -- u = clamp(t_nm / 2000, 0, 1), v = clamp(view cos, 0,
+ * 1) with the view cosine of bubble.wgsl:491-494, bilinear sampling of the
+ * 256 x 64 RGBA8 LUT with clamp-to-edge addressing (pipeline.rs:350-359),
+ * times `intensity` (pipeline.rs:75 default 4), clamped to [0, 1] -- written
+ * as RGBA8 with alpha 255 into rgba_out[n_theta*n_phi*4], the frame layout
+ * export_frame writes to PNG (image_export.rs:42).  lut_rgba8 NULL: the LUT
+ * of sf_lut_rgba8(refractive_index, 1) as the renderer builds it
+ * (pipeline.rs:311-314).  cam NULL: (0, 0, 4R). */
+int sf_render_lut_rgba8(sf_drainage* ctx, const uint8_t* lut_rgba8, double refractive_index, float intensity,
+                        const sf_camera* cam, uint8_t* rgba_out);
 /* Device pointer of the last spectral RGB planes of ctx (3*n_theta*n_phi). */
 int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb);
 /* Average duration in ms of the spectral kernel launches since the last call
  * (HIP events on the context's stream); enable switches recording on/off. */
 int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
                               int64_t* launches);
+
+/* ---- caustic intensity (consumer of the thickness field) ------------------
+ * CausticParams of caustics.rs:20-54 without the grid size (taken from the
+ * field).  Only focal_length, caustic_intensity, caustic_sharpness and
+ * branch_threshold enter the map (caustics_compute.wgsl:89-114); the others
+ * are carried for the ground-plane renderer. */
+typedef struct sf_caustic_params {
+    float refractive_index, film_thickness_scale;
+    float light_dir[3];
+    float light_intensity, focal_length, caustic_intensity, caustic_sharpness, branch_threshold;
+    float ground_y, bubble_radius, time;
+} sf_caustic_params;
+
+/* Defaults of CausticParams::default (caustics.rs:56-78). */
+int sf_caustic_params_default(sf_caustic_params* params);
+/* Caustic-intensity map of the context's current thickness (metres, FP32
+ * h = hi + lo), [n_theta][n_phi] (caustics_compute.wgsl:117-126 dispatched by
+ * CausticRenderer::compute, caustics.rs:441-457).  map_host may be NULL: the
+ * map then stays on the device (sf_caustics_device_map). */
+int sf_caustics(sf_drainage* ctx, const sf_caustic_params* params, float* map_host);
+int sf_caustics_device_map(sf_drainage* ctx, const float** map);
+/* The same on a host thickness array [n_theta][n_phi]. */
+int sf_caustics_field(const sf_caustic_params* params, int32_t n_theta, int32_t n_phi, const float* thickness,
+                      float* map);
 
 /* ---- batched foam --------------------------------------------------------
  * Many bubbles, each with its own grid, radius, viscosity and time step,

@@ -111,6 +111,10 @@ def lib():
             "orc_spectral_rgb": (None, [dp, C.c_int, C.c_int, C.c_double, dp, C.c_double, dp, dp]),
             "orc_spectral_cells": (None, [dp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int,
                                           C.c_double, dp, C.c_double, dp, dp]),
+            "orc_render_lut": (None, [dp, C.c_int, C.c_int, C.c_double, dp, C.POINTER(C.c_uint8), C.c_double,
+                                      C.POINTER(C.c_uint8)]),
+            "orc_caustics": (None, [C.POINTER(C.c_float), C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -313,4 +317,31 @@ def spectral_cells(h, cells, radius, cam, n_film, cmf):
     out = np.zeros((cl.size, 3), np.float64)
     lib().orc_spectral_cells(_dp(h), nt, npf, cl.ctypes.data_as(C.POINTER(C.c_int64)), cl.size,
                              radius, _dp(c), n_film, _dp(cm), _dp(out))
+    return out
+
+
+def caustics(h, focal_length=0.1, caustic_intensity=2.0, caustic_sharpness=1.5, branch_threshold=0.001):
+    """Caustic intensity map of an FP32 thickness grid (caustics_compute.wgsl)
+    and its branch mask; defaults of CausticParams (caustics.rs:56-78)."""
+    h = np.ascontiguousarray(h, np.float32)
+    nt, npf = h.shape
+    out = np.zeros((nt, npf), np.float32)
+    br = np.zeros((nt, npf), np.uint8)
+    fp = C.POINTER(C.c_float)
+    lib().orc_caustics(h.ctypes.data_as(fp), nt, npf, focal_length, caustic_intensity, caustic_sharpness,
+                       branch_threshold, out.ctypes.data_as(fp), br.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out, br
+
+
+def render_lut(h, radius, cam, lut_rgba8, intensity=4.0):
+    """RGBA8 frame [n_theta][n_phi][4] of a thickness grid (metres) sampled
+    through an RGBA8 LUT (bubble.wgsl:425-439)."""
+    h = np.ascontiguousarray(h, np.float64)
+    nt, npf = h.shape
+    lut = np.ascontiguousarray(lut_rgba8, np.uint8)
+    out = np.zeros((nt, npf, 4), np.uint8)
+    c = np.asarray(cam, np.float64)
+    u8 = C.POINTER(C.c_uint8)
+    lib().orc_render_lut(_dp(h), nt, npf, radius, _dp(c), lut.ctypes.data_as(u8), intensity,
+                         out.ctypes.data_as(u8))
     return out

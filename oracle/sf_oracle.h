@@ -118,6 +118,13 @@ void   orc_spectral_cells(const double* h, int n_theta, int n_phi, const int64_t
                           int n_cells, double radius, const double cam[3], double n_film,
                           const double* cmf, double* rgb);
 
+void   orc_render_lut(const double* h, int n_theta, int n_phi, double radius, const double cam[3],
+                      const uint8_t* lut, double intensity, uint8_t* out /* n*4 */);
+
+/* ---- caustic intensity (caustics_compute.wgsl:27-126), FP32 ---------------- */
+void   orc_caustics(const float* h, int n_theta, int n_phi, float focal_length, float caustic_intensity,
+                    float caustic_sharpness, float branch_threshold, float* out, uint8_t* branch);
+
 #ifdef __cplusplus
 }
 #endif

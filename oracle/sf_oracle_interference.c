@@ -324,3 +324,38 @@ void orc_spectral_cells(const double* h, int n_theta, int n_phi, const int64_t* 
         spectral_point((double)i * dtheta, (double)j * dphi, h[c], radius, cam, n_film, cmf, rgb + 3 * k);
     }
 }
+
+/* Thickness field rendered through the RGBA8 runtime LUT (bubble.wgsl:425-439
+ * sampled with the linear clamp-to-edge sampler of pipeline.rs:350-359), one
+ * pixel per grid point; FP64 texture coordinates and weights, output
+ * quantised by round-to-nearest. */
+void orc_render_lut(const double* h, int n_theta, int n_phi, double radius, const double cam[3],
+                    const uint8_t* lut, double intensity, uint8_t* out)
+{
+    const double dth = ORC_PI / (double)(n_theta - 1), dph = 2.0 * ORC_PI / (double)n_phi;
+    for (int i = 0; i < n_theta; ++i)
+        for (int j = 0; j < n_phi; ++j) {
+            const size_t c = (size_t)i * n_phi + j;
+            const double cv = orc_view_cos(i * dth, j * dph, radius, cam);
+            double u = h[c] * 1e9 / 2000.0, v = cv;
+            u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+            v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+            const double x = u * 256.0 - 0.5, y = v * 64.0 - 0.5;
+            const double x0 = floor(x), y0 = floor(y), fx = x - x0, fy = y - y0;
+            int i0 = (int)x0, i1 = (int)x0 + 1, j0 = (int)y0, j1 = (int)y0 + 1;
+            i0 = i0 < 0 ? 0 : (i0 > 255 ? 255 : i0);
+            i1 = i1 < 0 ? 0 : (i1 > 255 ? 255 : i1);
+            j0 = j0 < 0 ? 0 : (j0 > 63 ? 63 : j0);
+            j1 = j1 < 0 ? 0 : (j1 > 63 ? 63 : j1);
+            for (int ch = 0; ch < 3; ++ch) {
+                const double t00 = lut[((size_t)j0 * 256 + i0) * 4 + ch] / 255.0;
+                const double t10 = lut[((size_t)j0 * 256 + i1) * 4 + ch] / 255.0;
+                const double t01 = lut[((size_t)j1 * 256 + i0) * 4 + ch] / 255.0;
+                const double t11 = lut[((size_t)j1 * 256 + i1) * 4 + ch] / 255.0;
+                double val = ((t00 * (1 - fx) + t10 * fx) * (1 - fy) + (t01 * (1 - fx) + t11 * fx) * fy) * intensity;
+                val = val < 0.0 ? 0.0 : (val > 1.0 ? 1.0 : val);
+                out[c * 4 + ch] = (uint8_t)floor(val * 255.0 + 0.5);
+            }
+            out[c * 4 + 3] = 255;
+        }
+}

@@ -96,6 +96,16 @@ class Bubble(C.Structure):
     ]
 
 
+class CausticParams(C.Structure):
+    """Mirror of ``sf_caustic_params``."""
+    _fields_ = [
+        ("refractive_index", C.c_float), ("film_thickness_scale", C.c_float), ("light_dir", C.c_float * 3),
+        ("light_intensity", C.c_float), ("focal_length", C.c_float), ("caustic_intensity", C.c_float),
+        ("caustic_sharpness", C.c_float), ("branch_threshold", C.c_float), ("ground_y", C.c_float),
+        ("bubble_radius", C.c_float), ("time", C.c_float),
+    ]
+
+
 class FnoConfig(C.Structure):
     """Mirror of ``sf_fno_config``."""
     _fields_ = [
@@ -168,6 +178,11 @@ SIGNATURES = {
     "sf_drainage_synchronize": (C.c_int, [_vp]),
     "sf_spectral_rgb": (C.c_int, [_vp, C.POINTER(Camera), C.c_double, _fp, _fp]),
     "sf_spectral_device_rgb": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sf_render_lut_rgba8": (C.c_int, [_vp, _u8p, C.c_double, C.c_float, C.POINTER(Camera), _u8p]),
+    "sf_caustic_params_default": (C.c_int, [C.POINTER(CausticParams)]),
+    "sf_caustics": (C.c_int, [_vp, C.POINTER(CausticParams), _fp]),
+    "sf_caustics_device_map": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sf_caustics_field": (C.c_int, [C.POINTER(CausticParams), C.c_int32, C.c_int32, _fp, _fp]),
     "sf_foam_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(Bubble), C.POINTER(_vp)]),
     "sf_foam_destroy": (None, [_vp]),
     "sf_foam_layout": (C.c_int, [_vp, C.POINTER(C.c_int64)]),

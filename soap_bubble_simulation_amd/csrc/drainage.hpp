@@ -84,6 +84,7 @@ struct Drainage {
 
     // spectral interference
     float* rgb = nullptr;
+    float* caustic = nullptr;       // caustic-intensity map (caustics.hip), allocated on first use
     float* cmf_dev = nullptr;
     float4* spec_rows = nullptr;
     float2* spec_cols = nullptr;

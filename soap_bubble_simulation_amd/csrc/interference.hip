@@ -599,8 +599,9 @@ static std::vector<uint2> pack_cmf(const float* cmf) {
 }
 
 namespace sf {
-void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf) {
-    if (!d->rgb) d->rgb = dalloc_i<float>(3 * d->n);
+// Per-row (sin, cos theta) and per-column (sin, cos phi) tables of the view
+// geometry, built once per context in FP64 and rounded.
+static void ensure_view_tables(Drainage* d) {
     if (!d->spec_rows) {
         d->spec_rows = dalloc_i<float4>(d->nt);
         d->spec_cols = dalloc_i<float2>(d->np);
@@ -618,6 +619,11 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
         SF_HIP(hipMemcpyAsync(d->spec_cols, cl.data(), d->np * sizeof(float2), hipMemcpyHostToDevice, d->stream));
         SF_HIP(hipStreamSynchronize(d->stream));
     }
+}
+
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf) {
+    if (!d->rgb) d->rgb = dalloc_i<float>(3 * d->n);
+    ensure_view_tables(d);
     if (cmf) {
         if (!d->cmf_dev) d->cmf_dev = reinterpret_cast<float*>(dalloc_i<uint2>(2 * kSpecChunks * 64));
         const std::vector<uint2> frag = pack_cmf(cmf);
@@ -693,5 +699,107 @@ extern "C" int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb) {
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
         SF_REQUIRE(d->rgb != nullptr, SF_ERR_INVALID, "sf_spectral_rgb has not run on this context");
         *rgb = d->rgb;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Rendering of the thickness field through the runtime LUT (the fragment
+// path of bubble.wgsl:425-439 evaluated on the simulation grid): per cell the
+// view cosine |dot(normalize(cam - p), n)| (bubble.wgsl:491-494), texture
+// coordinates u = clamp(t_nm / 2000, 0, 1), v = clamp(cos, 0, 1), bilinear
+// sampling of the 256 x 64 RGBA8 LUT with clamp-to-edge addressing
+// (pipeline.rs:350-359), times the interference intensity, clamped to [0, 1]
+// and written as RGBA8 (alpha 255).
+
+struct LutRenderArgs {
+    const float* hi;
+    const float* lo;
+    const float4* rows;
+    const float2* cols;
+    const uint8_t* lut;       // [64][256][4]
+    uint8_t* out;             // [n][4]
+    int np;
+    size_t n;
+    float cx, cy, cz, cam2, radius, intensity;
+};
+
+__device__ inline float lut_texel(const uint8_t* lut, int i, int j, int c) {
+    return static_cast<float>(lut[(static_cast<size_t>(j) * kLutT + i) * 4 + c]) * (1.f / 255.f);
+}
+
+__global__ __launch_bounds__(256) void lut_render_kernel(const LutRenderArgs a) {
+    const size_t cell = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (cell >= a.n) return;
+    const int i = static_cast<int>(cell / a.np), j = static_cast<int>(cell % a.np);
+    const float4 rw = a.rows[i];
+    const float2 cl = a.cols[j];
+    const float nx = rw.x * cl.y, ny = rw.y, nz = rw.x * cl.x;
+    const float cn = a.cx * nx + a.cy * ny + a.cz * nz;
+    const float len2 = a.cam2 - 2.f * a.radius * cn + a.radius * a.radius;
+    const float cos_v = fminf(fabsf(cn - a.radius) * rsqrtf(len2), 1.f);
+    const float t_nm = (a.hi[cell] + a.lo[cell]) * 1e9f;
+    const float u = fminf(fmaxf(t_nm / kLutMaxNm, 0.f), 1.f);
+    const float v = fminf(fmaxf(cos_v, 0.f), 1.f);
+    const float x = u * static_cast<float>(kLutT) - 0.5f, y = v * static_cast<float>(kLutA) - 0.5f;
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int i0 = min(max(static_cast<int>(x0), 0), kLutT - 1), i1 = min(max(static_cast<int>(x0) + 1, 0), kLutT - 1);
+    const int j0 = min(max(static_cast<int>(y0), 0), kLutA - 1), j1 = min(max(static_cast<int>(y0) + 1, 0), kLutA - 1);
+    uchar4 px;
+    uint8_t ch[3];
+    for (int c = 0; c < 3; ++c) {
+        const float top = lut_texel(a.lut, i0, j0, c) * (1.f - fx) + lut_texel(a.lut, i1, j0, c) * fx;
+        const float bot = lut_texel(a.lut, i0, j1, c) * (1.f - fx) + lut_texel(a.lut, i1, j1, c) * fx;
+        const float val = fminf(fmaxf((top * (1.f - fy) + bot * fy) * a.intensity, 0.f), 1.f);
+        ch[c] = static_cast<uint8_t>(val * 255.f + 0.5f);
+    }
+    px.x = ch[0];
+    px.y = ch[1];
+    px.z = ch[2];
+    px.w = 255;
+    reinterpret_cast<uchar4*>(a.out)[cell] = px;
+}
+
+extern "C" int sf_render_lut_rgba8(sf_drainage* ctx, const uint8_t* lut_rgba8, double refractive_index,
+                                   float intensity, const sf_camera* cam, uint8_t* rgba_out) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && rgba_out != nullptr, SF_ERR_INVALID, "NULL argument");
+        SF_REQUIRE(std::isfinite(intensity) && refractive_index > 0.0, SF_ERR_INVALID, "bad intensity or index");
+        Drainage* d = reinterpret_cast<Drainage*>(ctx);
+        ensure_view_tables(d);
+        d->materialize();
+        DevBuf lut(static_cast<size_t>(kLutT) * kLutA * 4);
+        if (lut_rgba8) {
+            SF_HIP(hipMemcpyAsync(lut.p, lut_rgba8, static_cast<size_t>(kLutT) * kLutA * 4, hipMemcpyHostToDevice,
+                                  d->stream));
+        } else {   // the renderer's LUT: built at intensity 1, scaled at sampling (pipeline.rs:311-314)
+            hipLaunchKernelGGL(lut_rgb_kernel, dim3(kLutT / 64, kLutA), dim3(64), 0, d->stream,
+                               static_cast<float>(refractive_index), 1.f, static_cast<float*>(nullptr),
+                               static_cast<uint8_t*>(lut.p));
+            SF_HIP(hipGetLastError());
+        }
+        DevBuf out(d->n * 4);
+        LutRenderArgs a;
+        a.hi = d->hi[d->cur_hi];
+        a.lo = d->lo[d->cur];
+        a.rows = d->spec_rows;
+        a.cols = d->spec_cols;
+        a.lut = static_cast<const uint8_t*>(lut.p);
+        a.out = static_cast<uint8_t*>(out.p);
+        a.np = d->np;
+        a.n = d->n;
+        const sf_camera c = cam ? *cam : sf_camera{{0.0, 0.0, 4.0 * d->p.bubble_radius}};
+        a.cx = static_cast<float>(c.position[0]);
+        a.cy = static_cast<float>(c.position[1]);
+        a.cz = static_cast<float>(c.position[2]);
+        a.cam2 = static_cast<float>(c.position[0] * c.position[0] + c.position[1] * c.position[1] +
+                                    c.position[2] * c.position[2]);
+        a.radius = static_cast<float>(d->p.bubble_radius);
+        a.intensity = intensity;
+        hipLaunchKernelGGL(lut_render_kernel, dim3(static_cast<unsigned>((d->n + 255) / 256)), dim3(256), 0,
+                           d->stream, a);
+        SF_HIP(hipGetLastError());
+        SF_HIP(hipMemcpyAsync(rgba_out, out.p, d->n * 4, hipMemcpyDeviceToHost, d->stream));
+        SF_HIP(hipStreamSynchronize(d->stream));
     });
 }

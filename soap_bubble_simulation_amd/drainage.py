@@ -147,6 +147,14 @@ class DrainageContext:
         L.check(self._lib.sf_drainage_stream(self._h, C.byref(s)))
         return s.value or 0
 
+    def caustics(self, params=None) -> np.ndarray:
+        """Caustic-intensity map of the current thickness (caustics_compute.wgsl)."""
+        from .caustics import default_params
+        p = params if params is not None else default_params()
+        out = np.empty((self.n_theta, self.n_phi), np.float32)
+        L.check(self._lib.sf_caustics(self.handle, p, L.fptr(out)))
+        return out
+
     def kernel_timing(self, enable: bool) -> Tuple[float, int]:
         """Average drainage-kernel duration (ms) over the launches since the
         previous call, measured with HIP events on the context's stream."""

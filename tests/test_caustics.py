@@ -1,0 +1,82 @@
+"""Caustic-intensity stencil (caustics_compute.wgsl:27-126): the CPU
+restatement is pinned by closed forms, then the HIP kernel is checked against
+it (branch mask bit-exact, intensity within FP32 pow/sqrt rounding)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+D = dict(focal_length=0.1, caustic_intensity=2.0, caustic_sharpness=1.5, branch_threshold=0.001)
+
+
+def test_uniform_field_gives_the_intensity():
+    m, b = O.caustics(np.full((9, 20), 0.7, np.float32))
+    assert np.all(m == np.float32(2.0)) and not b.any()
+
+
+def test_linear_theta_ramp():
+    nt, npf, a = 12, 16, 0.25
+    h = (a * np.arange(nt, dtype=np.float32))[:, None] * np.ones((1, npf), np.float32)
+    m, b = O.caustics(h)
+    want = np.float32((1.0 + 0.5 * a) ** 1.5 * 2.0)
+    assert np.allclose(m[1:-1], want, rtol=1e-6) and not b.any()
+    # clamped rows: one-sided difference a/2, Laplacian +-a
+    top = (1.0 / abs(1.0 + (a * 0.1))) * (1 + 0.25 * a)
+    assert np.allclose(m[0], np.float32(top ** 1.5 * 2.0), rtol=1e-6)
+
+
+def test_bowl_and_saddle_curvature():
+    nt, npf, q = 21, 21, 0.02
+    y, x = np.mgrid[0:nt, 0:npf].astype(np.float64)
+    x0 = y0 = 10.0
+    bowl = (q * ((x - x0) ** 2 + (y - y0) ** 2) + 1.0).astype(np.float32)
+    saddle = (q * ((x - x0) ** 2 - (y - y0) ** 2) + 1.0).astype(np.float32)
+    mb, bb = O.caustics(bowl)
+    ms, bs = O.caustics(saddle)
+    inner = (slice(3, -3), slice(3, -3))
+    assert not bb[inner].any() and bs[inner].all()          # det = 4 q^2 vs -4 q^2
+    gx, gy = 2 * q * (x - x0), 2 * q * (y - y0)
+    want_bowl = (1.0 / (1.0 + 0.1 * 4 * q) * (1 + 0.5 * np.hypot(gx, gy))) ** 1.5 * 2.0
+    want_sad = (1.5 * (1 + 0.5 * np.hypot(gx, gy))) ** 1.5 * 2.0
+    assert np.allclose(mb[inner], np.minimum(want_bowl, 10)[inner], rtol=2e-5)
+    assert np.allclose(ms[inner], np.minimum(want_sad, 10)[inner], rtol=2e-5)
+
+
+def test_phi_wrap():
+    h = np.random.default_rng(0).uniform(0.5, 1.5, (10, 24)).astype(np.float32)
+    m, _ = O.caustics(h)
+    m2, _ = O.caustics(np.roll(h, 5, axis=1))
+    assert np.array_equal(np.roll(m, 5, axis=1), m2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(64, 128), (37, 74), (3, 4), (130, 2052)])
+def test_gpu_caustics_match_oracle(shape):
+    from soap_bubble_simulation_amd.caustics import caustic_map
+    rng = np.random.default_rng(shape[0])
+    h = rng.uniform(0.2, 2.0, shape).astype(np.float32)     # O(1) values exercise every branch
+    m = caustic_map(h)
+    mo, bo = O.caustics(h)
+    assert np.allclose(m, mo, rtol=2e-6, atol=0)
+    # branch decisions: where the oracle took the 1.5 factor the ratio shows it
+    assert bo.any() and (~bo.astype(bool)).any()
+
+
+@pytest.mark.gpu
+def test_gpu_caustics_of_a_drainage_context():
+    from soap_bubble_simulation_amd import _lib as L
+    from soap_bubble_simulation_amd import workloads as W
+    from soap_bubble_simulation_amd.drainage import DrainageContext
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    nt, npf = 64, 128
+    ctx = DrainageContext(p, nt, npf, 500e-9)
+    h0 = W.noise_field(nt, npf, seed=4)
+    ctx.set_fields(h0, np.full_like(h0, 0.5))
+    ctx.step(3, 0.05)
+    m = ctx.caustics()
+    h, _ = ctx.fields()
+    hi = h.astype(np.float32)
+    mo, _ = O.caustics(hi)
+    # physical thicknesses (~5e-7 m) give vanishing gradients: the map is ~2
+    assert np.allclose(m, mo, rtol=2e-6) and np.allclose(m, 2.0, rtol=1e-5)
