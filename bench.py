@@ -315,7 +315,7 @@ def bench_surrogate(args, L, W, n_bubbles):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
               "(the per-GPU figure at --gpus 1 is the measurement)", file=sys.stderr)
         sys.exit(2)
 

@@ -3,7 +3,7 @@
 Every comparison starts the HIP path and the oracle from bit-identical fields
 (float-float exact FP64 values, ``workloads.ff_exact``).  Bars:
   * rupture mask, TFE membership, pole/phi indexing, query addressing: exact;
-  * thickness h: relative L2 <= 1e-6 (north-star bar), and the increment
+  * thickness h: relative L2 <= 1e-6, and the increment
     h - h0 is reported and held to a tighter relative L2 where stated;
   * Γ: FP32 state like the reference's WGSL kernel, |ΔΓ| <= 1e-6 per step.
 """

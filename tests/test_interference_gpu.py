@@ -3,7 +3,7 @@
 Bars (tolerances stated per test):
   * LUT addressing (thickness / cos of every texel) is formed on the host in
     FP64 like the reference, so texel coordinates match bit for bit;
-  * RGB within 1e-3 of the FP64 oracle (north-star bar);
+  * RGB within 1e-3 of the FP64 oracle;
   * the RGBA8 LUT is compared before quantisation (one LSB = 3.9e-3), and the
     quantised bytes may differ by one LSB only where the value sits within
     1e-5 of a quantisation boundary.
