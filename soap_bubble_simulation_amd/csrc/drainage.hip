@@ -187,11 +187,17 @@ static void launch_vec4(dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hip
     hipExtLaunchKernelGGL((drainage_vec4_kernel<RB, GM, M, D, ROWS, kPrefetchRows>), grid, block, 0, s, t0, t1, 0, a);
 }
 
+template <bool RB, int GM, bool M, bool D>
+static void launch_band(dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const StepArgs& a) {
+    hipExtLaunchKernelGGL((drainage_band_kernel<RB, GM, M, D>), grid, block, 0, s, t0, t1, 0, a);
+}
+
 // Kernel table indexed by [rebase][gmode][marangoni][diag].  The scalar
-// kernel serves any n_phi; the float4 kernel needs n_phi % 4 == 0 and comes
-// in row-band heights of 4 and 8.
+// kernel serves any n_phi; the float4 kernels need n_phi % 4 == 0: unrolled
+// row bands of 4 and 8, or the band-streaming kernel for 16..64 rows.
 static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag, bool vec4, int rows) {
 #define SF_E(RB, GM, M, D) &launch_step<RB, GM, M, D>
+#define SF_B(RB, GM, M, D) &launch_band<RB, GM, M, D>
 #define SF_V(RB, GM, M, D, R) &launch_vec4<RB, GM, M, D, R>
 #define SF_ROW(E, RB, GM)                                                   \
     {{E(RB, GM, false, false), E(RB, GM, false, true)},                     \
@@ -204,7 +210,9 @@ static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag, bool ve
     static const StepFn scalar[2][3][2][2] = SF_TAB(SF_E);
     static const StepFn v4[2][3][2][2] = SF_TAB(SF_V4);
     static const StepFn v8[2][3][2][2] = SF_TAB(SF_V8);
+    static const StepFn band[2][3][2][2] = SF_TAB(SF_B);
 #undef SF_E
+#undef SF_B
 #undef SF_V
 #undef SF_ROW
 #undef SF_TAB
@@ -212,7 +220,8 @@ static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag, bool ve
 #undef SF_V8
     if (!vec4) return scalar[rebase][gmode][marangoni][diag];
     if (rows <= 4) return v4[rebase][gmode][marangoni][diag];
-    return v8[rebase][gmode][marangoni][diag];
+    if (rows <= 8) return v8[rebase][gmode][marangoni][diag];
+    return band[rebase][gmode][marangoni][diag];
 }
 
 }  // namespace sf
@@ -233,9 +242,11 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     wx = static_cast<int>(std::min<size_t>(4, cdiv(np, cols_per_wave)));
     block = 64 * wx;
     ntiles = static_cast<int>(cdiv(np, static_cast<size_t>(cols_per_wave) * wx));
-    rows_per_block = vec4 ? 8 : 16;
+    // float4 path: 16-row bands (the band-streaming kernel) when that still
+    // gives two blocks per CU, else the unrolled 8-row kernel.
+    rows_per_block = vec4 ? (static_cast<size_t>(ntiles) * cdiv(nt - 2, 16) >= 512 ? 16 : 8) : 16;
     if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK")) rows_per_block = std::max(1, std::atoi(e));
-    if (vec4) rows_per_block = rows_per_block <= 4 ? 4 : 8;
+    if (vec4) rows_per_block = rows_per_block <= 4 ? 4 : (rows_per_block <= 8 ? 8 : std::min(rows_per_block, kBandMax));
     for (int k = 0; k < 2; ++k) {
         hi[k] = dalloc<float>(n);
         lo[k] = dalloc<float>(n);

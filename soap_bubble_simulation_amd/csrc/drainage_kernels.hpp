@@ -701,4 +701,269 @@ __global__ __launch_bounds__(kBlockMax) void drainage_vec4_kernel(const StepArgs
 
     if (DIAG) diag_epilogue(a, mass, lam, lane, wave, nwaves, red_d);
 }
+// Band-streaming variant of drainage_vec4_kernel: the same row update on a
+// band of a.rows_per_block rows (up to kBandMax) streamed through a window of
+// five row buffers, so tall bands cost no extra registers and re-read only
+// two halo rows per band.
+constexpr int kBandMax = 64;
+
+template <bool REBASE, int GMODE, bool MARANGONI, bool DIAG>
+__global__ __launch_bounds__(kBlockMax) void drainage_band_kernel(const StepArgs a) {
+    constexpr bool GREAD = GMODE >= 1;
+    constexpr bool GWRITE = GMODE == 2;
+    const int np = a.np, nt = a.nt;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each
+    // XCD walks a contiguous band of rows and the theta-neighbour rows it
+    // re-reads stay in its own L2.
+    const unsigned nblk = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+    const unsigned per_xcd = (nblk + kXcds - 1) / kXcds;
+    unsigned v = (b % kXcds) * per_xcd + b / kXcds;
+    if (nblk % kXcds != 0) v = b;                 // uneven split: keep the launch order
+    const unsigned bx = v % gridDim.x, by = v / gridDim.x;
+    const int c0 = (static_cast<int>(bx) * nwaves + wave) * 256 + 4 * lane;
+    const bool valid = c0 < np;
+    const int cc = valid ? c0 : np - 4;
+    const int jw = cc == 0 ? np - 1 : cc - 1;
+    const int je = cc + 4 == np ? 0 : cc + 4;
+    const int RB = a.rows_per_block;              // band height, <= kBandMax
+    const int r0 = 1 + static_cast<int>(by) * RB;
+    const int r1 = min(r0 + RB, nt - 1);          // rows [r0, r1) are updated
+    const bool touches_top = r0 == 1;
+    const bool touches_bot = r1 == nt - 1;
+
+    __shared__ float2 red_ff[4][kBlockMax / 64];
+    __shared__ double red_d[2][kBlockMax / 64];
+
+    float2 ptop = make_float2(0.f, 0.f), pbot = make_float2(0.f, 0.f);
+    float gtop = 0.f, gbot = 0.f;
+    if (a.h_pole_mean) {
+        if (touches_top) ptop = pole_mean(a.hpole_in, a.ntiles, a.np_f);
+        if (touches_bot) pbot = pole_mean(a.hpole_in + a.ntiles, a.ntiles, a.np_f);
+    }
+    if (GREAD && a.g_pole_mean) {
+        if (touches_top) { const float2 m = pole_mean(a.gpole_in, a.ntiles, a.np_f); gtop = m.x + m.y; }
+        if (touches_bot) { const float2 m = pole_mean(a.gpole_in + a.ntiles, a.ntiles, a.np_f); gbot = m.x + m.y; }
+    }
+
+    // The band's per-row coefficients go to LDS before any row is streamed,
+    // so that reading them inside the row loop never waits on the vector
+    // memory counter that the prefetched rows are queued on.
+    __shared__ RowF s_rowf[kBandMax];
+    __shared__ RowD s_rowd[DIAG ? kBandMax : 1];
+    if (threadIdx.x < RB) {
+        const int ri = min(r0 + static_cast<int>(threadIdx.x), nt - 1);
+        s_rowf[threadIdx.x] = a.rows[ri];
+        if (DIAG) s_rowd[threadIdx.x] = a.rowd[ri];
+    }
+
+    // Five row buffers: north, centre, south and two rows in flight.  The
+    // loop is unrolled over the five rotations, so no buffer is ever copied
+    // while its load is outstanding.
+    Row4 P0, P1, P2, P3, P4;
+    load_row4<GMODE>(a, r0 - 1, cc, jw, je, P0);
+    load_row4<GMODE>(a, r0, cc, jw, je, P1);
+    load_row4<GMODE>(a, r0 + 1, cc, jw, je, P2);
+    load_row4<GMODE>(a, r0 + 2, cc, jw, je, P3);
+    load_row4<GMODE>(a, r0 + 3, cc, jw, je, P4);
+    __syncthreads();
+
+    const float G0 = a.inv_dth2;
+    float top_h[4] = {0.f, 0.f, 0.f, 0.f}, top_l[4] = {0.f, 0.f, 0.f, 0.f}, top_g[4] = {0.f, 0.f, 0.f, 0.f};
+    float bot_h[4] = {0.f, 0.f, 0.f, 0.f}, bot_l[4] = {0.f, 0.f, 0.f, 0.f}, bot_g[4] = {0.f, 0.f, 0.f, 0.f};
+    double mass = 0.0, lam = 0.0;
+
+    // Updates row i from (N, C, Sx) = rows (i - 1, i, i + 1).
+    auto update_row = [&](int i, Row4& N, const Row4& C, Row4& Sx) {
+        // pole rows are substituted in place: row 0 leaves the window after
+        // row 1, and row nt - 2 is the last row any band updates
+        if (i == 1) pole_row(N, ptop, gtop, a.h_pole_mean, GREAD && a.g_pole_mean);
+        if (i == nt - 2) pole_row(Sx, pbot, gbot, a.h_pole_mean, GREAD && a.g_pole_mean);
+        const int t = i - r0;
+        const RowF rc = s_rowf[t];
+        const float G1 = rc.cot * a.inv_2dth;
+        const float G2 = rc.inv_s2 * a.inv_dph2;
+        const float dtD = a.dt * a.d_r2;
+        const float cS = a.dt * a.coef * rc.sin_t;
+
+        // West / east neighbours of the lane's first and last column.
+        const float hw = C.hw, lw = C.lw, he = C.he, le = C.le;
+        const float gw = GREAD ? C.gw : 0.f, ge = GREAD ? C.ge : 0.f;
+
+        const float ch[4] = {C.h.x, C.h.y, C.h.z, C.h.w};
+        const float cl[4] = {C.l.x, C.l.y, C.l.z, C.l.w};
+        const float nh[4] = {N.h.x, N.h.y, N.h.z, N.h.w};
+        const float nl[4] = {N.l.x, N.l.y, N.l.z, N.l.w};
+        const float sh[4] = {Sx.h.x, Sx.h.y, Sx.h.z, Sx.h.w};
+        const float sl[4] = {Sx.l.x, Sx.l.y, Sx.l.z, Sx.l.w};
+        const float wh[4] = {hw, C.h.x, C.h.y, C.h.z};
+        const float wl[4] = {lw, C.l.x, C.l.y, C.l.z};
+        const float eh[4] = {C.h.y, C.h.z, C.h.w, he};
+        const float el[4] = {C.l.y, C.l.z, C.l.w, le};
+        float cg[4] = {0.f, 0.f, 0.f, 0.f}, ng[4] = {0.f, 0.f, 0.f, 0.f}, sg[4] = {0.f, 0.f, 0.f, 0.f};
+        float wg[4] = {0.f, 0.f, 0.f, 0.f}, eg[4] = {0.f, 0.f, 0.f, 0.f};
+        if (GREAD) {
+            cg[0] = C.g.x; cg[1] = C.g.y; cg[2] = C.g.z; cg[3] = C.g.w;
+            ng[0] = N.g.x; ng[1] = N.g.y; ng[2] = N.g.z; ng[3] = N.g.w;
+            sg[0] = Sx.g.x; sg[1] = Sx.g.y; sg[2] = Sx.g.z; sg[3] = Sx.g.w;
+            wg[0] = gw; wg[1] = C.g.x; wg[2] = C.g.y; wg[3] = C.g.z;
+            eg[0] = C.g.y; eg[1] = C.g.z; eg[2] = C.g.w; eg[3] = ge;
+        }
+
+        float lo_new[4], g_new[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            // Rupture mask on the input state (drainage.rs:331-333): frozen
+            // cells keep h and Γ; the update is computed for every cell and
+            // selected, which keeps the wave free of divergent branches.
+            const bool rupt = ff_less(ch[v], cl[v], a.hc0, a.hc1, a.hc2);
+            const float dtp = (sh[v] - ch[v]) + (sl[v] - cl[v]);   // toward theta + 1
+            const float dtm = (ch[v] - nh[v]) + (cl[v] - nl[v]);
+            const float dpp = (eh[v] - ch[v]) + (el[v] - cl[v]);
+            const float dpm = (ch[v] - wh[v]) + (cl[v] - wl[v]);
+            const float lap = (dtp - dtm) * G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
+            const float H = ch[v] + cl[v];
+            float rate_dt = fmaf(dtD, lap, -cS * (H * H * H));
+            if (MARANGONI) {
+                const float gth = (sg[v] - ng[v]) * a.inv_2dth * a.inv_r;
+                const float gph = (eg[v] - wg[v]) * a.inv_2dph * a.inv_r * rc.inv_s;
+                const float mag = sqrtf(gth * gth + gph * gph);
+                rate_dt += a.dt * a.mgr * mag * (H * H);
+            }
+            const float ln = fmaxf(cl[v] + rate_dt, -ch[v]);          // h >= 0 (drainage.rs:378)
+            lo_new[v] = rupt ? cl[v] : ln;
+            g_new[v] = cg[v];
+            if (GWRITE) {
+                const float gtp = sg[v] - cg[v], gtm = cg[v] - ng[v];
+                const float gpp = eg[v] - cg[v], gpm = cg[v] - wg[v];
+                const float glap = (gtp - gtm) * G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
+                float grate = a.ds_r2 * glap;
+                if (MARANGONI) grate += 0.001f * lap * cg[v];
+                const float gn = fminf(fmaxf(fmaf(a.dt, grate, cg[v]), 0.f), 1.f);
+                g_new[v] = rupt ? cg[v] : gn;
+            }
+        }
+
+        // Keep the ring rows' new values (before the fronts act, as
+        // update_poles runs before update_marginal_regeneration) for the
+        // pole means; they are reduced once after the row loop.
+        if (i == 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { top_h[v] = ch[v]; top_l[v] = lo_new[v]; top_g[v] = g_new[v]; }
+        }
+        if (i == nt - 2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { bot_h[v] = ch[v]; bot_l[v] = lo_new[v]; bot_g[v] = g_new[v]; }
+        }
+
+        if (a.nfronts > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (in_fronts(a, i, cc + v)) {
+                    const float p = ch[v] * a.tfe_g_hi;
+                    const float pe = fmaf(ch[v], a.tfe_g_hi, -p);
+                    const float corr = pe + (ch[v] * a.tfe_g_lo + lo_new[v] * a.tfe_g_hi);
+                    lo_new[v] = lo_new[v] + (p + corr);
+                }
+            }
+        }
+
+        if (valid) {
+            const size_t off = static_cast<size_t>(i) * np + cc;
+            if (REBASE) {
+                float4 hh, ll;
+                two_sum(ch[0], lo_new[0], hh.x, ll.x);
+                two_sum(ch[1], lo_new[1], hh.y, ll.y);
+                two_sum(ch[2], lo_new[2], hh.z, ll.z);
+                two_sum(ch[3], lo_new[3], hh.w, ll.w);
+                *reinterpret_cast<float4*>(a.hi_out + off) = hh;
+                *reinterpret_cast<float4*>(a.lo_out + off) = ll;
+            } else {
+                *reinterpret_cast<float4*>(a.lo_out + off) =
+                    make_float4(lo_new[0], lo_new[1], lo_new[2], lo_new[3]);
+            }
+            if (GWRITE)
+                *reinterpret_cast<float4*>(a.g_out + off) = make_float4(g_new[0], g_new[1], g_new[2], g_new[3]);
+            if (DIAG) {
+                // The lane's four cells share the row's area and CFL terms:
+                // their FP64 sum is multiplied once per row.
+                double hsum = 0.0;
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    hsum += static_cast<double>(ch[v]) + static_cast<double>(lo_new[v]);
+                const float hmax = fmaxf(fmaxf(ch[0] + lo_new[0], ch[1] + lo_new[1]),
+                                         fmaxf(ch[2] + lo_new[2], ch[3] + lo_new[3]));
+                const RowD rd = s_rowd[DIAG ? t : 0];
+                mass = fma(hsum, rd.area, mass);
+                const double hm = static_cast<double>(hmax);
+                lam = fmax(lam, rd.lam_grid + a.coef3 * rd.sin_abs * hm * hm);
+            }
+        }
+    };
+#define SF_BAND_ROW(Nb, Cb, Sb)                                                  \
+    {                                                                            \
+        if (i >= r1) break;                                                      \
+        update_row(i, Nb, Cb, Sb);                                               \
+        if (i + 4 <= r1) load_row4<GMODE>(a, i + 4, cc, jw, je, Nb);             \
+        ++i;                                                                     \
+    }
+    for (int i = r0;;) {
+        SF_BAND_ROW(P0, P1, P2)
+        SF_BAND_ROW(P1, P2, P3)
+        SF_BAND_ROW(P2, P3, P4)
+        SF_BAND_ROW(P3, P4, P0)
+        SF_BAND_ROW(P4, P0, P1)
+    }
+#undef SF_BAND_ROW
+
+    if (touches_top || touches_bot) {
+        float2 sum_top = make_float2(0.f, 0.f), sum_bot = make_float2(0.f, 0.f);
+        float2 gsum_top = make_float2(0.f, 0.f), gsum_bot = make_float2(0.f, 0.f);
+        if (valid) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float s_, e_;
+                two_sum(top_h[v], top_l[v], s_, e_);
+                sum_top = ff_add(sum_top, make_float2(s_, e_));
+                two_sum(bot_h[v], bot_l[v], s_, e_);
+                sum_bot = ff_add(sum_bot, make_float2(s_, e_));
+                gsum_top = ff_add(gsum_top, make_float2(top_g[v], 0.f));
+                gsum_bot = ff_add(gsum_bot, make_float2(bot_g[v], 0.f));
+            }
+        }
+        sum_top = wave_ff_sum(sum_top);
+        sum_bot = wave_ff_sum(sum_bot);
+        if (GWRITE) {
+            gsum_top = wave_ff_sum(gsum_top);
+            gsum_bot = wave_ff_sum(gsum_bot);
+        }
+        if (lane == 0) {
+            red_ff[0][wave] = sum_top;
+            red_ff[1][wave] = sum_bot;
+            red_ff[2][wave] = gsum_top;
+            red_ff[3][wave] = gsum_bot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float2 t[4];
+            for (int q = 0; q < 4; ++q) {
+                t[q] = red_ff[q][0];
+                for (int w = 1; w < nwaves; ++w) t[q] = ff_add(t[q], red_ff[q][w]);
+            }
+            if (a.h_pole_sum) {
+                if (touches_top) a.hpole_out[bx] = t[0];
+                if (touches_bot) a.hpole_out[a.ntiles + bx] = t[1];
+            }
+            if (GWRITE && a.g_pole_sum) {
+                if (touches_top) a.gpole_out[bx] = t[2];
+                if (touches_bot) a.gpole_out[a.ntiles + bx] = t[3];
+            }
+        }
+    }
+
+    if (DIAG) diag_epilogue(a, mass, lam, lane, wave, nwaves, red_d);
+}
 }  // namespace sf
