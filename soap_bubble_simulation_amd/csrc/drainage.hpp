@@ -85,7 +85,9 @@ struct Drainage {
     // spectral interference
     float* rgb = nullptr;
     float* caustic = nullptr;       // caustic-intensity map (caustics.hip), allocated on first use
-    float* cmf_dev = nullptr;
+    float* cmf_dev = nullptr;       // colour curves as MFMA B fragments
+    float cmf_sum[3] = {0.f, 0.f, 0.f};   // sum over the 81 wavelengths of each curve
+    int cmf_lo = 0;                 // the curves are not exact in BF16
     float4* spec_rows = nullptr;
     float2* spec_cols = nullptr;
 

@@ -7,7 +7,7 @@
 //                       per-point colour, intensity and intermediate terms
 //   spectral_kernel     81-wavelength Airy/Fresnel reflectance at every grid
 //                       cell's view angle, contracted with cmf[81][3] on BF16
-//                       MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation
+//                       MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -226,47 +226,122 @@ __global__ void calc_kernel(CalcArgs a) {
 // ---------------------------------------------------------------------------
 // Spectral 81-wavelength interference with the BF16 MFMA contraction.
 //
-// One wave handles 16 consecutive cells per iteration.  For the
-// 16x16x16 BF16 MFMA, lane l holds A[m][k] for m = l % 16 and
-// k = 4 (l / 16) + v (v = 0..3), B[k][n] for n = l % 16, and receives
-// D[4 (l / 16) + v][n].  A is the Airy reflectance of cell m at wavelength
-// 16 t + k (t = 0..5 chunks, 96 slots for 81 wavelengths), B the colour
-// curves, so each lane evaluates 21 reflectances per 16 cells with no
-// redundancy beyond the padding.  A and B are split into BF16 hi + lo parts
-// and three MFMAs (hi*hi + hi*lo + lo*hi) keep ~16 mantissa bits.
+// The Airy reflectance is I = F c^2 / (1 + F c^2) = 1 - r with
+// r = 1 / (1 + F c^2), c = cos(pi opd / lambda), so with S_c = sum_k cmf[k][c]
+//   sum_k cmf[k][c] I_k = S_c - sum_k cmf[k][c] r_k,
+// and the MFMA contracts r.  Where F <= kSpecFCos2 for the whole wave,
+// 1 + F c^2 = (1 + F/2) + (F/2) cos(2 pi opd / lambda) saves the square; past
+// that the cancellation in 1 + cos would cost accuracy, and the square form
+// runs (with v_fract_f32 range reduction, which also covers long paths).
+//
+// v_mfma_f32_16x16x32_bf16: lane l holds A[m][8 q + v] and B[8 q + v][m]
+// (m = l % 16, q = l / 16, v = 0..7) and receives D[4 q + v][m].  A row m is
+// a cell of a 16-cell group; lane (m, q) evaluates r for wavelengths
+// k = q + 4 s, s = 0..20 (84 slots, 81 used, B zero on the rest), slot s in
+// MFMA t = s / 8 at v = s % 8; slots 21..23 are zero.  r is split into
+// truncated BF16 hi and lo = r - hi (exact), each contracted with the BF16
+// curves (plus hi times the curves' own lo part when they are not exact in
+// BF16), which keeps about 16 bits of r.  A wave takes 64 cells at a time
+// as four 16-cell groups.
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSpecLambdas = 81;
-constexpr int kSpecChunks = 6;
+constexpr int kSpecSlots = 21;      // wavelength slots per lane and cell
+constexpr int kSpecMfma = 3;        // 16x16x32 MFMAs per group and operand part
+constexpr float kSpecFCos2 = 16.f;
+// Paths below which v_cos_f32 (argument in revolutions, exact to its
+// specification for |x| <= 256) takes opd / lambda unreduced.
+constexpr float kSpecOpFast = 5.0e4f;
 
 struct SpecArgs {
     const float* hi;
     const float* lo;
     const float4* rows;     // per theta row: sin, cos
     const float2* cols;     // per phi column: sin, cos
-    const uint2* bfrag;     // [2 (hi, lo)][6 chunks][64 lanes] packed bf16x4
+    const u32x4* bfrag;     // [2 (hi, lo)][3 MFMAs][64 lanes] B fragments, bf16 x 8
     float* rgb;             // 3 planes of n cells
     int np;
     size_t n;
     float cx, cy, cz, cam2;  // camera and |cam|^2
     float radius, n_film, inv_n_film;
+    float s0, s1, s2;       // sum over the wavelengths of each (BF16 hi + lo) curve
+    int b_lo;               // the curves have a nonzero BF16 lo part
 };
-
-__device__ inline uint32_t bf16_hi_bits(float x) { return __float_as_uint(x) & 0xFFFF0000u; }
-
-// Truncating split x = hi + lo (+ O(2^-16 x)), packed as bf16 pairs.
-__device__ inline void split_pack(float a, float b, uint32_t& hi2, uint32_t& lo2) {
-    const uint32_t ah = bf16_hi_bits(a), bh = bf16_hi_bits(b);
-    const float al = a - __uint_as_float(ah);
-    const float bl = b - __uint_as_float(bh);
-    hi2 = (ah >> 16) | bh;
-    lo2 = (bf16_hi_bits(al) >> 16) | bf16_hi_bits(bl);
-}
 
 __device__ inline float lane_read(float v, int src_lane) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+}
+
+// Per-cell terms of the Airy sum: coefficient of finesse F and optical path
+// difference opc (nm) of cell cc at its view angle.
+__device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, float inv_np, float& Fc, float& opc) {
+    int i = static_cast<int>(static_cast<float>(cc) * inv_np);
+    int j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
+    if (j < 0) { --i; j += a.np; }
+    if (j >= a.np) { ++i; j -= a.np; }
+    const float4 rw = a.rows[i];
+    const float2 cl = a.cols[j];
+    // View angle |dot(normalize(cam - p), n)|, p = R n (bubble.wgsl:491-494).
+    const float nx = rw.x * cl.y, ny = rw.y, nz = rw.x * cl.x;
+    const float cn = a.cx * nx + a.cy * ny + a.cz * nz;
+    const float len2 = a.cam2 - 2.f * a.radius * cn + a.radius * a.radius;
+    const float cos_i = fminf(fabsf(cn - a.radius) * rsqrtf(len2), 1.f);
+    // Snell and unpolarised Fresnel (interference_lut.rs:96-120).
+    const float sin_i = sqrtf(fmaxf(1.f - cos_i * cos_i, 0.f));
+    const float sin_t = sin_i * a.inv_n_film;
+    const float cos_t = sqrtf(fmaxf(1.f - sin_t * sin_t, 0.f));
+    const float rs = (cos_i - a.n_film * cos_t) * __builtin_amdgcn_rcpf(fmaxf(cos_i + a.n_film * cos_t, 1e-4f));
+    const float rp = (a.n_film * cos_i - cos_t) * __builtin_amdgcn_rcpf(fmaxf(a.n_film * cos_i + cos_t, 1e-4f));
+    const float R = (rs * rs + rp * rp) * 0.5f;
+    const float omr = fmaxf(1.f - R, 1e-3f);
+    Fc = 4.f * R * __builtin_amdgcn_rcpf(omr * omr);
+    const float d_nm = (a.hi[cc] + a.lo[cc]) * 1e9f;
+    opc = 2.f * a.n_film * d_nm * cos_t;
+}
+
+// r of one slot.  FAST: P = 2 opd, A = 1 + F/2, B = F/2 (cos 2x form);
+// otherwise P = opd, A = 1, B = F (square form, reduced argument).
+template <bool FAST>
+__device__ inline float spec_r(float P, float A, float B, float u) {
+    if (FAST) {
+        const float C = __builtin_amdgcn_cosf(P * u);
+        return __builtin_amdgcn_rcpf(fmaf(B, C, A));
+    }
+    const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(P * u));
+    return __builtin_amdgcn_rcpf(fmaf(B, c * c, A));
+}
+
+// High 16 bits of two floats packed as bf16 x 2 (truncation): x0 low, x1 high.
+__device__ inline uint32_t pack_hi16(float x0, float x1) {
+    return __builtin_amdgcn_perm(__float_as_uint(x1), __float_as_uint(x0), 0x07060302u);
+}
+
+template <bool FAST>
+__device__ inline void spec_group(const float (&u)[kSpecSlots], const u32x4 (&bh)[kSpecMfma],
+                                  const u32x4 (&bl)[kSpecMfma], bool b_lo, float P, float A, float B,
+                                  v4f& acc) {
+#pragma unroll
+    for (int t = 0; t < kSpecMfma; ++t) {
+        float r[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) r[v] = (8 * t + v < kSpecSlots) ? spec_r<FAST>(P, A, B, u[8 * t + v]) : 0.f;
+        u32x4 h, l;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const float h0 = __uint_as_float(__float_as_uint(r[2 * p]) & 0xFFFF0000u);
+            const float h1 = __uint_as_float(__float_as_uint(r[2 * p + 1]) & 0xFFFF0000u);
+            h[p] = pack_hi16(r[2 * p], r[2 * p + 1]);
+            l[p] = pack_hi16(r[2 * p] - h0, r[2 * p + 1] - h1);
+        }
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, h);
+        const bf16x8 al = __builtin_bit_cast(bf16x8, l);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, __builtin_bit_cast(bf16x8, bh[t]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, __builtin_bit_cast(bf16x8, bh[t]), acc, 0, 0, 0);
+        if (b_lo) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, __builtin_bit_cast(bf16x8, bl[t]), acc, 0, 0, 0);
+    }
 }
 
 __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
@@ -276,93 +351,56 @@ __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
     const size_t wave_global = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const size_t nwaves = (static_cast<size_t>(gridDim.x) * blockDim.x) >> 6;
 
-    // B fragments and 1 / (2 lambda) of this lane's wavelength slots.
-    v4s bh[kSpecChunks], bl[kSpecChunks];
-    float u[kSpecChunks][4];
-    for (int t = 0; t < kSpecChunks; ++t) {
-        const uint2 h2 = a.bfrag[t * 64 + lane];
-        const uint2 l2 = a.bfrag[kSpecChunks * 64 + t * 64 + lane];
-        bh[t] = v4s{static_cast<short>(h2.x & 0xFFFF), static_cast<short>(h2.x >> 16),
-                    static_cast<short>(h2.y & 0xFFFF), static_cast<short>(h2.y >> 16)};
-        bl[t] = v4s{static_cast<short>(l2.x & 0xFFFF), static_cast<short>(l2.x >> 16),
-                    static_cast<short>(l2.y & 0xFFFF), static_cast<short>(l2.y >> 16)};
-        for (int v = 0; v < 4; ++v) {
-            const int k = 16 * t + 4 * q + v;
-            u[t][v] = k < kSpecLambdas ? 0.5f / (380.f + 5.f * static_cast<float>(k)) : 0.f;
-        }
+    // 1 / (2 lambda) of this lane's slots (0 on the padding) and its B fragments.
+    float u[kSpecSlots];
+#pragma unroll
+    for (int s = 0; s < kSpecSlots; ++s) {
+        const int k = q + 4 * s;
+        u[s] = k < kSpecLambdas ? 0.5f / (380.f + 5.f * static_cast<float>(k)) : 0.f;
     }
+    u32x4 bh[kSpecMfma], bl[kSpecMfma];
+#pragma unroll
+    for (int t = 0; t < kSpecMfma; ++t) {
+        bh[t] = a.bfrag[t * 64 + lane];
+        bl[t] = a.bfrag[(kSpecMfma + t) * 64 + lane];
+    }
+    const bool b_lo = a.b_lo != 0;
     const float scale = 1.f / static_cast<float>(kSpecLambdas);
+    const float S = m == 0 ? a.s0 : (m == 1 ? a.s1 : a.s2);
     const float inv_np = 1.f / static_cast<float>(a.np);
+    const bool vec_out = (a.n & 3) == 0;
 
-    // Each wave takes 64 consecutive cells at a time: every lane prepares
-    // one cell (view angle, Fresnel, finesse, optical path), then the four
-    // 16-cell MFMA groups fetch their cell's values from the owning lane.
     for (size_t sg = wave_global; sg * 64 < a.n; sg += nwaves) {
         const size_t cell = sg * 64 + lane;
-        const size_t cc = cell < a.n ? cell : a.n - 1;
-        int i = static_cast<int>(static_cast<float>(cc) * inv_np);
-        int j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
-        if (j < 0) { --i; j += a.np; }
-        if (j >= a.np) { ++i; j -= a.np; }
-        const float4 rw = a.rows[i];
-        const float2 cl = a.cols[j];
-        // View angle |dot(normalize(cam - p), n)|, p = R n (bubble.wgsl:491-494).
-        const float nx = rw.x * cl.y, ny = rw.y, nz = rw.x * cl.x;
-        const float cn = a.cx * nx + a.cy * ny + a.cz * nz;
-        const float len2 = a.cam2 - 2.f * a.radius * cn + a.radius * a.radius;
-        const float cos_i = fminf(fabsf(cn - a.radius) * rsqrtf(len2), 1.f);
-        // Snell and unpolarised Fresnel (interference_lut.rs:96-120).
-        const float sin_i = sqrtf(fmaxf(1.f - cos_i * cos_i, 0.f));
-        const float sin_t = sin_i * a.inv_n_film;
-        const float cos_t = sqrtf(fmaxf(1.f - sin_t * sin_t, 0.f));
-        const float rs = (cos_i - a.n_film * cos_t) * __builtin_amdgcn_rcpf(fmaxf(cos_i + a.n_film * cos_t, 1e-4f));
-        const float rp = (a.n_film * cos_i - cos_t) * __builtin_amdgcn_rcpf(fmaxf(a.n_film * cos_i + cos_t, 1e-4f));
-        const float R = (rs * rs + rp * rp) * 0.5f;
-        const float omr = fmaxf(1.f - R, 1e-3f);
-        const float Fc = 4.f * R * __builtin_amdgcn_rcpf(omr * omr);
-        const float d_nm = (a.hi[cc] + a.lo[cc]) * 1e9f;
-        const float opc = 2.f * a.n_film * d_nm * cos_t;
-
+        float Fc, opc;
+        spec_cell_terms(a, cell < a.n ? cell : a.n - 1, inv_np, Fc, opc);
+        const bool fast = !__any(!(Fc <= kSpecFCos2 && opc <= kSpecOpFast));
+        v4f acc[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const size_t grp = sg * 4 + g;
-            if (grp * 16 >= a.n) break;
             const float F = lane_read(Fc, 16 * g + m);
             const float op = lane_read(opc, 16 * g + m);
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
+            if (fast)
+                spec_group<true>(u, bh, bl, b_lo, 2.f * op, fmaf(0.5f, F, 1.f), 0.5f * F, acc[g]);
+            else
+                spec_group<false>(u, bh, bl, b_lo, op, 1.f, F, acc[g]);
+        }
+        // acc[g][v] = D[4 q + v][m]: cell 64 sg + 16 g + 4 q + v, colour m.
+        if (m < 3) {
+            float* plane = a.rgb + static_cast<size_t>(m) * a.n;
 #pragma unroll
-            for (int t = 0; t < kSpecChunks; ++t) {
-                float I[4];
-                const int nv = (t == kSpecChunks - 1) ? 1 : 4;   // slots 84..95 are padding
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    if (v < nv) {
-                        // sin(phase / 2) with phase = 2 pi op / lambda + pi is
-                        // cos(pi op / lambda) = cos(2 pi x), x = op / (2 lambda);
-                        // v_cos_f32 takes its argument in revolutions.
-                        const float c = __builtin_amdgcn_cosf(op * u[t][v]);
-                        const float fs2 = F * (c * c);
-                        I[v] = fs2 * __builtin_amdgcn_rcpf(1.f + fs2);
-                    } else {
-                        I[v] = 0.f;
-                    }
-                }
-                uint32_t h01, l01, h23, l23;
-                split_pack(I[0], I[1], h01, l01);
-                split_pack(I[2], I[3], h23, l23);
-                const v4s ah = v4s{static_cast<short>(h01 & 0xFFFF), static_cast<short>(h01 >> 16),
-                                   static_cast<short>(h23 & 0xFFFF), static_cast<short>(h23 >> 16)};
-                const v4s al = v4s{static_cast<short>(l01 & 0xFFFF), static_cast<short>(l01 >> 16),
-                                   static_cast<short>(l23 & 0xFFFF), static_cast<short>(l23 >> 16)};
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[t], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[t], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[t], acc, 0, 0, 0);
-            }
-            // acc[v] = D[4 q + v][m]: cell grp*16 + 4q + v, colour m.
-            if (m < 3) {
-                for (int v = 0; v < 4; ++v) {
-                    const size_t oc = grp * 16 + 4 * q + v;
-                    if (oc < a.n) a.rgb[static_cast<size_t>(m) * a.n + oc] = acc[v] * scale;
+            for (int g = 0; g < 4; ++g) {
+                const size_t c0 = sg * 64 + 16 * g + 4 * q;
+                const float4 o = make_float4((S - acc[g][0]) * scale, (S - acc[g][1]) * scale,
+                                             (S - acc[g][2]) * scale, (S - acc[g][3]) * scale);
+                if (vec_out && c0 + 3 < a.n) {
+                    *reinterpret_cast<float4*>(plane + c0) = o;
+                } else {
+                    if (c0 < a.n) plane[c0] = o.x;
+                    if (c0 + 1 < a.n) plane[c0 + 1] = o.y;
+                    if (c0 + 2 < a.n) plane[c0 + 2] = o.z;
+                    if (c0 + 3 < a.n) plane[c0 + 3] = o.w;
                 }
             }
         }
@@ -568,34 +606,64 @@ extern "C" int sf_phase_difference(int32_t n, const float* opd_nm, const float* 
     });
 }
 
-// Packs cmf[81][3] into the MFMA B-operand layout (see spectral_kernel),
-// split into truncated-BF16 hi and lo parts.
-static std::vector<uint2> pack_cmf(const float* cmf) {
-    std::vector<uint2> frag(2 * kSpecChunks * 64);
-    auto bits = [](float x) {
-        uint32_t u;
-        std::memcpy(&u, &x, 4);
-        return u;
-    };
-    for (int t = 0; t < kSpecChunks; ++t)
+// Colour curves on the device: B fragments of spectral_kernel,
+// [2 (hi, lo)][3 MFMAs][64 lanes] of bf16 x 8.
+constexpr size_t kCmfFrags = 2 * kSpecMfma * 64;
+
+static uint16_t bf16_rne(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+static float bf16_value(uint16_t h) {
+    const uint32_t u = static_cast<uint32_t>(h) << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+struct PackedCmf {
+    std::vector<u32x4> frag;
+    float sum[3];
+    int has_lo;
+};
+
+// Packs cmf[81][3] into the B-operand layout of spectral_kernel: lane
+// 16 q + m of MFMA t holds cmf[q + 4 (8 t + v)][m] at v = 0..7 (zero past
+// slot 20, wavelength 80 or colour 2), as round-to-nearest BF16 hi and the
+// BF16 of the remainder.  The sums are those of hi + lo, the values the
+// MFMA multiplies.
+static PackedCmf pack_cmf(const float* cmf) {
+    PackedCmf out;
+    out.frag.assign(kCmfFrags, u32x4{0u, 0u, 0u, 0u});
+    out.has_lo = 0;
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < kSpecMfma; ++t)
         for (int lane = 0; lane < 64; ++lane) {
-            uint16_t hv[4], lv[4];
-            for (int v = 0; v < 4; ++v) {
-                const int k = 16 * t + 4 * (lane >> 4) + v;
-                const int col = lane & 15;
-                const float x = (k < kSpecLambdas && col < 3) ? cmf[3 * k + col] : 0.f;
-                const uint32_t hb = bits(x) & 0xFFFF0000u;
-                float hf;
-                std::memcpy(&hf, &hb, 4);
-                const float rem = x - hf;
-                hv[v] = static_cast<uint16_t>(hb >> 16);
-                lv[v] = static_cast<uint16_t>(bits(rem) >> 16);
+            const int m = lane & 15, q = lane >> 4;
+            uint16_t hv[8], lv[8];
+            for (int v = 0; v < 8; ++v) {
+                const int s = 8 * t + v;
+                const int k = q + 4 * s;
+                const float x = (s < kSpecSlots && k < kSpecLambdas && m < 3) ? cmf[3 * k + m] : 0.f;
+                hv[v] = bf16_rne(x);
+                lv[v] = bf16_rne(x - bf16_value(hv[v]));
+                if (lv[v] & 0x7FFFu) out.has_lo = 1;
+                if (s < kSpecSlots && k < kSpecLambdas && m < 3)
+                    sum[m] += static_cast<double>(bf16_value(hv[v])) + static_cast<double>(bf16_value(lv[v]));
             }
-            frag[t * 64 + lane] = make_uint2(hv[0] | (uint32_t(hv[1]) << 16), hv[2] | (uint32_t(hv[3]) << 16));
-            frag[kSpecChunks * 64 + t * 64 + lane] =
-                make_uint2(lv[0] | (uint32_t(lv[1]) << 16), lv[2] | (uint32_t(lv[3]) << 16));
+            u32x4 h, l;
+            for (int p = 0; p < 4; ++p) {
+                h[p] = hv[2 * p] | (static_cast<uint32_t>(hv[2 * p + 1]) << 16);
+                l[p] = lv[2 * p] | (static_cast<uint32_t>(lv[2 * p + 1]) << 16);
+            }
+            out.frag[t * 64 + lane] = h;
+            out.frag[(kSpecMfma + t) * 64 + lane] = l;
         }
-    return frag;
+    for (int c = 0; c < 3; ++c) out.sum[c] = static_cast<float>(sum[c]);
+    return out;
 }
 
 namespace sf {
@@ -625,10 +693,15 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     if (!d->rgb) d->rgb = dalloc_i<float>(3 * d->n);
     ensure_view_tables(d);
     if (cmf) {
-        if (!d->cmf_dev) d->cmf_dev = reinterpret_cast<float*>(dalloc_i<uint2>(2 * kSpecChunks * 64));
-        const std::vector<uint2> frag = pack_cmf(cmf);
-        SF_HIP(hipMemcpyAsync(d->cmf_dev, frag.data(), frag.size() * sizeof(uint2), hipMemcpyHostToDevice, d->stream));
+        for (int k = 0; k < 3 * kSpecLambdas; ++k)
+            SF_REQUIRE(std::isfinite(cmf[k]), SF_ERR_INVALID, "colour-matching curve value %d is not finite", k);
+        if (!d->cmf_dev) d->cmf_dev = dalloc_i<float>(kCmfFrags * 4);
+        const PackedCmf packed = pack_cmf(cmf);
+        SF_HIP(hipMemcpyAsync(d->cmf_dev, packed.frag.data(), kCmfFrags * sizeof(u32x4), hipMemcpyHostToDevice,
+                              d->stream));
         SF_HIP(hipStreamSynchronize(d->stream));
+        for (int c = 0; c < 3; ++c) d->cmf_sum[c] = packed.sum[c];
+        d->cmf_lo = packed.has_lo;
     }
     SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
     d->materialize_poles();
@@ -637,7 +710,11 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.lo = d->lo[d->cur];
     a.rows = d->spec_rows;
     a.cols = d->spec_cols;
-    a.bfrag = reinterpret_cast<const uint2*>(d->cmf_dev);
+    a.bfrag = reinterpret_cast<const u32x4*>(d->cmf_dev);
+    a.s0 = d->cmf_sum[0];
+    a.s1 = d->cmf_sum[1];
+    a.s2 = d->cmf_sum[2];
+    a.b_lo = d->cmf_lo;
     a.rgb = d->rgb;
     a.np = d->np;
     a.n = d->n;

@@ -91,14 +91,23 @@ def smooth_bubble(n_theta: int, n_phi: int, seed: int, lmax: int = 16, amp: floa
     return ff_exact(base * (1.0 + amp * sh_noise(n_theta, n_phi, lmax, seed)))
 
 
-def spectral_cmf(seed: int) -> np.ndarray:
-    """cmf[k, c] for the 81 wavelengths and three seeded Gaussian curves."""
+def round_bf16(x) -> np.ndarray:
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def spectral_cmf(seed: int, bf16: bool = True) -> np.ndarray:
+    """cmf[k, c] for the 81 wavelengths and three seeded Gaussian curves,
+    stored at BF16 precision (the MFMA operand type) unless bf16 is False."""
     rng = np.random.default_rng(seed)
     centres = np.array([610.0, 550.0, 460.0]) + rng.uniform(-15.0, 15.0, 3)
     widths = rng.uniform(25.0, 45.0, 3)
     heights = rng.uniform(0.8, 1.2, 3)
     lam = WAVELENGTHS_NM[:, None]
-    return heights * np.exp(-0.5 * ((lam - centres) / widths) ** 2)
+    cmf = heights * np.exp(-0.5 * ((lam - centres) / widths) ** 2)
+    return round_bf16(cmf) if bf16 else cmf
 
 
 def camera(radius: float) -> Tuple[float, float, float]:

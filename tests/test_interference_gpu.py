@@ -192,6 +192,35 @@ def test_spectral_small_grid(oracle):
     assert np.max(np.abs(want)) > 0.05
 
 
+def test_spectral_curves_not_exact_in_bf16(oracle):
+    """Curves with a nonzero BF16 remainder take the third (hi x curve-lo) MFMA."""
+    nt, npf = 64, 128
+    ctx, h0, p, cam, _ = _spectral_case(nt, npf, 5)
+    cmf = W.spectral_cmf(5, bf16=False).astype(np.float32)
+    assert np.any(cmf.view(np.uint32) & 0xFFFF)
+    got = spectral_rgb(ctx, cam, cmf)
+    want = oracle.spectral_rgb(h0, p.bubble_radius, cam, 1.33, cmf.astype(np.float64))
+    err = np.max(np.abs(got - want))
+    print("spectral, FP32 curves, max |rgb - oracle| =", err)
+    assert err <= 1e-3
+
+
+def test_spectral_thick_film(oracle):
+    """Optical paths past 5e4 nm (films of 20-60 um) take the range-reduced
+    square form in every wave."""
+    nt, npf = 48, 96
+    ctx, h0, p, cam, cmf = _spectral_case(nt, npf, 6)
+    h = h0 * 60.0
+    ctx.set_fields(h, np.full_like(h, 0.5))
+    cm = cmf.astype(np.float32)
+    got = spectral_rgb(ctx, cam, cm)
+    want = oracle.spectral_rgb(h, p.bubble_radius, cam, 1.33, cm.astype(np.float64))
+    err = np.max(np.abs(got - want))
+    print("spectral, 20-60 um films, max |rgb - oracle| =", err)
+    assert err <= 1e-3
+    assert np.max(np.abs(want)) > 0.05
+
+
 def test_spectral_full_grid_sampled(oracle):
     """2048 x 4096 (the benchmark grid): 20000 sampled cells against FP64."""
     nt, npf = 2048, 4096
