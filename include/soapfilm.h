@@ -242,7 +242,8 @@ int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_in
 /* n_steps of {drainage step, spectral RGB of the new state} back to back on
  * the context's stream: the per-step workload of a rendered simulation.
  * dt as in sf_drainage_step; cmf may be NULL after the first call.
- * Asynchronous; the RGB planes stay on the device. */
+ * Asynchronous; the RGB planes of the last step stay on the device
+ * (sf_spectral_copy_rgb, sf_spectral_device_rgb). */
 int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, double dt, const sf_camera* cam,
                               double refractive_index, const float* cmf);
 /* Blocks until all work enqueued on ctx has finished. */
@@ -261,6 +262,9 @@ int sf_render_lut_rgba8(sf_drainage* ctx, const uint8_t* lut_rgba8, double refra
                         const sf_camera* cam, uint8_t* rgba_out);
 /* Device pointer of the last spectral RGB planes of ctx (3*n_theta*n_phi). */
 int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb);
+/* Copies the last spectral RGB planes to rgb_host[3*n_theta*n_phi] after the
+ * work enqueued on ctx (blocks). */
+int sf_spectral_copy_rgb(sf_drainage* ctx, float* rgb_host);
 /* Average duration in ms of the spectral kernel launches since the last call
  * (HIP events on the context's stream); enable switches recording on/off. */
 int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,

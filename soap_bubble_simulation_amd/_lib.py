@@ -178,6 +178,7 @@ SIGNATURES = {
     "sf_drainage_synchronize": (C.c_int, [_vp]),
     "sf_spectral_rgb": (C.c_int, [_vp, C.POINTER(Camera), C.c_double, _fp, _fp]),
     "sf_spectral_device_rgb": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sf_spectral_copy_rgb": (C.c_int, [_vp, _fp]),
     "sf_render_lut_rgba8": (C.c_int, [_vp, _u8p, C.c_double, C.c_float, C.POINTER(Camera), _u8p]),
     "sf_caustic_params_default": (C.c_int, [C.POINTER(CausticParams)]),
     "sf_caustics": (C.c_int, [_vp, C.POINTER(CausticParams), _fp]),

@@ -33,7 +33,7 @@ struct KernelTimer {
 };
 
 struct Drainage;
-void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf);
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st);
 
 struct Drainage {
     static constexpr int64_t kHistCap = 1 << 16;

@@ -262,6 +262,7 @@ struct SpecArgs {
     const float4* rows;     // per theta row: sin, cos
     const float2* cols;     // per phi column: sin, cos
     const u32x4* bfrag;     // [2 (hi, lo)][3 MFMAs][64 lanes] B fragments, bf16 x 8
+    const float* ulam;      // [21 slots][4 q] 1 / (2 lambda) of wavelength q + 4 s (0 past 80)
     float* rgb;             // 3 planes of n cells
     int np;
     size_t n;
@@ -354,10 +355,7 @@ __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
     // 1 / (2 lambda) of this lane's slots (0 on the padding) and its B fragments.
     float u[kSpecSlots];
 #pragma unroll
-    for (int s = 0; s < kSpecSlots; ++s) {
-        const int k = q + 4 * s;
-        u[s] = k < kSpecLambdas ? 0.5f / (380.f + 5.f * static_cast<float>(k)) : 0.f;
-    }
+    for (int s = 0; s < kSpecSlots; ++s) u[s] = a.ulam[4 * s + q];
     u32x4 bh[kSpecMfma], bl[kSpecMfma];
 #pragma unroll
     for (int t = 0; t < kSpecMfma; ++t) {
@@ -607,8 +605,10 @@ extern "C" int sf_phase_difference(int32_t n, const float* opd_nm, const float* 
 }
 
 // Colour curves on the device: B fragments of spectral_kernel,
-// [2 (hi, lo)][3 MFMAs][64 lanes] of bf16 x 8.
+// [2 (hi, lo)][3 MFMAs][64 lanes] of bf16 x 8, then the slot table of
+// 1 / (2 lambda) (FP64 quotients, rounded).
 constexpr size_t kCmfFrags = 2 * kSpecMfma * 64;
+constexpr size_t kCmfFloats = kCmfFrags * 4 + 4 * kSpecSlots;
 
 static uint16_t bf16_rne(float x) {
     uint32_t u;
@@ -626,6 +626,7 @@ static float bf16_value(uint16_t h) {
 
 struct PackedCmf {
     std::vector<u32x4> frag;
+    float ulam[4 * kSpecSlots];
     float sum[3];
     int has_lo;
 };
@@ -663,6 +664,11 @@ static PackedCmf pack_cmf(const float* cmf) {
             out.frag[(kSpecMfma + t) * 64 + lane] = l;
         }
     for (int c = 0; c < 3; ++c) out.sum[c] = static_cast<float>(sum[c]);
+    for (int s = 0; s < kSpecSlots; ++s)
+        for (int q = 0; q < 4; ++q) {
+            const int k = q + 4 * s;
+            out.ulam[4 * s + q] = k < kSpecLambdas ? static_cast<float>(0.5 / (380.0 + 5.0 * k)) : 0.f;
+        }
     return out;
 }
 
@@ -689,28 +695,38 @@ static void ensure_view_tables(Drainage* d) {
     }
 }
 
-void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf) {
+// Output plane, view tables and (when given) the colour curves, on the
+// context's stream.
+static void spectral_prepare(Drainage* d, const float* cmf) {
     if (!d->rgb) d->rgb = dalloc_i<float>(3 * d->n);
     ensure_view_tables(d);
     if (cmf) {
         for (int k = 0; k < 3 * kSpecLambdas; ++k)
             SF_REQUIRE(std::isfinite(cmf[k]), SF_ERR_INVALID, "colour-matching curve value %d is not finite", k);
-        if (!d->cmf_dev) d->cmf_dev = dalloc_i<float>(kCmfFrags * 4);
+        if (!d->cmf_dev) d->cmf_dev = dalloc_i<float>(kCmfFloats);
         const PackedCmf packed = pack_cmf(cmf);
         SF_HIP(hipMemcpyAsync(d->cmf_dev, packed.frag.data(), kCmfFrags * sizeof(u32x4), hipMemcpyHostToDevice,
+                              d->stream));
+        SF_HIP(hipMemcpyAsync(d->cmf_dev + kCmfFrags * 4, packed.ulam, sizeof(packed.ulam), hipMemcpyHostToDevice,
                               d->stream));
         SF_HIP(hipStreamSynchronize(d->stream));
         for (int c = 0; c < 3; ++c) d->cmf_sum[c] = packed.sum[c];
         d->cmf_lo = packed.has_lo;
     }
+}
+
+// Launches spectral_kernel on stream st over the context's current state
+// (pole rows materialised by the caller).
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st) {
+    spectral_prepare(d, cmf);
     SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
-    d->materialize_poles();
     SpecArgs a;
     a.hi = d->hi[d->cur_hi];
     a.lo = d->lo[d->cur];
     a.rows = d->spec_rows;
     a.cols = d->spec_cols;
     a.bfrag = reinterpret_cast<const u32x4*>(d->cmf_dev);
+    a.ulam = d->cmf_dev + kCmfFrags * 4;
     a.s0 = d->cmf_sum[0];
     a.s1 = d->cmf_sum[1];
     a.s2 = d->cmf_sum[2];
@@ -728,12 +744,12 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.n_film = static_cast<float>(n_film);
     a.inv_n_film = static_cast<float>(1.0 / n_film);
     const size_t supergroups = (d->n + 63) / 64;
-    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 20));
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 10));
     if (d->spec_timer.enabled) {
-        auto& ev = d->spec_timer.next(d->stream);
-        hipExtLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, ev.first, ev.second, 0, a);
+        auto& ev = d->spec_timer.next(st);
+        hipExtLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, st, ev.first, ev.second, 0, a);
     } else {
-        hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, d->stream, a);
+        hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, st, a);
     }
     SF_HIP(hipGetLastError());
 }
@@ -745,7 +761,8 @@ extern "C" int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double re
         SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
         SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
-        spectral_launch(d, cam, refractive_index, cmf);
+        d->materialize_poles();
+        spectral_launch(d, cam, refractive_index, cmf, d->stream);
         if (rgb_host) {
             SF_HIP(hipMemcpyAsync(rgb_host, d->rgb, 3 * d->n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
             SF_HIP(hipStreamSynchronize(d->stream));
@@ -762,9 +779,11 @@ extern "C" int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, doub
         SF_REQUIRE(std::isfinite(dt), SF_ERR_INVALID, "dt must be finite");
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
         if (n_steps > 0) dt = d->resolve_dt(dt);
+        if (n_steps > 0) spectral_prepare(d, cmf);
         for (int s = 0; s < n_steps; ++s) {
             d->step(1, dt, false);
-            spectral_launch(d, cam, refractive_index, s == 0 ? cmf : nullptr);
+            d->materialize_poles();
+            spectral_launch(d, cam, refractive_index, nullptr, d->stream);
         }
         d->finalize_diag();
     });
@@ -776,6 +795,16 @@ extern "C" int sf_spectral_device_rgb(sf_drainage* ctx, const float** rgb) {
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
         SF_REQUIRE(d->rgb != nullptr, SF_ERR_INVALID, "sf_spectral_rgb has not run on this context");
         *rgb = d->rgb;
+    });
+}
+
+extern "C" int sf_spectral_copy_rgb(sf_drainage* ctx, float* rgb_host) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && rgb_host != nullptr, SF_ERR_INVALID, "NULL argument");
+        Drainage* d = reinterpret_cast<Drainage*>(ctx);
+        SF_REQUIRE(d->rgb != nullptr, SF_ERR_INVALID, "no spectral RGB has been computed on this context");
+        SF_HIP(hipMemcpyAsync(rgb_host, d->rgb, 3 * d->n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+        SF_HIP(hipStreamSynchronize(d->stream));
     });
 }
 

@@ -8,6 +8,7 @@ Bars (tolerances stated per test):
     quantised bytes may differ by one LSB only where the value sits within
     1e-5 of a quantisation boundary.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -234,4 +235,49 @@ def test_spectral_full_grid_sampled(oracle):
     g = got.reshape(3, -1)[:, cells].T
     err = np.max(np.abs(g - want))
     print("spectral 2048x4096 sampled max |rgb - oracle| =", err)
+    assert err <= 1e-3
+
+
+@pytest.mark.parametrize("nt,npf,steps,flags", [
+    (96, 192, 7, L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG),
+    (512, 1024, 20, L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG | L.FLAG_MARANGONI),
+])
+def test_step_spectral_matches_serial(oracle, nt, npf, steps, flags):
+    """sf_drainage_step_spectral gives bit-identical state and RGB to the
+    sequence of separate calls {sf_drainage_step, sf_spectral_rgb}; rebases
+    fall inside the run.  The RGB is then checked against FP64 on the final
+    state."""
+    h0 = W.smooth_bubble(nt, npf, 21)
+    cmf = W.spectral_cmf(21).astype(np.float32)
+    cam_xyz = None
+    ctxs = []
+    for _ in range(2):
+        p = L.Params()
+        L.check(L.lib().sf_params_cpu_default(p))
+        p.flags = flags
+        ctx = DrainageContext(p, nt, npf, 500e-9)
+        ctx.set_fields(h0, np.full_like(h0, 0.5))
+        ctxs.append(ctx)
+        cam_xyz = W.camera(p.bubble_radius)
+    a, b = ctxs
+    dt = a.stats().dt_cfl * a.params().cfl_safety
+    cam = L.Camera()
+    for k in range(3):
+        cam.position[k] = cam_xyz[k]
+    lib = L.lib()
+    L.check(lib.sf_drainage_step_spectral(a.handle, steps, dt, C.byref(cam), 1.33, L.fptr(cmf)))
+    rgb_a = np.empty((3, nt, npf), np.float32)
+    L.check(lib.sf_spectral_copy_rgb(a.handle, L.fptr(rgb_a)))
+    for _ in range(steps):
+        b.step(1, dt)
+        rgb_b = spectral_rgb(b, cam_xyz, cmf)
+    ha, ga = a.fields()
+    hb, gb = b.fields()
+    assert np.array_equal(ha, hb) and np.array_equal(ga, gb)
+    assert np.array_equal(rgb_a, rgb_b)
+    rng = np.random.default_rng(4)
+    cells = rng.choice(nt * npf, 3000, replace=False)
+    want = oracle.spectral_cells(ha, cells, a.params().bubble_radius, cam_xyz, 1.33, cmf.astype(np.float64))
+    err = np.max(np.abs(rgb_a.reshape(3, -1)[:, cells].T - want))
+    print("step_spectral %dx%d after %d steps: max |rgb - oracle| = %.3g" % (nt, npf, steps, err))
     assert err <= 1e-3
