@@ -18,6 +18,8 @@ oracle on a bounded sample of the same workload on the host cores.
 
 ``--workload c3`` is the batched foam (4096 bubbles of uneven grids, one
 launch per step, 16-step HIP graphs, tile shapes autotuned per size bucket).
+``--workload cluster`` is the foam cluster dynamics (all-pairs forces and
+integration of 4096 bubbles per step).
 """
 from __future__ import annotations
 
@@ -48,7 +50,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3", "c4", "c5", "cluster"])
     ap.add_argument("--train-windows", type=int, default=300,
                     help="c4/c5: online training windows before the timed surrogate run")
     ap.add_argument("--tol", type=float, default=1e-3, help="c4/c5: judge tolerance")
@@ -207,6 +209,66 @@ def bench_foam(args, L, W):
     print(json.dumps(out))
 
 
+CLUSTER_FLOP_PER_PAIR = 13.0     # delta 3, |delta|^2 5, sqrt 1, direction 3, r_a + r_b 1
+FP32_VALU_PEAK_TFLOPS = 78.6     # MI355X FP32 vector without packed math
+
+
+def bench_cluster(args, L, W):
+    """Foam cluster dynamics (FoamSimulator::step): all-pairs forces,
+    integration; one step = one FoamSimulator step of the whole cluster."""
+    from soap_bubble_simulation_amd.cluster import FoamSimulator
+
+    n = args.bubbles or 4096
+    box = 0.5 * (n / 4096.0) ** (1.0 / 3.0)
+    pos, r, vel, th = W.cluster_bubbles(n, seed=9, box=box)
+    sim = FoamSimulator(pos, r, vel, th)
+    dt = 1e-3
+    sim.step_many(max(1, args.warmup), dt)
+    sim.synchronize()
+    sim.kernel_timing(True)
+    sim.step_many(min(args.steps, 500), dt)
+    k_ms, k_n = sim.kernel_timing(False)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step_many(args.steps, dt)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    pairs = n * (n - 1) / 2.0
+    flops = CLUSTER_FLOP_PER_PAIR * n * n
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    b = sim.bubbles()
+    out = {
+        "metric": "cluster pair interactions/s (FoamSimulator::step)", "value": pairs * args.steps / elapsed,
+        "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "foam cluster of %d bubbles, positions U in a %.3f m cube, radius logU[2.5,25] mm, "
+                               "velocity U[-0.02,0.02] m/s, film U[200,1000] nm, dt 1 ms, no coalescence"
+                               % (n, box), "bubbles": n},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "cluster_pair_kernel", "avg_launch_ms": k_ms, "launches_timed": k_n,
+                     "algorithmic_flops_per_launch": flops},
+        "state": {"finite": bool(np.isfinite(b["position"]).all() and np.isfinite(b["velocity"]).all()),
+                  "max_speed": float(np.linalg.norm(b["velocity"], axis=1).max())},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import cluster_ref as R
+        m = min(n, 512)
+        c = R.Cluster.from_arrays(pos[:m], r[:m], vel[:m], th[:m])
+        p = R.Params()
+        t0 = time.perf_counter()
+        steps = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            R.step(c, p, dt)
+            steps += 1
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": m * (m - 1) / 2.0 * steps / el, "unit": "pair-interactions/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "%d FP64 numpy oracle steps of the first %d bubbles" % (steps, m)}
+    print(json.dumps(out))
+
+
 def fno_window_flops(cfg, cells):
     """Algorithmic FLOPs of one network inference over `cells` cells (GEMM
     products: truncated DFTs and their inverses, pointwise mixing, lift and
@@ -333,6 +395,9 @@ def main():
 
     if args.workload == "c3":
         bench_foam(args, L, W)
+        return
+    if args.workload == "cluster":
+        bench_cluster(args, L, W)
         return
     if args.workload in ("c4", "c5"):
         bench_surrogate(args, L, W, 1 if args.workload == "c4" else (args.bubbles or 1024))

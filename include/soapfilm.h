@@ -377,6 +377,99 @@ int sf_foam_stream(sf_foam* foam, void** stream);
 /* Number of bubbles of the foam. */
 int sf_foam_num_bubbles(sf_foam* foam, int32_t* n);
 
+/* ---- foam cluster dynamics ------------------------------------------------
+ * FoamSimulator (foam_dynamics.rs:9-202) over a BubbleCluster (foam.rs:208-
+ * 399): all-pairs van der Waals / Hertzian repulsion / contact adhesion
+ * forces, buoyancy and Stokes drag, semi-implicit Euler with damping and a
+ * 0.1 m/s speed clamp, contact connections and coalescence.  FP32 state as
+ * the reference (glam Vec3 / f32); pair forces in FP32, their sums in FP64.
+ * Bubbles keep the reference's ids and vector order (removal keeps order,
+ * additions and merges append with the next id). */
+typedef struct sf_cluster_params {
+    float surface_tension;          /* N/m                         0.025   */
+    float viscosity;                /* Pa s                        0.001   */
+    float density;                  /* kg/m^3                      1000    */
+    float gravity[3];               /* m/s^2                   (0,-9.81,0) */
+    float van_der_waals_strength;   /*                             0.001   */
+    float repulsion_stiffness;      /*                             5.0     */
+    float damping;                  /* velocity factor per step    0.8     */
+    int32_t coalescence_enabled;    /*                             0       */
+    float coalescence_threshold;    /* Weber number                0.5     */
+} sf_cluster_params;
+
+/* Bubble (foam.rs:31-49) without its neighbour list. */
+typedef struct sf_cluster_bubble {
+    uint32_t id;
+    float position[3];      /* m   */
+    float radius;           /* m   */
+    float velocity[3];      /* m/s */
+    float thickness_nm;
+    float aspect_ratio;
+    float refractive_index;
+} sf_cluster_bubble;
+
+/* BubbleConnection (foam.rs:89-139). */
+typedef struct sf_cluster_connection {
+    uint32_t bubble_a, bubble_b;
+    float contact_point[3];
+    float contact_normal[3];        /* from a toward b                     */
+    float wall_curvature_radius;    /* +inf for equal radii               */
+    float wall_thickness_nm;
+} sf_cluster_connection;
+
+typedef struct sf_cluster sf_cluster;
+
+/* FoamSimulator::new defaults (foam_dynamics.rs:37-53). */
+int sf_cluster_params_default(sf_cluster_params* p);
+/* BubbleCluster from bubbles in vector order; the next id is one past the
+ * largest given.  Replaces FoamSimulator::new (foam_dynamics.rs:37). */
+int sf_cluster_create(const sf_cluster_params* params, int32_t n, const sf_cluster_bubble* bubbles,
+                      sf_cluster** out);
+void sf_cluster_destroy(sf_cluster* c);
+int sf_cluster_set_params(sf_cluster* c, const sf_cluster_params* params);
+/* n_steps of FoamSimulator::step(dt) (foam_dynamics.rs:166-202), enqueued
+ * on the cluster's stream (with coalescence on, each step waits for its
+ * merge test). */
+int sf_cluster_step(sf_cluster* c, float dt, int32_t n_steps);
+/* Blocks until the cluster's work has finished. */
+int sf_cluster_synchronize(sf_cluster* c);
+/* Average duration in ms of the all-pairs force kernel since the last call
+ * (HIP events on the cluster's stream); enable switches recording on/off. */
+int sf_cluster_kernel_timing(sf_cluster* c, int32_t enable, double* avg_ms, int64_t* launches);
+/* compute_forces (foam_dynamics.rs:61-92) of the current state:
+ * forces[n][3]. */
+int sf_cluster_compute_forces(sf_cluster* c, float* forces);
+int sf_cluster_num_bubbles(sf_cluster* c, int32_t* n);
+/* Bubbles in vector order (BubbleCluster::bubbles, foam.rs:273-275). */
+int sf_cluster_get_bubbles(sf_cluster* c, sf_cluster_bubble* out);
+/* Replaces every bubble (state written by a caller, e.g. from a saved run). */
+int sf_cluster_set_bubbles(sf_cluster* c, int32_t n, const sf_cluster_bubble* bubbles);
+/* Connections of the last update in (i, j) vector order
+ * (BubbleCluster::connections, foam.rs:305-307; connection_count,
+ * foam_dynamics.rs:236-238). */
+int sf_cluster_num_connections(sf_cluster* c, int32_t* n);
+int sf_cluster_get_connections(sf_cluster* c, sf_cluster_connection* out);
+/* Neighbour id lists (Bubble::neighbors): offsets[n + 1] and ids[offsets[n]]
+ * (ids NULL: offsets only). */
+int sf_cluster_neighbors(sf_cluster* c, int32_t* offsets, uint32_t* ids);
+/* update_connections (foam.rs:321-360) on the current state. */
+int sf_cluster_update_connections(sf_cluster* c);
+/* add_bubble (foam.rs:235-244), remove_bubble (249-259), merge_bubbles
+ * (365-399). */
+int sf_cluster_add_bubble(sf_cluster* c, const float position[3], float radius, uint32_t* id);
+int sf_cluster_remove_bubble(sf_cluster* c, uint32_t id, int32_t* removed);
+int sf_cluster_merge(sf_cluster* c, uint32_t id_a, uint32_t id_b, uint32_t* new_id);
+/* Feeds the cluster into batched drainage: one sf_bubble per cluster bubble
+ * (vector order) with its radius, film thickness (h0), the cluster's
+ * viscosity, CFL dt, and n_theta mapped log-linearly from radius_min..max to
+ * n_theta_min..max (n_phi = 2 n_theta).  The reference keeps only a scalar
+ * thickness per foam bubble (foam.rs:44). */
+int sf_cluster_foam_bubbles(sf_cluster* c, int32_t n_theta_min, int32_t n_theta_max, float radius_min,
+                            float radius_max, sf_bubble* out);
+/* Sets each bubble's thickness_nm to its foam bubble's mean film thickness
+ * (FP64 film volume / 4 pi R^2); the foam must have as many bubbles. */
+int sf_cluster_thickness_from_foam(sf_cluster* c, sf_foam* foam);
+
 /* ---- learned surrogate (Fourier neural operator) --------------------------
  * No reference counterpart: a network that advances a bubble's film
  * thickness by `window` drainage steps, trained on trajectories of the plain

@@ -125,6 +125,34 @@ class FnoStats(C.Structure):
     ]
 
 
+class ClusterParams(C.Structure):
+    """Mirror of ``sf_cluster_params``."""
+    _fields_ = [
+        ("surface_tension", C.c_float), ("viscosity", C.c_float), ("density", C.c_float),
+        ("gravity", C.c_float * 3), ("van_der_waals_strength", C.c_float),
+        ("repulsion_stiffness", C.c_float), ("damping", C.c_float),
+        ("coalescence_enabled", C.c_int32), ("coalescence_threshold", C.c_float),
+    ]
+
+
+class ClusterBubble(C.Structure):
+    """Mirror of ``sf_cluster_bubble``."""
+    _fields_ = [
+        ("id", C.c_uint32), ("position", C.c_float * 3), ("radius", C.c_float),
+        ("velocity", C.c_float * 3), ("thickness_nm", C.c_float), ("aspect_ratio", C.c_float),
+        ("refractive_index", C.c_float),
+    ]
+
+
+class ClusterConnection(C.Structure):
+    """Mirror of ``sf_cluster_connection``."""
+    _fields_ = [
+        ("bubble_a", C.c_uint32), ("bubble_b", C.c_uint32), ("contact_point", C.c_float * 3),
+        ("contact_normal", C.c_float * 3), ("wall_curvature_radius", C.c_float),
+        ("wall_thickness_nm", C.c_float),
+    ]
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _u8p = C.POINTER(C.c_uint8)
@@ -220,6 +248,27 @@ SIGNATURES = {
     "sf_fno_advance_foam": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_double, C.c_int32,
                                       C.POINTER(FnoStats)]),
     "sf_fno_features": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_int32, _fp]),
+    "sf_cluster_params_default": (C.c_int, [C.POINTER(ClusterParams)]),
+    "sf_cluster_create": (C.c_int, [C.POINTER(ClusterParams), C.c_int32, C.POINTER(ClusterBubble),
+                                    C.POINTER(_vp)]),
+    "sf_cluster_destroy": (None, [_vp]),
+    "sf_cluster_set_params": (C.c_int, [_vp, C.POINTER(ClusterParams)]),
+    "sf_cluster_step": (C.c_int, [_vp, C.c_float, C.c_int32]),
+    "sf_cluster_synchronize": (C.c_int, [_vp]),
+    "sf_cluster_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "sf_cluster_compute_forces": (C.c_int, [_vp, _fp]),
+    "sf_cluster_num_bubbles": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "sf_cluster_get_bubbles": (C.c_int, [_vp, C.POINTER(ClusterBubble)]),
+    "sf_cluster_set_bubbles": (C.c_int, [_vp, C.c_int32, C.POINTER(ClusterBubble)]),
+    "sf_cluster_num_connections": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "sf_cluster_get_connections": (C.c_int, [_vp, C.POINTER(ClusterConnection)]),
+    "sf_cluster_neighbors": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]),
+    "sf_cluster_update_connections": (C.c_int, [_vp]),
+    "sf_cluster_add_bubble": (C.c_int, [_vp, _fp, C.c_float, C.POINTER(C.c_uint32)]),
+    "sf_cluster_remove_bubble": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_int32)]),
+    "sf_cluster_merge": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "sf_cluster_foam_bubbles": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, C.c_float, C.POINTER(Bubble)]),
+    "sf_cluster_thickness_from_foam": (C.c_int, [_vp, _vp]),
 }
 
 _lib = None

@@ -62,6 +62,11 @@ class FoamDrainage:
         L.check(self._lib.sf_foam_layout(self._h, offs))
         self.offsets = np.array(offs[:], dtype=np.int64)
 
+    @property
+    def handle(self):
+        """The ``sf_foam*`` of this foam (for the C ABI)."""
+        return self._h
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.sf_foam_destroy(self._h)

@@ -143,3 +143,16 @@ def foam_noise(offsets: np.ndarray, sizes, h0, seed: int, rel: float = 0.05,
         out[a:a + n] = h0[k] * (1.0 + rel * rng.uniform(-1.0, 1.0, n))
         out[a + n:b] = h0[k]
     return out
+
+
+def cluster_bubbles(n: int, seed: int, box: float = 0.25, radius=(2.5e-3, 25e-3), speed: float = 0.02):
+    """Seeded foam cluster: positions uniform in a cube of side ``box`` (m),
+    radii log-uniform in ``radius``, velocity components uniform in
+    [-speed, speed] m/s, film thickness U[200, 1000] nm; FP32 values.
+    Returns (pos [n][3], radius [n], vel [n][3], thickness_nm [n])."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-box / 2.0, box / 2.0, (n, 3)).astype(np.float32)
+    r = np.exp(rng.uniform(math.log(radius[0]), math.log(radius[1]), n)).astype(np.float32)
+    vel = rng.uniform(-speed, speed, (n, 3)).astype(np.float32)
+    th = rng.uniform(200.0, 1000.0, n).astype(np.float32)
+    return pos, r, vel, th

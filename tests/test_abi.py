@@ -109,3 +109,32 @@ def test_foam_rejects_unsupported_flags():
 def test_bubble_struct_layout():
     assert C.sizeof(L.Bubble) == 40
     assert L.Bubble.bubble_radius.offset == 8
+
+
+def test_cluster_struct_layouts_and_defaults():
+    assert C.sizeof(L.ClusterParams) == 44
+    assert C.sizeof(L.ClusterBubble) == 44 and L.ClusterBubble.radius.offset == 16
+    assert C.sizeof(L.ClusterConnection) == 40 and L.ClusterConnection.wall_curvature_radius.offset == 32
+    p = L.ClusterParams()
+    L.check(L.lib().sf_cluster_params_default(p))
+    # FoamSimulator::new (foam_dynamics.rs:37-53)
+    assert (p.surface_tension, p.density) == (C.c_float(0.025).value, 1000.0)
+    assert p.viscosity == C.c_float(0.001).value and tuple(p.gravity) == (0.0, C.c_float(-9.81).value, 0.0)
+    assert p.van_der_waals_strength == C.c_float(0.001).value and p.repulsion_stiffness == 5.0
+    assert p.damping == C.c_float(0.8).value and p.coalescence_enabled == 0
+    assert p.coalescence_threshold == 0.5
+
+
+def test_cluster_argument_validation():
+    p = L.ClusterParams()
+    L.check(L.lib().sf_cluster_params_default(p))
+    h = C.c_void_p()
+    b = (L.ClusterBubble * 1)()
+    b[0].radius = -1.0
+    assert L.lib().sf_cluster_create(C.byref(p), 1, b, C.byref(h)) == L.SF_ERR_INVALID
+    assert "radius" in L.last_error()
+    assert L.lib().sf_cluster_step(None, 0.001, 1) == L.SF_ERR_INVALID
+    if not L.device_available():
+        b[0].radius = 0.01
+        assert L.lib().sf_cluster_create(C.byref(p), 1, b, C.byref(h)) == L.SF_ERR_DEVICE
+        assert not h.value
