@@ -473,7 +473,8 @@ def main():
     traffic, pmc = pmc_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": "drainage_band_kernel" if n_phi % 4 == 0 else "drainage_step_kernel",
+                # the float4 kernel the context selects: 16-row bands at 2048 x 4096, 8-row blocks at 128 x 256
+                "kernel": "drainage_band_kernel" if n_theta >= 1024 else "drainage_vec4_kernel",
                 "avg_launch_ms": drain_ms,
                 "launches_timed": drain_n,
                 "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells}
