@@ -270,6 +270,10 @@ struct SpecArgs {
     float radius, n_film, inv_n_film;
     float s0, s1, s2;       // sum over the wavelengths of each (BF16 hi + lo) curve
     int b_lo;               // the curves have a nonzero BF16 lo part
+    // Pole rows held as ring partials (lazy poles): their h is the ring mean
+    // from hpole, as materialize_poles_kernel would write it.
+    const float2* hpole;
+    int pole_virtual, nt, ntiles;
 };
 
 __device__ inline float lane_read(float v, int src_lane) {
@@ -299,7 +303,14 @@ __device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, float inv_n
     const float R = (rs * rs + rp * rp) * 0.5f;
     const float omr = fmaxf(1.f - R, 1e-3f);
     Fc = 4.f * R * __builtin_amdgcn_rcpf(omr * omr);
-    const float d_nm = (a.hi[cc] + a.lo[cc]) * 1e9f;
+    float h;
+    if (a.pole_virtual && (i == 0 || i == a.nt - 1)) {
+        const float2 m = pole_mean(a.hpole + (i == 0 ? 0 : a.ntiles), a.ntiles, static_cast<float>(a.np));
+        h = m.x + m.y;
+    } else {
+        h = a.hi[cc] + a.lo[cc];
+    }
+    const float d_nm = h * 1e9f;
     opc = 2.f * a.n_film * d_nm * cos_t;
 }
 
@@ -716,7 +727,7 @@ static void spectral_prepare(Drainage* d, const float* cmf) {
 }
 
 // Launches spectral_kernel on stream st over the context's current state
-// (pole rows materialised by the caller).
+// (lazy pole rows are taken from the ring partials).
 void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st) {
     spectral_prepare(d, cmf);
     SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
@@ -731,6 +742,10 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.s1 = d->cmf_sum[1];
     a.s2 = d->cmf_sum[2];
     a.b_lo = d->cmf_lo;
+    a.hpole = d->hpole[d->cur_pole];
+    a.pole_virtual = (d->poles_virtual && !(d->p.flags & SF_FLAG_POLES_COPY_H)) ? 1 : 0;
+    a.nt = d->nt;
+    a.ntiles = d->ntiles;
     a.rgb = d->rgb;
     a.np = d->np;
     a.n = d->n;
@@ -761,7 +776,6 @@ extern "C" int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double re
         SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
         SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
-        d->materialize_poles();
         spectral_launch(d, cam, refractive_index, cmf, d->stream);
         if (rgb_host) {
             SF_HIP(hipMemcpyAsync(rgb_host, d->rgb, 3 * d->n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
@@ -780,9 +794,10 @@ extern "C" int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, doub
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
         if (n_steps > 0) dt = d->resolve_dt(dt);
         if (n_steps > 0) spectral_prepare(d, cmf);
+        // The pole rows stay lazy (ring partials): the spectral kernel takes
+        // their ring mean itself, and the next step reads the partials.
         for (int s = 0; s < n_steps; ++s) {
             d->step(1, dt, false);
-            d->materialize_poles();
             spectral_launch(d, cam, refractive_index, nullptr, d->stream);
         }
         d->finalize_diag();
