@@ -404,7 +404,10 @@ __global__ __launch_bounds__(256) void spectral_kernel(const SpecArgs a) {
                 const float4 o = make_float4((S - acc[g][0]) * scale, (S - acc[g][1]) * scale,
                                              (S - acc[g][2]) * scale, (S - acc[g][3]) * scale);
                 if (vec_out && c0 + 3 < a.n) {
-                    *reinterpret_cast<float4*>(plane + c0) = o;
+                    // Streaming store: the RGB planes are not re-read here, so they
+                    // should not displace the film state from the caches.
+                    typedef float f4v __attribute__((ext_vector_type(4)));
+                    __builtin_nontemporal_store(f4v{o.x, o.y, o.z, o.w}, reinterpret_cast<f4v*>(plane + c0));
                 } else {
                     if (c0 < a.n) plane[c0] = o.x;
                     if (c0 + 1 < a.n) plane[c0 + 1] = o.y;
