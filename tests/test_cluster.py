@@ -364,3 +364,25 @@ def test_gpu_graph_replay_matches_single_steps():
     ba, bb = a.bubbles(), b.bubbles()
     assert np.array_equal(ba["position"], bb["position"]) and np.array_equal(ba["velocity"], bb["velocity"])
     assert [(x.bubble_a, x.bubble_b) for x in a.connections()] == [(x.bubble_a, x.bubble_b) for x in b.connections()]
+
+
+@pytest.mark.gpu
+def test_gpu_cluster_edge_cases():
+    from soap_bubble_simulation_amd.cluster import FoamSimulator
+    empty = FoamSimulator()
+    empty.step_many(20, 1e-3)
+    assert empty.bubble_count() == 0 and empty.connection_count() == 0
+    assert empty.compute_forces().shape == (0, 3) and empty.neighbors() == []
+    # a lone bubble: buoyancy and drag only, as the oracle
+    one = _sim([(0.01, 0.02, 0.03)], [0.004], [(0.01, 0.0, 0.0)], [20000.0])
+    c = R.Cluster.from_arrays([(0.01, 0.02, 0.03)], [0.004], [(0.01, 0.0, 0.0)], [20000.0])
+    p = _oracle_params()
+    for _ in range(5):
+        one.step(1e-3)
+        R.step(c, p, 1e-3)
+    b = one.bubbles()
+    assert np.abs(b["velocity"] - c.vel).max() <= 1e-6 and one.connection_count() == 0
+    # coincident centres: no interaction force (distance < 1e-6, foam_dynamics.rs:120-122)
+    two = _sim([(0, 0, 0), (0, 0, 0)], [0.01, 0.01], gravity=np.zeros(3))
+    assert np.abs(two.compute_forces()).max() == 0.0
+    assert two.connection_count() == 1
