@@ -272,7 +272,7 @@ Drainage::~Drainage() {
     }
     dfree(rows); dfree(rowd); dfree(diag_part[0]); dfree(diag_part[1]); dfree(diag_hist);
     dfree(stats_part); dfree(stats_out); dfree(gather_idx); dfree(gather_out);
-    dfree(rgb); dfree(cmf_dev); dfree(spec_rows); dfree(spec_cols); dfree(caustic);
+    dfree(rgb); dfree(cmf_dev); dfree(spec_rows); dfree(spec_cols); dfree(spec_view); dfree(caustic);
     step_timer.destroy();
     spec_timer.destroy();
     if (stream) (void)hipStreamDestroy(stream);

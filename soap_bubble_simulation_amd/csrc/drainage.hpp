@@ -89,6 +89,9 @@ struct Drainage {
     float cmf_sum[3] = {0.f, 0.f, 0.f};   // sum over the 81 wavelengths of each curve
     int cmf_lo = 0;                 // the curves are not exact in BF16
     float4* spec_rows = nullptr;
+    float2* spec_view = nullptr;    // per-cell (F, cos theta_t) of spec_view_key's camera and index
+    float spec_view_key[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool spec_view_valid = false;
     float2* spec_cols = nullptr;
 
     // time and TFE schedule (drainage.rs:237-255)

@@ -274,6 +274,9 @@ struct SpecArgs {
     // from hpole, as materialize_poles_kernel would write it.
     const float2* hpole;
     int pole_virtual, nt, ntiles;
+    // Per-cell (F, cos theta_t) of the current camera and film index; the
+    // geometry is fixed while they are, so it is computed once (NULL: inline).
+    const float2* view;
 };
 
 __device__ inline float lane_read(float v, int src_lane) {
@@ -282,11 +285,16 @@ __device__ inline float lane_read(float v, int src_lane) {
 
 // Per-cell terms of the Airy sum: coefficient of finesse F and optical path
 // difference opc (nm) of cell cc at its view angle.
-__device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, float inv_np, float& Fc, float& opc) {
-    int i = static_cast<int>(static_cast<float>(cc) * inv_np);
-    int j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
+// Row and column of cell cc (FP32 quotient, corrected).
+__device__ inline void spec_cell_ij(const SpecArgs& a, size_t cc, float inv_np, int& i, int& j) {
+    i = static_cast<int>(static_cast<float>(cc) * inv_np);
+    j = static_cast<int>(cc - static_cast<size_t>(i) * a.np);
     if (j < 0) { --i; j += a.np; }
     if (j >= a.np) { ++i; j -= a.np; }
+}
+
+// Coefficient of finesse F and cos theta_t of cell (i, j) at its view angle.
+__device__ inline void spec_view_terms(const SpecArgs& a, int i, int j, float& Fc, float& cos_t_out) {
     const float4 rw = a.rows[i];
     const float2 cl = a.cols[j];
     // View angle |dot(normalize(cam - p), n)|, p = R n (bubble.wgsl:491-494).
@@ -303,6 +311,25 @@ __device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, float inv_n
     const float R = (rs * rs + rp * rp) * 0.5f;
     const float omr = fmaxf(1.f - R, 1e-3f);
     Fc = 4.f * R * __builtin_amdgcn_rcpf(omr * omr);
+    cos_t_out = cos_t;
+}
+
+// Per-cell terms of the Airy sum: coefficient of finesse F and optical path
+// difference opc (nm) of cell cc at its view angle.
+__device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, float inv_np, float& Fc, float& opc) {
+    int i, j;
+    spec_cell_ij(a, cc, inv_np, i, j);
+    float cos_t;
+    if (a.view) {
+        // Streaming load: this VALU-bound pass has bandwidth to spare, and the
+        // film state should keep the cache.
+        typedef float f2v __attribute__((ext_vector_type(2)));
+        const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(a.view) + cc);
+        Fc = v.x;
+        cos_t = v.y;
+    } else {
+        spec_view_terms(a, i, j, Fc, cos_t);
+    }
     float h;
     if (a.pole_virtual && (i == 0 || i == a.nt - 1)) {
         const float2 m = pole_mean(a.hpole + (i == 0 ? 0 : a.ntiles), a.ntiles, static_cast<float>(a.np));
@@ -419,6 +446,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void spectral_view_kernel(const SpecArgs a, float2* __restrict__ out) {
+    const size_t cc = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (cc >= a.n) return;
+    int i, j;
+    spec_cell_ij(a, cc, 1.f / static_cast<float>(a.np), i, j);
+    float F, cos_t;
+    spec_view_terms(a, i, j, F, cos_t);
+    out[cc] = make_float2(F, cos_t);
 }
 
 static inline unsigned cdiv_u(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
@@ -763,6 +800,17 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.radius = static_cast<float>(d->p.bubble_radius);
     a.n_film = static_cast<float>(n_film);
     a.inv_n_film = static_cast<float>(1.0 / n_film);
+    a.view = nullptr;
+    // View terms of this camera and film index, cached on the context.
+    const float key[6] = {a.cx, a.cy, a.cz, a.radius, a.n_film, a.inv_n_film};
+    if (!d->spec_view) d->spec_view = dalloc_i<float2>(d->n);
+    if (!d->spec_view_valid || std::memcmp(key, d->spec_view_key, sizeof(key)) != 0) {
+        hipLaunchKernelGGL(spectral_view_kernel, dim3(cdiv_u(d->n, 256)), dim3(256), 0, st, a, d->spec_view);
+        SF_HIP(hipGetLastError());
+        std::memcpy(d->spec_view_key, key, sizeof(key));
+        d->spec_view_valid = true;
+    }
+    a.view = d->spec_view;
     const size_t supergroups = (d->n + 63) / 64;
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 10));
     if (d->spec_timer.enabled) {
