@@ -9,9 +9,11 @@
 //   cluster_pair_kernel       all-pairs forces: lane = bubble i of a 64-bubble
 //                             tile, wave = one contiguous run of partners j;
 //                             FP32 pair force as the reference computes it,
-//                             FP64 sums, one partial per (block row, i)
+//                             compensated FP32 sums, one partial per
+//                             (block row, i)
 //   cluster_integrate_kernel  buoyancy + the partials (in run order) +
-//                             drag, then the FP32 integration; double-buffered
+//                             drag (compensated FP32), then the FP32
+//                             integration; double-buffered
 //                             state, so forces read only the old positions
 //   cluster_contact_kernel    per (i, run): contacts j > i counted, the first
 //                             coalescence candidate in (i, j) order
@@ -62,8 +64,23 @@ struct Consts {
 // distance < 3 (r_a + r_b): pairs clearly beyond (squared distance past
 // 9 s^2 with a 1e-5 margin for rounding) return at once, the common case.
 // The distance and 1 / distance use v_sqrt_f32 and v_rcp_f32 (1 ulp) where
-// the reference divides: the direction and the 1 / d^2 law move by ulps.
-__device__ inline void pair_force_add(float4 a, float4 b, const Consts& c, double& fx, double& fy, double& fz) {
+// the reference divides, and overlap^1.5 is overlap sqrt(overlap) where it
+// calls powf: the results move by ulps.
+// Compensated (Kahan) FP32 accumulation for the folds of partial sums: s + c
+// carries the running sum to about twice FP32 precision.
+struct KSum {
+    float s = 0.f, c = 0.f;
+    __device__ inline void add(float x) {
+#pragma clang fp contract(off)
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ inline float value() const { return s - c; }
+};
+
+__device__ inline void pair_force_add(float4 a, float4 b, const Consts& c, float& fx, float& fy, float& fz) {
 #pragma clang fp contract(off)
     const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
@@ -76,12 +93,12 @@ __device__ inline void pair_force_add(float4 a, float4 b, const Consts& c, doubl
     float m = 0.f;                                   // signed magnitude along b - a
     if (dist < s3 && dist > s) m += c.vdw * (a.w * b.w) * (inv * inv);
     const float overlap = s - dist;
-    if (overlap > 0.f) m -= c.k_rep * powf(overlap, 1.5f);
+    if (overlap > 0.f) m -= c.k_rep * (overlap * __builtin_amdgcn_sqrtf(overlap));   // overlap^1.5
     const float thr = s * 1.2f;
     if (dist < thr && dist > s * 0.95f) m += kAdhesion * (thr - dist);
-    fx += static_cast<double>(dx * inv * m);
-    fy += static_cast<double>(dy * inv * m);
-    fz += static_cast<double>(dz * inv * m);
+    fx += dx * inv * m;
+    fy += dy * inv * m;
+    fz += dz * inv * m;
 }
 
 // Partner positions of a run are loaded once per 64, one per lane, and
@@ -98,13 +115,15 @@ __device__ inline float4 lane_bcast(const float4& v, int t) {
 // One partial per (block row, bubble): the block's four runs are folded in
 // run order through LDS.
 __global__ __launch_bounds__(256) void cluster_pair_kernel(const float4* __restrict__ pr, Consts c,
-                                                           double* __restrict__ part) {
-    __shared__ double s_f[kRunsPerBlock][3][kTile];
+                                                           float* __restrict__ part) {
+    __shared__ float s_f[kRunsPerBlock][3][kTile];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int run = blockIdx.y * kRunsPerBlock + w;
     const int i = blockIdx.x * kTile + lane;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
+    // Within a run only partners closer than 3 (r_a + r_b) add a term, a
+    // handful at cluster densities; plain FP32 sums, folded compensated.
+    float fx = 0.f, fy = 0.f, fz = 0.f;
     if (run < c.nruns) {                 // whole wave
         const float4 a = pr[i < c.n ? i : c.n - 1];
         const int j0 = run * c.run, j1 = min(c.n, j0 + c.run);
@@ -119,10 +138,12 @@ __global__ __launch_bounds__(256) void cluster_pair_kernel(const float4* __restr
     s_f[w][2][lane] = fz;
     __syncthreads();
     if (w == 0 && i < c.n) {
-        double o[3];
-        for (int d = 0; d < 3; ++d) o[d] = ((s_f[0][d][lane] + s_f[1][d][lane]) + s_f[2][d][lane]) + s_f[3][d][lane];
-        double* dst = part + (static_cast<size_t>(blockIdx.y) * c.n + i) * 3;
-        dst[0] = o[0]; dst[1] = o[1]; dst[2] = o[2];
+        float* dst = part + (static_cast<size_t>(blockIdx.y) * c.n + i) * 3;
+        for (int d = 0; d < 3; ++d) {
+            KSum k;
+            for (int r = 0; r < kRunsPerBlock; ++r) k.add(s_f[r][d][lane]);
+            dst[d] = k.value();
+        }
     }
 }
 
@@ -138,7 +159,7 @@ __device__ inline float buoyancy_mag(float r, float t_nm, const Consts& c) {
 
 __global__ __launch_bounds__(256) void cluster_integrate_kernel(const float4* __restrict__ pr,
                                                                 const float4* __restrict__ vt,
-                                                                const double* __restrict__ part, Consts c,
+                                                                const float* __restrict__ part, Consts c,
                                                                 float4* __restrict__ pr_out,
                                                                 float4* __restrict__ vt_out,
                                                                 float* __restrict__ force_out) {
@@ -149,17 +170,19 @@ __global__ __launch_bounds__(256) void cluster_integrate_kernel(const float4* __
     const float4 v = vt[i];
     const float bm = buoyancy_mag(p.w, v.w, c);
     const float dc = c.drag_c * p.w;     // 6 pi eta R (foam_dynamics.rs:159)
-    double fx = static_cast<double>(c.gdir[0] * bm), fy = static_cast<double>(c.gdir[1] * bm),
-           fz = static_cast<double>(c.gdir[2] * bm);
+    KSum fx, fy, fz;
+    fx.add(c.gdir[0] * bm);
+    fy.add(c.gdir[1] * bm);
+    fz.add(c.gdir[2] * bm);
     const int rows = (c.nruns + kRunsPerBlock - 1) / kRunsPerBlock;
     for (int r = 0; r < rows; ++r) {
-        const double* q = part + (static_cast<size_t>(r) * c.n + i) * 3;
-        fx += q[0]; fy += q[1]; fz += q[2];
+        const float* q = part + (static_cast<size_t>(r) * c.n + i) * 3;
+        fx.add(q[0]); fy.add(q[1]); fz.add(q[2]);
     }
-    fx += static_cast<double>(-v.x * dc);
-    fy += static_cast<double>(-v.y * dc);
-    fz += static_cast<double>(-v.z * dc);
-    const float Fx = static_cast<float>(fx), Fy = static_cast<float>(fy), Fz = static_cast<float>(fz);
+    fx.add(-v.x * dc);
+    fy.add(-v.y * dc);
+    fz.add(-v.z * dc);
+    const float Fx = fx.value(), Fy = fy.value(), Fz = fz.value();
     if (force_out) {
         force_out[3 * i + 0] = Fx;
         force_out[3 * i + 1] = Fy;
@@ -358,7 +381,7 @@ struct Cluster {
     std::vector<uint32_t> ids;
     std::vector<float> aspect, refr;
     uint32_t next_id = 0;
-    double* part = nullptr;
+    float* part = nullptr;
     size_t part_cap = 0;
     int* count = nullptr;
     int* offs = nullptr;
@@ -398,14 +421,12 @@ struct Cluster {
 
     Consts consts(float dt) const {
         Consts c{};
-        const double g2 = static_cast<double>(p.gravity[0]) * p.gravity[0] +
-                          static_cast<double>(p.gravity[1]) * p.gravity[1] +
-                          static_cast<double>(p.gravity[2]) * p.gravity[2];
         // glam length (f32) and normalize_or_zero
         const float gl = sqrtf(p.gravity[0] * p.gravity[0] + p.gravity[1] * p.gravity[1] +
                                p.gravity[2] * p.gravity[2]);
         c.gmag = gl;
-        for (int k = 0; k < 3; ++k) c.gdir[k] = (g2 > 0.0 && std::isfinite(1.f / gl)) ? p.gravity[k] * (1.f / gl) : 0.f;
+        const float rcp = 1.f / gl;
+        for (int k = 0; k < 3; ++k) c.gdir[k] = (std::isfinite(rcp) && rcp > 0.f) ? p.gravity[k] * rcp : 0.f;
         c.density = p.density;
         c.vdw = p.van_der_waals_strength;
         c.k_rep = p.repulsion_stiffness;
@@ -448,7 +469,7 @@ struct Cluster {
         const size_t np = static_cast<size_t>(c.nruns) * n * 3;
         if (np > part_cap) {
             dfree(part);
-            part = dnew<double>(np);
+            part = dnew<float>(np);
             part_cap = np;
         }
         const size_t m = static_cast<size_t>(c.nruns) * n;
