@@ -59,21 +59,25 @@ struct BubbleWin {
     float s_diff;         // K dt D / (R^2 dtheta^2)
 };
 
-// Mean thickness of each bubble (one block per bubble, FP64).
+// Mean thickness of each bubble (one block per bubble, compensated FP32).
+// It is a reference value: the features and the reconstruction of a window
+// use the same one, so its last bits do not matter.
 __global__ __launch_bounds__(256) void bubble_mean_kernel(const float* hi, const float* lo, const BubbleWin* win,
-                                                          int first, int n_cells, double* mean) {
+                                                          int first, int n_cells, float* mean) {
     const BubbleWin w = win[first + blockIdx.x];
-    double s = 0.0;
-    for (int c = threadIdx.x; c < n_cells; c += blockDim.x)
-        s += static_cast<double>(hi[w.off + c]) + static_cast<double>(lo[w.off + c]);
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+    Kahan s;
+    for (int c = threadIdx.x; c < n_cells; c += blockDim.x) {
+        s.add(hi[w.off + c]);
+        s.add(lo[w.off + c]);
     }
-    if (threadIdx.x == 0) mean[blockIdx.x] = red[0] / n_cells;
+    __shared__ float red[256];
+    red[threadIdx.x] = s.value();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Kahan t;
+        for (int q = 0; q < 256; ++q) t.add(red[q]);
+        mean[blockIdx.x] = t.value() / static_cast<float>(n_cells);
+    }
 }
 
 struct FeatArgs {
@@ -81,7 +85,7 @@ struct FeatArgs {
     const float* lo;
     const float* g;
     const BubbleWin* win;
-    const double* href;   // per bubble of the chunk
+    const float* href;    // per bubble of the chunk
     int first, nb, nt, np;
     float inv_sx;         // 1 / scale of the relative perturbation
     float sink_scale, diff_scale;
@@ -97,8 +101,10 @@ __global__ void features_kernel(const FeatArgs a) {
         const int b = static_cast<int>(q / n_cells);
         const long long c = q - static_cast<long long>(b) * n_cells;
         const BubbleWin w = a.win[a.first + b];
-        const double h = static_cast<double>(a.hi[w.off + c]) + static_cast<double>(a.lo[w.off + c]);
-        const double r = h / a.href[b] - 1.0;
+        // h / h_ref - 1 from the float-float pair: hi - h_ref is exact
+        // (Sterbenz) for a film within a factor 2 of its mean.
+        const float hr = a.href[b];
+        const float r = ((a.hi[w.off + c] - hr) + a.lo[w.off + c]) / hr;
         const int i = static_cast<int>(c / a.np);
         float* o = a.in + q * kCin;
         o[0] = static_cast<float>(r) * a.inv_sx;
@@ -111,7 +117,7 @@ __global__ void features_kernel(const FeatArgs a) {
 
 // target[b][c][j] = (h_j - h_t) / (h_ref s_y), j = 0 (step K - 1), 1 (step K).
 __global__ void targets_kernel(const float* hi_t, const float* lo_t, const float* hi_a, const float* lo_a,
-                               const float* hi_b, const float* lo_b, const BubbleWin* win, const double* href,
+                               const float* hi_b, const float* lo_b, const BubbleWin* win, const float* href,
                                int first, int nb, long long n_cells, float inv_sy, float* target) {
     const long long total = n_cells * nb;
     for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
@@ -119,19 +125,19 @@ __global__ void targets_kernel(const float* hi_t, const float* lo_t, const float
         const int b = static_cast<int>(q / n_cells);
         const long long c = q - static_cast<long long>(b) * n_cells;
         const long long o = win[first + b].off + c;
-        const double ht = static_cast<double>(hi_t[o]) + static_cast<double>(lo_t[o]);
-        const double ha = static_cast<double>(hi_a[o]) + static_cast<double>(lo_a[o]);
-        const double hb = static_cast<double>(hi_b[o]) + static_cast<double>(lo_b[o]);
-        const double s = 1.0 / href[b];
-        target[q * kCout + 0] = static_cast<float>((ha - ht) * s) * inv_sy;
-        target[q * kCout + 1] = static_cast<float>((hb - ht) * s) * inv_sy;
+        // Increments of float-float pairs: the hi differences are exact.
+        const float da = (hi_a[o] - hi_t[o]) + (lo_a[o] - lo_t[o]);
+        const float db = (hi_b[o] - hi_t[o]) + (lo_b[o] - lo_t[o]);
+        const float s = 1.f / href[b];
+        target[q * kCout + 0] = (da * s) * inv_sy;
+        target[q * kCout + 1] = (db * s) * inv_sy;
     }
 }
 
 // Predicted states as float-float arrays in the foam layout:
 // (hiA, loA) = h_t + out[.., 0] h_ref s_y,  (hiB, loB) = h_t + out[.., 1] h_ref s_y.
 __global__ void predict_states_kernel(const float* hi_t, const float* lo_t, const float* out, const BubbleWin* win,
-                                      const double* href, int first, int nb, long long n_cells, float sy, float* hiA,
+                                      const float* href, int first, int nb, long long n_cells, float sy, float* hiA,
                                       float* loA, float* hiB, float* loB) {
     const long long total = n_cells * nb;
     for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
@@ -139,7 +145,7 @@ __global__ void predict_states_kernel(const float* hi_t, const float* lo_t, cons
         const int b = static_cast<int>(q / n_cells);
         const long long c = q - static_cast<long long>(b) * n_cells;
         const long long o = win[first + b].off + c;
-        const float scale = static_cast<float>(href[b]) * sy;
+        const float scale = href[b] * sy;
         const float h = hi_t[o], l = lo_t[o];
         hiA[o] = h;
         loA[o] = l + out[q * kCout + 0] * scale;
@@ -148,33 +154,32 @@ __global__ void predict_states_kernel(const float* hi_t, const float* lo_t, cons
     }
 }
 
-// Per bubble: ||hJ - hB||^2, ||hB||^2, ||hB - h_t||^2 (one block per bubble).
+// Per bubble: ||hJ - hB||^2, ||hB||^2, ||hB - h_t||^2 (one block per bubble;
+// differences of float-float pairs, compensated FP32 sums).
 __global__ __launch_bounds__(256) void residual_kernel(const float* hiJ, const float* loJ, const float* hiB,
                                                        const float* loB, const float* hi_t, const float* lo_t,
-                                                       const BubbleWin* win, int n_cells, double* out) {
+                                                       const BubbleWin* win, int n_cells, float* out) {
     const long long off = win[blockIdx.x].off;
-    double e = 0.0, nrm = 0.0, inc = 0.0;
+    Kahan e, nrm, inc;
     for (int c = threadIdx.x; c < n_cells; c += blockDim.x) {
         const long long o = off + c;
-        const double hj = static_cast<double>(hiJ[o]) + static_cast<double>(loJ[o]);
-        const double hb = static_cast<double>(hiB[o]) + static_cast<double>(loB[o]);
-        const double ht = static_cast<double>(hi_t[o]) + static_cast<double>(lo_t[o]);
-        e += (hj - hb) * (hj - hb);
-        nrm += hb * hb;
-        inc += (hb - ht) * (hb - ht);
+        const float dj = (hiJ[o] - hiB[o]) + (loJ[o] - loB[o]);
+        const float hb = hiB[o] + loB[o];
+        const float di = (hiB[o] - hi_t[o]) + (loB[o] - lo_t[o]);
+        e.add(dj * dj);
+        nrm.add(hb * hb);
+        inc.add(di * di);
     }
-    __shared__ double red[3][256];
-    red[0][threadIdx.x] = e;
-    red[1][threadIdx.x] = nrm;
-    red[2][threadIdx.x] = inc;
+    __shared__ float red[3][256];
+    red[0][threadIdx.x] = e.value();
+    red[1][threadIdx.x] = nrm.value();
+    red[2][threadIdx.x] = inc.value();
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x < 3) {
+        Kahan t;
+        for (int q = 0; q < 256; ++q) t.add(red[threadIdx.x][q]);
+        out[3 * blockIdx.x + threadIdx.x] = t.value();
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) out[3 * blockIdx.x + k] = red[k][0];
 }
 
 // dst = keep[b] ? a : b_src per bubble (float-float pair).
@@ -259,8 +264,8 @@ struct Fno {
     float* dXs = nullptr;
     float* part = nullptr;     // split-K partials
     long long part_cap = 0;
-    double* loss_part = nullptr;
-    double* dscratch = nullptr;   // small FP64 scratch (means, residuals)
+    float* loss_part = nullptr;
+    float* dscratch = nullptr;    // small scratch (means, residuals)
     long long dscratch_cap = 0;
 
     // foam-facing buffers (sized on first use)
@@ -434,7 +439,7 @@ struct Fno {
         dxb = galloc<float>(R * C);
         dz = galloc<float>(R * C);
         dXs = galloc<float>(static_cast<long long>(Bm) * Mt * Mp * 2 * C);
-        loss_part = galloc<double>(4096);
+        loss_part = galloc<float>(4096);
         init_params(c.seed);
     }
 
@@ -521,13 +526,14 @@ struct Fno {
     double loss(int B) {
         const long long n = static_cast<long long>(B) * N * kCout;
         const unsigned nblk = blocks_for(n, 256, 4096);
-        hipLaunchKernelGGL(mse_kernel, dim3(nblk), dim3(256), 0, stream, out, target, dout, n, 1.0 / n, loss_part);
+        hipLaunchKernelGGL(mse_kernel, dim3(nblk), dim3(256), 0, stream, out, target, dout, n,
+                           static_cast<float>(2.0 / static_cast<double>(n)), loss_part);
         SF_HIP(hipGetLastError());
-        std::vector<double> h(nblk);
-        SF_HIP(hipMemcpyAsync(h.data(), loss_part, nblk * sizeof(double), hipMemcpyDeviceToHost, stream));
+        std::vector<float> h(nblk);
+        SF_HIP(hipMemcpyAsync(h.data(), loss_part, nblk * sizeof(float), hipMemcpyDeviceToHost, stream));
         SF_HIP(hipStreamSynchronize(stream));
         double s = 0.0;
-        for (double v : h) s += v;
+        for (float v : h) s += v;
         return s / static_cast<double>(n);
     }
 
@@ -622,7 +628,7 @@ struct Fno {
         }
         if (3LL * fi.nb > dscratch_cap) {
             (void)hipFree(dscratch);
-            dscratch = galloc<double>(3LL * fi.nb);
+            dscratch = galloc<float>(3LL * fi.nb);
             dscratch_cap = 3LL * fi.nb;
         }
         (void)prm;
@@ -651,16 +657,19 @@ struct Fno {
         hipLaunchKernelGGL(bubble_mean_kernel, dim3(fi.nb), dim3(256), 0, stream, hi, lo, d_win, 0,
                            static_cast<int>(N), dscratch);
         SF_HIP(hipGetLastError());
-        std::vector<double> m(fi.nb);
-        SF_HIP(hipMemcpyAsync(m.data(), dscratch, fi.nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+        std::vector<float> m(fi.nb);
+        SF_HIP(hipMemcpyAsync(m.data(), dscratch, fi.nb * sizeof(float), hipMemcpyDeviceToHost, stream));
         SF_HIP(hipStreamSynchronize(stream));
-        return m;
+        return std::vector<double>(m.begin(), m.end());
     }
 
     // h_ref of a chunk on the device (dscratch) and the chunk's input features.
     void build_features(const FoamInfo& fi, const std::vector<double>& href, int first, int nb, const float* hi,
                         const float* lo, const float* g) {
-        SF_HIP(hipMemcpyAsync(dscratch, href.data() + first, nb * sizeof(double), hipMemcpyHostToDevice, stream));
+        // the means are FP32 values carried in doubles on the host
+        std::vector<float> hr(href.begin() + first, href.begin() + first + nb);
+        SF_HIP(hipMemcpyAsync(dscratch, hr.data(), nb * sizeof(float), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipStreamSynchronize(stream));
         FeatArgs a;
         a.hi = hi;
         a.lo = lo;
@@ -769,7 +778,7 @@ struct Fno {
         auto ms_since = [](clk::time_point t) {
             return std::chrono::duration<double, std::milli>(clk::now() - t).count();
         };
-        std::vector<double> res(3LL * fi.nb);
+        std::vector<float> res(3LL * fi.nb);
         std::vector<uint8_t> take(fi.nb), fail(fi.nb);
         for (int wdx = 0; wdx < windows; ++wdx) {
             const float *hi, *lo, *g;
@@ -814,13 +823,14 @@ struct Fno {
             hipLaunchKernelGGL(residual_kernel, dim3(fi.nb), dim3(256), 0, stream, hi, lo, hb_hi, hb_lo, ht_hi,
                                ht_lo, d_win, static_cast<int>(N), dscratch);
             SF_HIP(hipGetLastError());
-            SF_HIP(hipMemcpyAsync(res.data(), dscratch, 3LL * fi.nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+            SF_HIP(hipMemcpyAsync(res.data(), dscratch, 3LL * fi.nb * sizeof(float), hipMemcpyDeviceToHost, stream));
             SF_HIP(hipStreamSynchronize(stream));
             const double ms_judge = ms_since(t1);
             int n_fail = 0;
             for (int b = 0; b < fi.nb; ++b) {
-                const double r = std::sqrt(res[3 * b] / std::max(res[3 * b + 1], 1e-300));
-                const double ri = std::sqrt(res[3 * b] / std::max(res[3 * b + 2], 1e-300));
+                const double e = res[3 * b], nrm = res[3 * b + 1], inc = res[3 * b + 2];
+                const double r = std::sqrt(e / std::max(nrm, 1e-300));
+                const double ri = std::sqrt(e / std::max(inc, 1e-300));
                 take[b] = r <= tol ? 1 : 0;
                 fail[b] = 1 - take[b];
                 n_fail += fail[b];

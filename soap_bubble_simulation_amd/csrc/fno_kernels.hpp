@@ -291,22 +291,37 @@ __global__ void act_grad_kernel(const float* z, const float* dx, float* dz, long
         dz[k] = act ? dx[k] * gelu_grad(z[k]) : dx[k];
 }
 
-// out[j] (+)= sum over rows of part[row][j], FP64 accumulation.  Block of
+// Compensated (Kahan) FP32 sum: s + c carries about twice FP32 precision,
+// so the reductions below need no FP64.
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ inline void add(float x) {
+#pragma clang fp contract(off)
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ inline float value() const { return s - c; }
+};
+
+// out[j] (+)= sum over rows of part[row][j], compensated FP32.  Block of
 // 256 threads: 32 consecutive columns x 8 row lanes, folded in LDS.
 __global__ __launch_bounds__(256) void column_sum_kernel(const float* part, long long rows, int cols, float* out,
                                                          int accumulate) {
     const int tx = threadIdx.x & 31, rl = threadIdx.x >> 5;
     const int j = blockIdx.x * 32 + tx;
-    double s = 0.0;
+    Kahan s;
     if (j < cols)
-        for (long long r = rl; r < rows; r += 8) s += part[r * cols + j];
-    __shared__ double red[8][32];
-    red[rl][tx] = s;
+        for (long long r = rl; r < rows; r += 8) s.add(part[r * cols + j]);
+    __shared__ float red[8][32];
+    red[rl][tx] = s.value();
     __syncthreads();
     if (rl == 0 && j < cols) {
-        double t = 0.0;
-        for (int q = 0; q < 8; ++q) t += red[q][tx];
-        out[j] = accumulate ? out[j] + static_cast<float>(t) : static_cast<float>(t);
+        Kahan t;
+        if (accumulate) t.add(out[j]);
+        for (int q = 0; q < 8; ++q) t.add(red[q][tx]);
+        out[j] = t.value();
     }
 }
 
@@ -315,19 +330,19 @@ __global__ __launch_bounds__(256) void column_partial_kernel(const float* x, lon
     // 256 threads: thread t sums column (t % C) over rows congruent to t / C.
     const int per = 256 / C;
     const int c = threadIdx.x % C, lane_row = threadIdx.x / C;
-    double s = 0.0;
+    Kahan s;
     if (lane_row < per) {
         for (long long r = blockIdx.x * static_cast<long long>(per) + lane_row; r < rows;
              r += static_cast<long long>(gridDim.x) * per)
-            s += x[r * C + c];
+            s.add(x[r * C + c]);
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
+    __shared__ float red[256];
+    red[threadIdx.x] = s.value();
     __syncthreads();
     if (threadIdx.x < C) {
-        double t = 0.0;
-        for (int q = 0; q < per; ++q) t += red[q * C + threadIdx.x];
-        partial[blockIdx.x * C + threadIdx.x] = static_cast<float>(t);
+        Kahan t;
+        for (int q = 0; q < per; ++q) t.add(red[q * C + threadIdx.x]);
+        partial[blockIdx.x * C + threadIdx.x] = t.value();
     }
 }
 
@@ -451,24 +466,25 @@ __global__ void adam_kernel(float* w, const float* g, float* m, float* v, long l
 }
 
 // Loss L = mean((out - target)^2) pieces: dout = 2 (out - target) / count and
-// per-block partial sums of the squared error (FP64).
+// per-block partial sums of the squared error (compensated FP32; the host
+// adds the partials).
 __global__ __launch_bounds__(256) void mse_kernel(const float* out, const float* target, float* dout, long long n,
-                                                  double inv_count, double* partial) {
-    double s = 0.0;
+                                                  float two_inv_count, float* partial) {
+    Kahan s;
     for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
          k += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const double d = static_cast<double>(out[k]) - static_cast<double>(target[k]);
-        s += d * d;
-        dout[k] = static_cast<float>(2.0 * d * inv_count);
+        const float d = out[k] - target[k];
+        s.add(d * d);
+        dout[k] = two_inv_count * d;
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
+    __shared__ float red[256];
+    red[threadIdx.x] = s.value();
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        Kahan t;
+        for (int q = 0; q < 256; ++q) t.add(red[q]);
+        partial[blockIdx.x] = t.value();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 }  // namespace fno
