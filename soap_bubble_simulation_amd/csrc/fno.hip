@@ -59,25 +59,55 @@ struct BubbleWin {
     float s_diff;         // K dt D / (R^2 dtheta^2)
 };
 
-// Mean thickness of each bubble (one block per bubble, compensated FP32).
-// It is a reference value: the features and the reconstruction of a window
-// use the same one, so its last bits do not matter.
+// Per-bubble reductions run in two passes: blockIdx.y slices a bubble's cells
+// into gridDim.y parts (enough blocks to fill the GPU at one bubble), each
+// block writes a partial, and fold_partials_kernel adds them in order.
+__device__ inline void slice_of(int n_cells, int& c0, int& c1) {
+    const int per = (n_cells + static_cast<int>(gridDim.y) - 1) / static_cast<int>(gridDim.y);
+    c0 = min(n_cells, static_cast<int>(blockIdx.y) * per);
+    c1 = min(n_cells, c0 + per);
+}
+
+// Block-wide compensated sum of v (256 threads), returned to thread 0.
+__device__ inline float block_sum(float v, float* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    float out = 0.f;
+    if (threadIdx.x == 0) {
+        Kahan t;
+        for (int q = 0; q < 256; ++q) t.add(red[q]);
+        out = t.value();
+    }
+    __syncthreads();
+    return out;
+}
+
+// Partial sums of h over a slice of each bubble (compensated FP32).  The
+// mean is a reference value: the features and the reconstruction of a
+// window use the same one, so its last bits do not matter.
 __global__ __launch_bounds__(256) void bubble_mean_kernel(const float* hi, const float* lo, const BubbleWin* win,
-                                                          int first, int n_cells, float* mean) {
+                                                          int first, int n_cells, float* part) {
     const BubbleWin w = win[first + blockIdx.x];
+    int c0, c1;
+    slice_of(n_cells, c0, c1);
     Kahan s;
-    for (int c = threadIdx.x; c < n_cells; c += blockDim.x) {
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
         s.add(hi[w.off + c]);
         s.add(lo[w.off + c]);
     }
     __shared__ float red[256];
-    red[threadIdx.x] = s.value();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Kahan t;
-        for (int q = 0; q < 256; ++q) t.add(red[q]);
-        mean[blockIdx.x] = t.value() / static_cast<float>(n_cells);
-    }
+    const float t = block_sum(s.value(), red);
+    if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.x) * gridDim.y + blockIdx.y] = t;
+}
+
+// out[b * k + j] = scale * sum over s of part[(b * S + s) * k + j].
+__global__ void fold_partials_kernel(const float* part, int nb, int S, int k, float scale, float* out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb * k) return;
+    const int b = q / k, j = q - b * k;
+    Kahan t;
+    for (int s = 0; s < S; ++s) t.add(part[(static_cast<size_t>(b) * S + s) * k + j]);
+    out[q] = t.value() * scale;
 }
 
 struct FeatArgs {
@@ -154,14 +184,16 @@ __global__ void predict_states_kernel(const float* hi_t, const float* lo_t, cons
     }
 }
 
-// Per bubble: ||hJ - hB||^2, ||hB||^2, ||hB - h_t||^2 (one block per bubble;
-// differences of float-float pairs, compensated FP32 sums).
+// Per bubble and slice: ||hJ - hB||^2, ||hB||^2, ||hB - h_t||^2 (differences
+// of float-float pairs, compensated FP32 sums), partials [b][slice][3].
 __global__ __launch_bounds__(256) void residual_kernel(const float* hiJ, const float* loJ, const float* hiB,
                                                        const float* loB, const float* hi_t, const float* lo_t,
-                                                       const BubbleWin* win, int n_cells, float* out) {
+                                                       const BubbleWin* win, int n_cells, float* part) {
     const long long off = win[blockIdx.x].off;
+    int c0, c1;
+    slice_of(n_cells, c0, c1);
     Kahan e, nrm, inc;
-    for (int c = threadIdx.x; c < n_cells; c += blockDim.x) {
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
         const long long o = off + c;
         const float dj = (hiJ[o] - hiB[o]) + (loJ[o] - loB[o]);
         const float hb = hiB[o] + loB[o];
@@ -170,15 +202,15 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* hiJ, const f
         nrm.add(hb * hb);
         inc.add(di * di);
     }
-    __shared__ float red[3][256];
-    red[0][threadIdx.x] = e.value();
-    red[1][threadIdx.x] = nrm.value();
-    red[2][threadIdx.x] = inc.value();
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        Kahan t;
-        for (int q = 0; q < 256; ++q) t.add(red[threadIdx.x][q]);
-        out[3 * blockIdx.x + threadIdx.x] = t.value();
+    __shared__ float red[256];
+    const float te = block_sum(e.value(), red);
+    const float tn = block_sum(nrm.value(), red);
+    const float ti = block_sum(inc.value(), red);
+    if (threadIdx.x == 0) {
+        float* o = part + (static_cast<size_t>(blockIdx.x) * gridDim.y + blockIdx.y) * 3;
+        o[0] = te;
+        o[1] = tn;
+        o[2] = ti;
     }
 }
 
@@ -267,6 +299,9 @@ struct Fno {
     float* loss_part = nullptr;
     float* dscratch = nullptr;    // small scratch (means, residuals)
     long long dscratch_cap = 0;
+    float* rpart = nullptr;       // per (bubble, slice) partial sums of the two-pass reductions
+    std::vector<float> href_f;    // host FP32 bubble means of the current window
+    long long rpart_cap = 0;
 
     // foam-facing buffers (sized on first use)
     long long foam_cells = 0;
@@ -280,7 +315,7 @@ struct Fno {
         if (stream) (void)hipStreamSynchronize(stream);
         void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, in, out, target, dout, ybuf, X1, G, Y, Z,
                       dxa, dxb, dz, dXs, part, loss_part, dscratch, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
-                      sel_hi, sel_lo, hp_hi, hp_lo, d_win, d_take};
+                      sel_hi, sel_lo, hp_hi, hp_lo, d_win, d_take, rpart};
         for (void* p : ps) (void)hipFree(p);
         for (float* p : xs) (void)hipFree(p);
         for (float* p : zs) (void)hipFree(p);
@@ -492,10 +527,10 @@ struct Fno {
     // training: keep the pre-activations for the backward pass
     void forward(int B, bool training) {
         const long long R = static_cast<long long>(B) * N;
-        // lift
-        gemm(in, w + o_wl, xs[0], static_cast<int>(R), C, kCin, kCin, C, C, 0, 0, 0, 1, 0, 0, 1.f, 0.f);
-        hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(R * C, 256, 16384)), dim3(256), 0, stream, xs[0],
-                           static_cast<const float*>(nullptr), w + o_bl, xs[0], R * C, C, 0);
+        // lift (FP32, bias included)
+        hipLaunchKernelGGL(lift_kernel<kCin>, dim3(blocks_for(R * ((C + 3) / 4), 256, 16384)), dim3(256), 0, stream, in,
+                           w + o_wl, w + o_bl, xs[0], R, C);
+        SF_HIP(hipGetLastError());
         int row_threads = 256;
         const RowKernel rk = row_kernel(&row_threads);
         if (rk) {
@@ -517,9 +552,9 @@ struct Fno {
                                    ybuf, w + o_b[l], xs[l + 1], R * C, C, l < L - 1 ? 1 : 0);
             }
         }
-        gemm(xs[L], w + o_q, out, static_cast<int>(R), kCout, C, C, kCout, kCout, 0, 0, 0, 1, 0, 0, 1.f, 0.f);
-        hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(R * kCout, 256, 16384)), dim3(256), 0, stream, out,
-                           static_cast<const float*>(nullptr), w + o_qb, out, R * kCout, kCout, 0);
+        // projection (FP32, bias included)
+        hipLaunchKernelGGL(project_kernel<kCout>, dim3(blocks_for(R, 256, 16384)), dim3(256), 0, stream, xs[L],
+                           w + o_q, w + o_qb, out, R, C);
         SF_HIP(hipGetLastError());
     }
 
@@ -631,7 +666,17 @@ struct Fno {
             dscratch = galloc<float>(3LL * fi.nb);
             dscratch_cap = 3LL * fi.nb;
         }
+        if (3LL * fi.nb * slices() > rpart_cap) {
+            (void)hipFree(rpart);
+            rpart = galloc<float>(3LL * fi.nb * slices());
+            rpart_cap = 3LL * fi.nb * slices();
+        }
         (void)prm;
+    }
+
+    // Slices per bubble of the two-pass reductions: about 2048 cells each.
+    int slices() const {
+        return static_cast<int>(std::max<long long>(1, std::min<long long>(256, (N + 2047) / 2048)));
     }
 
     // Window numbers of each bubble at mean thickness h_ref (host copy of the means).
@@ -654,8 +699,12 @@ struct Fno {
         std::vector<BubbleWin> tmp(fi.nb);
         for (int b = 0; b < fi.nb; ++b) tmp[b].off = fi.offs[b];
         SF_HIP(hipMemcpyAsync(d_win, tmp.data(), fi.nb * sizeof(BubbleWin), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(bubble_mean_kernel, dim3(fi.nb), dim3(256), 0, stream, hi, lo, d_win, 0,
-                           static_cast<int>(N), dscratch);
+        const int S = slices();
+        hipLaunchKernelGGL(bubble_mean_kernel, dim3(fi.nb, S), dim3(256), 0, stream, hi, lo, d_win, 0,
+                           static_cast<int>(N), rpart);
+        SF_HIP(hipGetLastError());
+        hipLaunchKernelGGL(fold_partials_kernel, dim3((fi.nb + 255) / 256), dim3(256), 0, stream, rpart, fi.nb, S, 1,
+                           static_cast<float>(1.0 / static_cast<double>(N)), dscratch);
         SF_HIP(hipGetLastError());
         std::vector<float> m(fi.nb);
         SF_HIP(hipMemcpyAsync(m.data(), dscratch, fi.nb * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -666,10 +715,13 @@ struct Fno {
     // h_ref of a chunk on the device (dscratch) and the chunk's input features.
     void build_features(const FoamInfo& fi, const std::vector<double>& href, int first, int nb, const float* hi,
                         const float* lo, const float* g) {
-        // the means are FP32 values carried in doubles on the host
-        std::vector<float> hr(href.begin() + first, href.begin() + first + nb);
-        SF_HIP(hipMemcpyAsync(dscratch, hr.data(), nb * sizeof(float), hipMemcpyHostToDevice, stream));
-        SF_HIP(hipStreamSynchronize(stream));
+        // The means are FP32 values carried in doubles on the host; the FP32
+        // copy of the whole window lives in a member buffer (rewritten with the
+        // same values for every chunk, and only after the window's last
+        // synchronisation for the next window), so no transfer reads a buffer
+        // that changes under it.
+        href_f.assign(href.begin(), href.end());
+        SF_HIP(hipMemcpyAsync(dscratch, href_f.data() + first, nb * sizeof(float), hipMemcpyHostToDevice, stream));
         FeatArgs a;
         a.hi = hi;
         a.lo = lo;
@@ -820,8 +872,11 @@ struct Fno {
             SF_REQUIRE(sf_foam_set_state(f, ha_hi, ha_lo, nullptr) == SF_OK, SF_ERR_DEVICE, "%s", sf_last_error());
             SF_REQUIRE(sf_foam_step(f, 1) == SF_OK, SF_ERR_DEVICE, "%s", sf_last_error());
             SF_REQUIRE(sf_foam_device_state(f, &hi, &lo, nullptr) == SF_OK, SF_ERR_DEVICE, "%s", sf_last_error());
-            hipLaunchKernelGGL(residual_kernel, dim3(fi.nb), dim3(256), 0, stream, hi, lo, hb_hi, hb_lo, ht_hi,
-                               ht_lo, d_win, static_cast<int>(N), dscratch);
+            hipLaunchKernelGGL(residual_kernel, dim3(fi.nb, slices()), dim3(256), 0, stream, hi, lo, hb_hi, hb_lo,
+                               ht_hi, ht_lo, d_win, static_cast<int>(N), rpart);
+            SF_HIP(hipGetLastError());
+            hipLaunchKernelGGL(fold_partials_kernel, dim3((3 * fi.nb + 255) / 256), dim3(256), 0, stream, rpart, fi.nb,
+                               slices(), 3, 1.f, dscratch);
             SF_HIP(hipGetLastError());
             SF_HIP(hipMemcpyAsync(res.data(), dscratch, 3LL * fi.nb * sizeof(float), hipMemcpyDeviceToHost, stream));
             SF_HIP(hipStreamSynchronize(stream));

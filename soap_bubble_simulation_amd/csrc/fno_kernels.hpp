@@ -284,6 +284,90 @@ __global__ void bias_act_kernel(float* z, const float* y, const float* bias, flo
     }
 }
 
+// Lift: x[r][c] = b[c] + sum_i in[r][i] W[i][c] for the NIN input channels,
+// in FP32.  Replaces a K = NIN GEMM and a bias pass over x.  When C is a
+// multiple of 4 whose quarter divides the block, a thread writes four
+// channels of one row with one 16-byte store (its weights held in registers)
+// and the block covers blockDim / (C/4) whole rows; otherwise one thread per
+// element.
+template <int NIN>
+__global__ __launch_bounds__(256) void lift_kernel(const float* __restrict__ in, const float* __restrict__ W,
+                                                   const float* __restrict__ b, float* __restrict__ x, long long rows,
+                                                   int C) {
+    const int C4 = C >> 2;
+    if (C4 > 0 && (C & 3) == 0 && static_cast<int>(blockDim.x) % C4 == 0) {
+        const int per = static_cast<int>(blockDim.x) / C4;
+        const int rl = static_cast<int>(threadIdx.x) / C4;
+        const int c0 = 4 * (static_cast<int>(threadIdx.x) - rl * C4);
+        float4 w[NIN];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) w[i] = *reinterpret_cast<const float4*>(W + i * C + c0);
+        const float4 bias = *reinterpret_cast<const float4*>(b + c0);
+        for (long long r = blockIdx.x * static_cast<long long>(per) + rl; r < rows;
+             r += static_cast<long long>(gridDim.x) * per) {
+            float4 v = bias;
+#pragma unroll
+            for (int i = 0; i < NIN; ++i) {
+                const float a = in[r * NIN + i];
+                v.x = fmaf(a, w[i].x, v.x);
+                v.y = fmaf(a, w[i].y, v.y);
+                v.z = fmaf(a, w[i].z, v.z);
+                v.w = fmaf(a, w[i].w, v.w);
+            }
+            *reinterpret_cast<float4*>(x + r * C + c0) = v;
+        }
+        return;
+    }
+    const long long total = rows * C;
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < total;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long r = k / C;
+        const int c = static_cast<int>(k - r * C);
+        float v = b[c];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) v = fmaf(in[r * NIN + i], W[i * C + c], v);
+        x[k] = v;
+    }
+}
+
+// Projection: out[r][j] = q[j] + sum_c x[r][c] Q[c][j] for the NOUT output
+// channels, in FP32, one thread per row.  Replaces an N = NOUT GEMM (a 64-wide
+// tile for two columns) and a bias pass.
+template <int NOUT>
+__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, const float* __restrict__ Q,
+                                                      const float* __restrict__ q, float* __restrict__ out,
+                                                      long long rows, int C) {
+    __shared__ float sQ[64 * NOUT + NOUT];
+    for (int t = threadIdx.x; t < C * NOUT; t += blockDim.x) sQ[t] = Q[t];
+    for (int t = threadIdx.x; t < NOUT; t += blockDim.x) sQ[64 * NOUT + t] = q[t];
+    __syncthreads();
+    for (long long r = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; r < rows;
+         r += static_cast<long long>(gridDim.x) * blockDim.x) {
+        float o[NOUT];
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) o[j] = sQ[64 * NOUT + j];
+        const float* xr = x + r * C;
+        if ((C & 3) == 0) {
+            for (int c = 0; c < C; c += 4) {
+                const float4 v = *reinterpret_cast<const float4*>(xr + c);
+#pragma unroll
+                for (int j = 0; j < NOUT; ++j) {
+                    o[j] = fmaf(v.x, sQ[(c + 0) * NOUT + j], o[j]);
+                    o[j] = fmaf(v.y, sQ[(c + 1) * NOUT + j], o[j]);
+                    o[j] = fmaf(v.z, sQ[(c + 2) * NOUT + j], o[j]);
+                    o[j] = fmaf(v.w, sQ[(c + 3) * NOUT + j], o[j]);
+                }
+            }
+        } else {
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int j = 0; j < NOUT; ++j) o[j] = fmaf(xr[c], sQ[c * NOUT + j], o[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) out[r * NOUT + j] = o[j];
+    }
+}
+
 // dz = dx * act'(z) (in place on dx when act, else a copy).
 __global__ void act_grad_kernel(const float* z, const float* dx, float* dz, long long total, int act) {
     for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < total;
