@@ -233,8 +233,12 @@ typedef struct sf_camera {
 /* Per-cell spectral interference: 81 wavelengths (380-780 nm, 5 nm step) of
  * the Airy/Fresnel model of interference_lut.rs:54-72 at each cell's view
  * angle |dot(normalize(cam - p), n)| (bubble.wgsl:491-494), contracted with
- * cmf[81][3] on BF16 MFMA with FP32 accumulation:
+ * cmf[81][3] on 16-bit MFMA with FP32 accumulation:
  *   rgb[c][cell] = (1/81) sum_k I_k cmf[k][c].
+ * Operand format (sf_spectral_set_operand): by default FP16 when its
+ * rounding of the reflectance moves no channel by more than 2.5e-4
+ * (2^-12 sum_k |cmf[k][c]| / 81) and the curves fit FP16, else a BF16 hi/lo
+ * pair that keeps about 16 bits.
  * Reads the context's current h; writes 3 planes of n_theta*n_phi floats
  * to host memory (rgb_host) and/or keeps them on the device. */
 int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_index,
@@ -269,6 +273,17 @@ int sf_spectral_copy_rgb(sf_drainage* ctx, float* rgb_host);
  * (HIP events on the context's stream); enable switches recording on/off. */
 int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
                               int64_t* launches);
+/* Operand format of the spectral contraction: AUTO (the default, see
+ * sf_spectral_rgb), BF16_PAIR, or FP16 (an error at the next spectral call
+ * if some |cmf| exceeds 2^14).  No counterpart in the reference, which has
+ * no spectral contraction; for accuracy studies and tests. */
+#define SF_SPECTRAL_OPERAND_AUTO 0
+#define SF_SPECTRAL_OPERAND_BF16_PAIR 1
+#define SF_SPECTRAL_OPERAND_FP16 2
+int sf_spectral_set_operand(sf_drainage* ctx, int32_t kind);
+/* Operand format the next spectral call uses with the curves last given
+ * (SF_SPECTRAL_OPERAND_BF16_PAIR or _FP16; AUTO before any curves). */
+int sf_spectral_get_operand(sf_drainage* ctx, int32_t* kind);
 
 /* ---- caustic intensity (consumer of the thickness field) ------------------
  * CausticParams of caustics.rs:20-54 without the grid size (taken from the

@@ -24,6 +24,9 @@ FLAG_TFE = 1 << 2
 FLAG_POLES_COPY_H = 1 << 3
 FLAG_POLES_AVG_GAMMA = 1 << 4
 FLAG_DIAG = 1 << 5
+SPECTRAL_OPERAND_AUTO = 0
+SPECTRAL_OPERAND_BF16_PAIR = 1
+SPECTRAL_OPERAND_FP16 = 2
 
 PREDICT_NETWORK = 0
 PREDICT_IDENTITY = 1
@@ -201,6 +204,8 @@ SIGNATURES = {
     "sf_optical_path": (C.c_int, [C.c_double, C.c_int32, _fp, _fp, _fp, _fp]),
     "sf_phase_difference": (C.c_int, [C.c_int32, _fp, _fp, _fp]),
     "sf_spectral_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "sf_spectral_set_operand": (C.c_int, [_vp, C.c_int32]),
+    "sf_spectral_get_operand": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_drainage_step_spectral": (C.c_int, [_vp, C.c_int32, C.c_double, C.POINTER(Camera),
                                             C.c_double, _fp]),
     "sf_drainage_synchronize": (C.c_int, [_vp]),

@@ -1140,3 +1140,26 @@ extern "C" int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, doubl
         timer_query(d, d->spec_timer, enable, avg_ms, launches);
     });
 }
+
+extern "C" int sf_spectral_set_operand(sf_drainage* ctx, int32_t kind) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
+        SF_REQUIRE(kind == SF_SPECTRAL_OPERAND_AUTO || kind == SF_SPECTRAL_OPERAND_BF16_PAIR ||
+                       kind == SF_SPECTRAL_OPERAND_FP16,
+                   SF_ERR_INVALID, "unknown spectral operand format %d", kind);
+        D(ctx)->spec_operand = kind;
+    });
+}
+
+extern "C" int sf_spectral_get_operand(sf_drainage* ctx, int32_t* kind) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && kind != nullptr, SF_ERR_INVALID, "bad arguments");
+        const Drainage* d = D(ctx);
+        if (!d->cmf_dev)
+            *kind = SF_SPECTRAL_OPERAND_AUTO;
+        else if (d->spec_operand == SF_SPECTRAL_OPERAND_AUTO)
+            *kind = (d->cmf_f16_range && d->cmf_f16_bound) ? SF_SPECTRAL_OPERAND_FP16 : SF_SPECTRAL_OPERAND_BF16_PAIR;
+        else
+            *kind = d->spec_operand;
+    });
+}

@@ -86,8 +86,14 @@ struct Drainage {
     float* rgb = nullptr;
     float* caustic = nullptr;       // caustic-intensity map (caustics.hip), allocated on first use
     float* cmf_dev = nullptr;       // colour curves as MFMA B fragments
-    float cmf_sum[3] = {0.f, 0.f, 0.f};   // sum over the 81 wavelengths of each curve
-    int cmf_lo = 0;                 // the curves are not exact in BF16
+    // Per operand format (0: BF16 hi/lo pair, 1: FP16): sum over the 81
+    // wavelengths of each curve as the MFMA sees it, and whether the curves
+    // need their lo fragments.
+    float cmf_sum[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    int cmf_lo[2] = {0, 0};
+    bool cmf_f16_bound = false;     // FP16 r keeps the RGB error within kSpecF16Budget
+    bool cmf_f16_range = false;     // the curves fit FP16
+    int spec_operand = 0;           // SF_SPECTRAL_OPERAND_*
     float4* spec_rows = nullptr;
     float2* spec_view = nullptr;    // per-cell (F, cos theta_t) of spec_view_key's camera and index
     float spec_view_key[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};

@@ -234,24 +234,37 @@ __global__ void calc_kernel(CalcArgs a) {
 // that the cancellation in 1 + cos would cost accuracy, and the square form
 // runs (with v_fract_f32 range reduction, which also covers long paths).
 //
-// v_mfma_f32_16x16x32_bf16: lane l holds A[m][8 q + v] and B[8 q + v][m]
+// v_mfma_f32_16x16x32_{f16,bf16}: lane l holds A[m][8 q + v] and B[8 q + v][m]
 // (m = l % 16, q = l / 16, v = 0..7) and receives D[4 q + v][m].  A row m is
 // a cell of a 16-cell group; lane (m, q) evaluates r for wavelengths
 // k = q + 4 s, s = 0..20 (84 slots, 81 used, B zero on the rest), slot s in
-// MFMA t = s / 8 at v = s % 8; slots 21..23 are zero.  r is split into
+// MFMA t = s / 8 at v = s % 8; slots 21..23 are zero.  A wave takes 64 cells
+// at a time as four 16-cell groups.
+//
+// Operand formats.  r lies in [0, 1], where FP16 (11-bit significand) rounds
+// to nearest within 2^-12, so one FP16 operand changes channel c by at most
+// 2^-12 sum_k |cmf[k][c]| / 81; that is the default (one v_cvt_pk_f16_f32 per
+// two slots, one MFMA per group and curve part) whenever this bound stays
+// within kSpecF16Budget and the curves fit FP16.  Otherwise r is split into
 // truncated BF16 hi and lo = r - hi (exact), each contracted with the BF16
-// curves (plus hi times the curves' own lo part when they are not exact in
-// BF16), which keeps about 16 bits of r.  A wave takes 64 cells at a time
-// as four 16-cell groups.
+// curves, which keeps about 16 bits of r at three more VALU operations per
+// slot.  Either way the curves are a hi part plus the remainder's lo part in
+// the same format, and hi times lo runs only when some lo is nonzero.
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSpecLambdas = 81;
 constexpr int kSpecSlots = 21;      // wavelength slots per lane and cell
 constexpr int kSpecMfma = 3;        // 16x16x32 MFMAs per group and operand part
 constexpr float kSpecFCos2 = 16.f;
+// Largest RGB change the FP16 rounding of r may cause for the FP16 operand
+// format to be chosen automatically (a quarter of the 1e-3 RGB tolerance).
+constexpr double kSpecF16Budget = 2.5e-4;
 // Paths below which v_cos_f32 (argument in revolutions, exact to its
 // specification for |x| <= 256) takes opd / lambda unreduced.
 constexpr float kSpecOpFast = 5.0e4f;
@@ -261,15 +274,15 @@ struct SpecArgs {
     const float* lo;
     const float4* rows;     // per theta row: sin, cos
     const float2* cols;     // per phi column: sin, cos
-    const u32x4* bfrag;     // [2 (hi, lo)][3 MFMAs][64 lanes] B fragments, bf16 x 8
+    const u32x4* bfrag;     // [2 (hi, lo)][3 MFMAs][64 lanes] B fragments of the operand format
     const float* ulam;      // [21 slots][4 q] 1 / (2 lambda) of wavelength q + 4 s (0 past 80)
     float* rgb;             // 3 planes of n cells
     int np;
     size_t n;
     float cx, cy, cz, cam2;  // camera and |cam|^2
     float radius, n_film, inv_n_film;
-    float s0, s1, s2;       // sum over the wavelengths of each (BF16 hi + lo) curve
-    int b_lo;               // the curves have a nonzero BF16 lo part
+    float s0, s1, s2;       // sum over the wavelengths of each (hi + lo) curve
+    int b_lo;               // the curves have a nonzero lo part
     // Pole rows held as ring partials (lazy poles): their h is the ring mean
     // from hpole, as materialize_poles_kernel would write it.
     const float2* hpole;
@@ -358,7 +371,7 @@ __device__ inline uint32_t pack_hi16(float x0, float x1) {
     return __builtin_amdgcn_perm(__float_as_uint(x1), __float_as_uint(x0), 0x07060302u);
 }
 
-template <bool FAST>
+template <bool FAST, bool F16>
 __device__ inline void spec_group(const float (&u)[kSpecSlots], const u32x4 (&bh)[kSpecMfma],
                                   const u32x4 (&bl)[kSpecMfma], bool b_lo, float P, float A, float B,
                                   v4f& acc) {
@@ -367,24 +380,36 @@ __device__ inline void spec_group(const float (&u)[kSpecSlots], const u32x4 (&bh
         float r[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) r[v] = (8 * t + v < kSpecSlots) ? spec_r<FAST>(P, A, B, u[8 * t + v]) : 0.f;
-        u32x4 h, l;
+        if (F16) {
+            u32x4 h;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const float h0 = __uint_as_float(__float_as_uint(r[2 * p]) & 0xFFFF0000u);
-            const float h1 = __uint_as_float(__float_as_uint(r[2 * p + 1]) & 0xFFFF0000u);
-            h[p] = pack_hi16(r[2 * p], r[2 * p + 1]);
-            l[p] = pack_hi16(r[2 * p] - h0, r[2 * p + 1] - h1);
+            for (int p = 0; p < 4; ++p)
+                h[p] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r[2 * p], r[2 * p + 1]}, f16x2));
+            const f16x8 ah = __builtin_bit_cast(f16x8, h);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(f16x8, bh[t]), acc, 0, 0, 0);
+            if (b_lo) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(f16x8, bl[t]), acc, 0, 0, 0);
+        } else {
+            u32x4 h, l;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const float h0 = __uint_as_float(__float_as_uint(r[2 * p]) & 0xFFFF0000u);
+                const float h1 = __uint_as_float(__float_as_uint(r[2 * p + 1]) & 0xFFFF0000u);
+                h[p] = pack_hi16(r[2 * p], r[2 * p + 1]);
+                l[p] = pack_hi16(r[2 * p] - h0, r[2 * p + 1] - h1);
+            }
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, h);
+            const bf16x8 al = __builtin_bit_cast(bf16x8, l);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, __builtin_bit_cast(bf16x8, bh[t]), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, __builtin_bit_cast(bf16x8, bh[t]), acc, 0, 0, 0);
+            if (b_lo)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, __builtin_bit_cast(bf16x8, bl[t]), acc, 0, 0, 0);
         }
-        const bf16x8 ah = __builtin_bit_cast(bf16x8, h);
-        const bf16x8 al = __builtin_bit_cast(bf16x8, l);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, __builtin_bit_cast(bf16x8, bh[t]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, __builtin_bit_cast(bf16x8, bh[t]), acc, 0, 0, 0);
-        if (b_lo) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, __builtin_bit_cast(bf16x8, bl[t]), acc, 0, 0, 0);
     }
 }
 
 // Five waves per SIMD (96 VGPRs; the compiler moves a few per-batch values
 // to scratch outside the wavelength loop): 1.4 % faster than four.
+template <bool F16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void spectral_kernel(const SpecArgs a) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 15;
@@ -420,9 +445,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             const float op = lane_read(opc, 16 * g + m);
             acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
             if (fast)
-                spec_group<true>(u, bh, bl, b_lo, 2.f * op, fmaf(0.5f, F, 1.f), 0.5f * F, acc[g]);
+                spec_group<true, F16>(u, bh, bl, b_lo, 2.f * op, fmaf(0.5f, F, 1.f), 0.5f * F, acc[g]);
             else
-                spec_group<false>(u, bh, bl, b_lo, op, 1.f, F, acc[g]);
+                spec_group<false, F16>(u, bh, bl, b_lo, op, 1.f, F, acc[g]);
         }
         // acc[g][v] = D[4 q + v][m]: cell 64 sg + 16 g + 4 q + v, colour m.
         if (m < 3) {
@@ -658,9 +683,10 @@ extern "C" int sf_phase_difference(int32_t n, const float* opd_nm, const float* 
 }
 
 // Colour curves on the device: B fragments of spectral_kernel,
-// [2 (hi, lo)][3 MFMAs][64 lanes] of bf16 x 8, then the slot table of
-// 1 / (2 lambda) (FP64 quotients, rounded).
-constexpr size_t kCmfFrags = 2 * kSpecMfma * 64;
+// [2 formats (BF16, FP16)][2 (hi, lo)][3 MFMAs][64 lanes] of 16-bit x 8,
+// then the slot table of 1 / (2 lambda) (FP64 quotients, rounded).
+constexpr size_t kCmfFragsPerFormat = 2 * kSpecMfma * 64;
+constexpr size_t kCmfFrags = 2 * kCmfFragsPerFormat;
 constexpr size_t kCmfFloats = kCmfFrags * 4 + 4 * kSpecSlots;
 
 static uint16_t bf16_rne(float x) {
@@ -677,46 +703,85 @@ static float bf16_value(uint16_t h) {
     return x;
 }
 
+// Round-to-nearest-even FP16 (the host compiler's conversion) and back.
+static uint16_t f16_rne(float x) {
+    const _Float16 h = static_cast<_Float16>(x);
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+static float f16_value(uint16_t u) {
+    _Float16 h;
+    std::memcpy(&h, &u, 2);
+    return static_cast<float>(h);
+}
+
 struct PackedCmf {
     std::vector<u32x4> frag;
     float ulam[4 * kSpecSlots];
-    float sum[3];
-    int has_lo;
+    float sum[2][3];          // [format][colour]
+    int has_lo[2];
+    bool f16_bound;           // 2^-12 sum_k |cmf[k][c]| / 81 <= kSpecF16Budget for every c
+    bool f16_range;           // every |cmf| <= 2^14
 };
 
 // Packs cmf[81][3] into the B-operand layout of spectral_kernel: lane
 // 16 q + m of MFMA t holds cmf[q + 4 (8 t + v)][m] at v = 0..7 (zero past
-// slot 20, wavelength 80 or colour 2), as round-to-nearest BF16 hi and the
-// BF16 of the remainder.  The sums are those of hi + lo, the values the
-// MFMA multiplies.
+// slot 20, wavelength 80 or colour 2), as the round-to-nearest hi part and
+// the rounded remainder (lo), once in BF16 and once in FP16.  The sums are
+// those of hi + lo, the values the MFMA multiplies.
 static PackedCmf pack_cmf(const float* cmf) {
     PackedCmf out;
     out.frag.assign(kCmfFrags, u32x4{0u, 0u, 0u, 0u});
-    out.has_lo = 0;
-    double sum[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < kSpecMfma; ++t)
-        for (int lane = 0; lane < 64; ++lane) {
-            const int m = lane & 15, q = lane >> 4;
-            uint16_t hv[8], lv[8];
-            for (int v = 0; v < 8; ++v) {
-                const int s = 8 * t + v;
-                const int k = q + 4 * s;
-                const float x = (s < kSpecSlots && k < kSpecLambdas && m < 3) ? cmf[3 * k + m] : 0.f;
-                hv[v] = bf16_rne(x);
-                lv[v] = bf16_rne(x - bf16_value(hv[v]));
-                if (lv[v] & 0x7FFFu) out.has_lo = 1;
-                if (s < kSpecSlots && k < kSpecLambdas && m < 3)
-                    sum[m] += static_cast<double>(bf16_value(hv[v])) + static_cast<double>(bf16_value(lv[v]));
-            }
-            u32x4 h, l;
-            for (int p = 0; p < 4; ++p) {
-                h[p] = hv[2 * p] | (static_cast<uint32_t>(hv[2 * p + 1]) << 16);
-                l[p] = lv[2 * p] | (static_cast<uint32_t>(lv[2 * p + 1]) << 16);
-            }
-            out.frag[t * 64 + lane] = h;
-            out.frag[(kSpecMfma + t) * 64 + lane] = l;
+    out.has_lo[0] = out.has_lo[1] = 0;
+    double sum[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double sabs[3] = {0.0, 0.0, 0.0};
+    double maxabs = 0.0;
+    for (int k = 0; k < kSpecLambdas; ++k)
+        for (int c = 0; c < 3; ++c) {
+            sabs[c] += std::fabs(static_cast<double>(cmf[3 * k + c]));
+            maxabs = std::max(maxabs, std::fabs(static_cast<double>(cmf[3 * k + c])));
         }
-    for (int c = 0; c < 3; ++c) out.sum[c] = static_cast<float>(sum[c]);
+    out.f16_range = maxabs <= 16384.0;
+    out.f16_bound = true;
+    for (int c = 0; c < 3; ++c)
+        if (std::ldexp(sabs[c], -12) / kSpecLambdas > kSpecF16Budget) out.f16_bound = false;
+    for (int f = 0; f < 2; ++f)
+        for (int t = 0; t < kSpecMfma; ++t)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int m = lane & 15, q = lane >> 4;
+                uint16_t hv[8], lv[8];
+                for (int v = 0; v < 8; ++v) {
+                    const int s = 8 * t + v;
+                    const int k = q + 4 * s;
+                    const bool used = s < kSpecSlots && k < kSpecLambdas && m < 3;
+                    const float x = used ? cmf[3 * k + m] : 0.f;
+                    double hx, lx;
+                    if (f == 0) {
+                        hv[v] = bf16_rne(x);
+                        lv[v] = bf16_rne(x - bf16_value(hv[v]));
+                        hx = bf16_value(hv[v]);
+                        lx = bf16_value(lv[v]);
+                    } else {
+                        hv[v] = out.f16_range ? f16_rne(x) : 0;
+                        lv[v] = out.f16_range ? f16_rne(x - f16_value(hv[v])) : 0;
+                        hx = f16_value(hv[v]);
+                        lx = f16_value(lv[v]);
+                    }
+                    if (lv[v] & 0x7FFFu) out.has_lo[f] = 1;
+                    if (used) sum[f][m] += hx + lx;
+                }
+                u32x4 h, l;
+                for (int p = 0; p < 4; ++p) {
+                    h[p] = hv[2 * p] | (static_cast<uint32_t>(hv[2 * p + 1]) << 16);
+                    l[p] = lv[2 * p] | (static_cast<uint32_t>(lv[2 * p + 1]) << 16);
+                }
+                out.frag[f * kCmfFragsPerFormat + t * 64 + lane] = h;
+                out.frag[f * kCmfFragsPerFormat + (kSpecMfma + t) * 64 + lane] = l;
+            }
+    for (int f = 0; f < 2; ++f)
+        for (int c = 0; c < 3; ++c) out.sum[f][c] = static_cast<float>(sum[f][c]);
     for (int s = 0; s < kSpecSlots; ++s)
         for (int q = 0; q < 4; ++q) {
             const int k = q + 4 * s;
@@ -763,9 +828,23 @@ static void spectral_prepare(Drainage* d, const float* cmf) {
         SF_HIP(hipMemcpyAsync(d->cmf_dev + kCmfFrags * 4, packed.ulam, sizeof(packed.ulam), hipMemcpyHostToDevice,
                               d->stream));
         SF_HIP(hipStreamSynchronize(d->stream));
-        for (int c = 0; c < 3; ++c) d->cmf_sum[c] = packed.sum[c];
-        d->cmf_lo = packed.has_lo;
+        for (int f = 0; f < 2; ++f) {
+            for (int c = 0; c < 3; ++c) d->cmf_sum[f][c] = packed.sum[f][c];
+            d->cmf_lo[f] = packed.has_lo[f];
+        }
+        d->cmf_f16_bound = packed.f16_bound;
+        d->cmf_f16_range = packed.f16_range;
     }
+}
+
+// Operand format of the next spectral pass: 1 (FP16) or 0 (BF16 hi/lo pair).
+static int spectral_format(const Drainage* d) {
+    if (d->spec_operand == SF_SPECTRAL_OPERAND_BF16_PAIR) return 0;
+    if (d->spec_operand == SF_SPECTRAL_OPERAND_FP16) {
+        SF_REQUIRE(d->cmf_f16_range, SF_ERR_INVALID, "colour-matching curves exceed the FP16 operand range");
+        return 1;
+    }
+    return (d->cmf_f16_range && d->cmf_f16_bound) ? 1 : 0;
 }
 
 // Launches spectral_kernel on stream st over the context's current state
@@ -778,12 +857,13 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.lo = d->lo[d->cur];
     a.rows = d->spec_rows;
     a.cols = d->spec_cols;
-    a.bfrag = reinterpret_cast<const u32x4*>(d->cmf_dev);
+    const int f = spectral_format(d);
+    a.bfrag = reinterpret_cast<const u32x4*>(d->cmf_dev) + f * kCmfFragsPerFormat;
     a.ulam = d->cmf_dev + kCmfFrags * 4;
-    a.s0 = d->cmf_sum[0];
-    a.s1 = d->cmf_sum[1];
-    a.s2 = d->cmf_sum[2];
-    a.b_lo = d->cmf_lo;
+    a.s0 = d->cmf_sum[f][0];
+    a.s1 = d->cmf_sum[f][1];
+    a.s2 = d->cmf_sum[f][2];
+    a.b_lo = d->cmf_lo[f];
     a.hpole = d->hpole[d->cur_pole];
     a.pole_virtual = (d->poles_virtual && !(d->p.flags & SF_FLAG_POLES_COPY_H)) ? 1 : 0;
     a.nt = d->nt;
@@ -813,11 +893,12 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.view = d->spec_view;
     const size_t supergroups = (d->n + 63) / 64;
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 10));
+    void (*kern)(const SpecArgs) = f ? spectral_kernel<true> : spectral_kernel<false>;
     if (d->spec_timer.enabled) {
         auto& ev = d->spec_timer.next(st);
-        hipExtLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, st, ev.first, ev.second, 0, a);
+        hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, st, ev.first, ev.second, 0, a);
     } else {
-        hipLaunchKernelGGL(spectral_kernel, dim3(blocks), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, st, a);
     }
     SF_HIP(hipGetLastError());
 }

@@ -84,6 +84,9 @@ def test_argument_validation():
     assert L.lib().sf_drainage_create(C.byref(bad), 16, 32, 5e-7, 0.5, C.byref(h)) == L.SF_ERR_INVALID
     assert L.lib().sf_drainage_step(None, 1, 0.1) == L.SF_ERR_INVALID
     assert L.lib().sf_lut_rgba8(1.33, 1.0, None) == L.SF_ERR_INVALID
+    assert L.lib().sf_spectral_set_operand(None, L.SPECTRAL_OPERAND_FP16) == L.SF_ERR_INVALID
+    k = C.c_int32()
+    assert L.lib().sf_spectral_get_operand(None, C.byref(k)) == L.SF_ERR_INVALID
 
 
 def test_cpp_api_compiles():

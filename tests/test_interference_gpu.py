@@ -206,6 +206,57 @@ def test_spectral_curves_not_exact_in_bf16(oracle):
     assert err <= 1e-3
 
 
+@pytest.mark.parametrize("scale", [1.0, 20.0])
+def test_spectral_operand_formats(oracle, scale):
+    """Both operand formats against FP64 on the same curves.  FP16 r rounds
+    to nearest within 2^-12 on [0, 1], so its RGB error is bounded by
+    2^-12 sum_k |cmf[k][c]| / 81 (plus FP32 rounding); the automatic choice
+    takes FP16 while that bound is at most 2.5e-4 and the BF16 hi/lo pair
+    past it (here: the curves scaled by 20)."""
+    nt, npf = 96, 192
+    ctx, h0, p, cam, cmf = _spectral_case(nt, npf, 14)
+    cm = (cmf * scale).astype(np.float32)
+    want = oracle.spectral_rgb(h0, p.bubble_radius, cam, 1.33, cm.astype(np.float64)).reshape(3, -1)
+    bound16 = 2.0 ** -12 * np.abs(cm.astype(np.float64)).sum(axis=0) / 81.0
+    auto = L.SPECTRAL_OPERAND_FP16 if bound16.max() <= 2.5e-4 else L.SPECTRAL_OPERAND_BF16_PAIR
+    assert auto == (L.SPECTRAL_OPERAND_FP16 if scale == 1.0 else L.SPECTRAL_OPERAND_BF16_PAIR)
+    lib = L.lib()
+    got = {}
+    for kind in (L.SPECTRAL_OPERAND_BF16_PAIR, L.SPECTRAL_OPERAND_FP16, L.SPECTRAL_OPERAND_AUTO):
+        L.check(lib.sf_spectral_set_operand(ctx.handle, kind))
+        got[kind] = spectral_rgb(ctx, cam, cm).reshape(3, -1)
+        used = C.c_int32(-1)
+        L.check(lib.sf_spectral_get_operand(ctx.handle, C.byref(used)))
+        assert used.value == (auto if kind == L.SPECTRAL_OPERAND_AUTO else kind)
+    assert np.array_equal(got[L.SPECTRAL_OPERAND_AUTO], got[auto])
+    err16 = np.max(np.abs(got[L.SPECTRAL_OPERAND_FP16] - want), axis=1)
+    errbf = np.max(np.abs(got[L.SPECTRAL_OPERAND_BF16_PAIR] - want), axis=1)
+    print("scale %g: FP16 max err per channel %s (bound %s), BF16 pair %s" % (scale, err16, bound16, errbf))
+    assert np.all(err16 <= bound16 + 2e-5 * scale)
+    assert np.all(errbf <= 1e-4 * scale)
+    assert np.max(errbf) <= 1e-3 and np.max(np.abs(got[auto] - want)) <= 1e-3
+    L.check(lib.sf_spectral_set_operand(ctx.handle, L.SPECTRAL_OPERAND_AUTO))
+
+
+def test_spectral_fp16_range():
+    """Curves past FP16's range: AUTO takes the BF16 pair, a forced FP16
+    format is an error at the spectral call."""
+    nt, npf = 32, 64
+    ctx, h0, p, cam, cmf = _spectral_case(nt, npf, 2)
+    cm = cmf.astype(np.float32).copy()
+    cm[40, 1] = 3.0e4
+    lib = L.lib()
+    rgb = spectral_rgb(ctx, cam, cm)
+    assert np.all(np.isfinite(rgb))
+    used = C.c_int32(-1)
+    L.check(lib.sf_spectral_get_operand(ctx.handle, C.byref(used)))
+    assert used.value == L.SPECTRAL_OPERAND_BF16_PAIR
+    L.check(lib.sf_spectral_set_operand(ctx.handle, L.SPECTRAL_OPERAND_FP16))
+    with pytest.raises(L.SoapFilmError, match="FP16"):
+        spectral_rgb(ctx, cam, cm)
+    assert lib.sf_spectral_set_operand(ctx.handle, 7) != 0
+
+
 def test_spectral_thick_film(oracle):
     """Optical paths past 5e4 nm (films of 20-60 um) take the range-reduced
     square form in every wave."""
