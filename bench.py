@@ -6,7 +6,8 @@ by the 81-wavelength spectral RGB of the new state):
   surfactant, smooth spherical-harmonic (degree <= 16, 10 %) initial film of
   200-1000 nm, CFL-limited dt from the FP64 reduction, per-step FP64 film
   volume and CFL reductions, fixed synthetic camera, three seeded Gaussian
-  colour curves contracted on BF16 MFMA.
+  colour curves contracted on 16-bit MFMA (FP16 operands; a BF16 hi/lo pair
+  for curves whose FP16 error bound exceeds 2.5e-4).
 
 Prints one JSON line.  ``value`` is cell-steps/s of the whole step with the
 fields resident in HBM; ``roofline`` is the drainage stencil kernel against
@@ -413,7 +414,7 @@ def main():
         h0 = W.smooth_bubble(n_theta, n_phi, seed=3)
         flags = L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG
         desc = ("single bubble 2048x4096 FP32 drainage, CFL dt"
-                + (" + 81-wavelength spectral RGB per step (BF16 MFMA)" if spectral else ""))
+                + (" + 81-wavelength spectral RGB per step (FP16 MFMA, FP32 accumulate)" if spectral else ""))
     if args.flags is not None:
         flags = args.flags
     p = L.Params()
