@@ -229,7 +229,8 @@ __global__ void calc_kernel(CalcArgs a) {
 // The Airy reflectance is I = F c^2 / (1 + F c^2) = 1 - r with
 // r = 1 / (1 + F c^2), c = cos(pi opd / lambda), so with S_c = sum_k cmf[k][c]
 //   sum_k cmf[k][c] I_k = S_c - sum_k cmf[k][c] r_k,
-// and the MFMA contracts r.  Where F <= kSpecFCos2 for the whole wave,
+// and the MFMA contracts r with -cmf into an accumulator that starts at S_c.
+// Where F <= kSpecFCos2 for the whole wave,
 // 1 + F c^2 = (1 + F/2) + (F/2) cos(2 pi opd / lambda) saves the square; past
 // that the cancellation in 1 + cos would cost accuracy, and the square form
 // runs (with v_fract_f32 range reduction, which also covers long paths).
@@ -288,7 +289,8 @@ struct SpecArgs {
     const float2* hpole;
     int pole_virtual, nt, ntiles;
     // Per-cell (F, cos theta_t) of the current camera and film index; the
-    // geometry is fixed while they are, so it is computed once (NULL: inline).
+    // geometry is fixed while they are, so spectral_view_kernel computes it
+    // once; spectral_kernel reads it from here.
     const float2* view;
 };
 
@@ -328,24 +330,20 @@ __device__ inline void spec_view_terms(const SpecArgs& a, int i, int j, float& F
 }
 
 // Per-cell terms of the Airy sum: coefficient of finesse F and optical path
-// difference opc (nm) of cell cc at its view angle.
-__device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, float inv_np, float& Fc, float& opc) {
-    int i, j;
-    spec_cell_ij(a, cc, inv_np, i, j);
-    float cos_t;
-    if (a.view) {
-        // Streaming load: this VALU-bound pass has bandwidth to spare, and the
-        // film state should keep the cache.
-        typedef float f2v __attribute__((ext_vector_type(2)));
-        const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(a.view) + cc);
-        Fc = v.x;
-        cos_t = v.y;
-    } else {
-        spec_view_terms(a, i, j, Fc, cos_t);
-    }
+// difference opc (nm) of cell cc at its view angle, from the view table
+// a.view (F, cos theta_t per cell) and the film state.  last_row is the
+// index of the first cell of row nt - 1.
+__device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, size_t last_row, float& Fc, float& opc) {
+    // Streaming load: this VALU-bound pass has bandwidth to spare, and the
+    // film state should keep the cache.
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(a.view) + cc);
+    Fc = v.x;
+    const float cos_t = v.y;
+    const bool top = cc < static_cast<size_t>(a.np);
     float h;
-    if (a.pole_virtual && (i == 0 || i == a.nt - 1)) {
-        const float2 m = pole_mean(a.hpole + (i == 0 ? 0 : a.ntiles), a.ntiles, static_cast<float>(a.np));
+    if (a.pole_virtual && (top || cc >= last_row)) {
+        const float2 m = pole_mean(a.hpole + (top ? 0 : a.ntiles), a.ntiles, static_cast<float>(a.np));
         h = m.x + m.y;
     } else {
         h = a.hi[cc] + a.lo[cc];
@@ -429,44 +427,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     }
     const bool b_lo = a.b_lo != 0;
     const float scale = 1.f / static_cast<float>(kSpecLambdas);
+    // The B fragments hold -cmf, so starting the accumulator at the curve's
+    // sum S gives S - sum_k cmf_k r_k directly.
     const float S = m == 0 ? a.s0 : (m == 1 ? a.s1 : a.s2);
-    const float inv_np = 1.f / static_cast<float>(a.np);
+    const size_t last_row = static_cast<size_t>(a.nt - 1) * static_cast<size_t>(a.np);
     const bool vec_out = (a.n & 3) == 0;
 
     for (size_t sg = wave_global; sg * 64 < a.n; sg += nwaves) {
         const size_t cell = sg * 64 + lane;
         float Fc, opc;
-        spec_cell_terms(a, cell < a.n ? cell : a.n - 1, inv_np, Fc, opc);
+        spec_cell_terms(a, cell < a.n ? cell : a.n - 1, last_row, Fc, opc);
         const bool fast = !__any(!(Fc <= kSpecFCos2 && opc <= kSpecOpFast));
+        // This lane's cell's (P, A, B) of the chosen form, read by the lanes
+        // of its 16-cell group.
+        const float Pc = fast ? 2.f * opc : opc;
+        const float Ac = fast ? fmaf(0.5f, Fc, 1.f) : 1.f;
+        const float Bc = fast ? 0.5f * Fc : Fc;
         v4f acc[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const float F = lane_read(Fc, 16 * g + m);
-            const float op = lane_read(opc, 16 * g + m);
-            acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
+            const float P = lane_read(Pc, 16 * g + m);
+            const float A = lane_read(Ac, 16 * g + m);
+            const float B = lane_read(Bc, 16 * g + m);
+            acc[g] = v4f{S, S, S, S};
             if (fast)
-                spec_group<true, F16>(u, bh, bl, b_lo, 2.f * op, fmaf(0.5f, F, 1.f), 0.5f * F, acc[g]);
+                spec_group<true, F16>(u, bh, bl, b_lo, P, A, B, acc[g]);
             else
-                spec_group<false, F16>(u, bh, bl, b_lo, op, 1.f, F, acc[g]);
+                spec_group<false, F16>(u, bh, bl, b_lo, P, A, B, acc[g]);
         }
         // acc[g][v] = D[4 q + v][m]: cell 64 sg + 16 g + 4 q + v, colour m.
         if (m < 3) {
             float* plane = a.rgb + static_cast<size_t>(m) * a.n;
+            if (vec_out && sg * 64 + 64 <= a.n) {
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const size_t c0 = sg * 64 + 16 * g + 4 * q;
-                const float4 o = make_float4((S - acc[g][0]) * scale, (S - acc[g][1]) * scale,
-                                             (S - acc[g][2]) * scale, (S - acc[g][3]) * scale);
-                if (vec_out && c0 + 3 < a.n) {
-                    // Streaming store: the RGB planes are not re-read here, so they
-                    // should not displace the film state from the caches.
+                for (int g = 0; g < 4; ++g) {
+                    // Streaming store: the RGB planes are not re-read here, so
+                    // they should not displace the film state from the caches.
                     typedef float f4v __attribute__((ext_vector_type(4)));
-                    __builtin_nontemporal_store(f4v{o.x, o.y, o.z, o.w}, reinterpret_cast<f4v*>(plane + c0));
-                } else {
-                    if (c0 < a.n) plane[c0] = o.x;
-                    if (c0 + 1 < a.n) plane[c0 + 1] = o.y;
-                    if (c0 + 2 < a.n) plane[c0 + 2] = o.z;
-                    if (c0 + 3 < a.n) plane[c0 + 3] = o.w;
+                    __builtin_nontemporal_store(f4v{acc[g][0] * scale, acc[g][1] * scale, acc[g][2] * scale,
+                                                    acc[g][3] * scale},
+                                                reinterpret_cast<f4v*>(plane + sg * 64 + 16 * g + 4 * q));
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const size_t c0 = sg * 64 + 16 * g + 4 * q;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (c0 + v < a.n) plane[c0 + v] = acc[g][v] * scale;
                 }
             }
         }
@@ -726,11 +734,11 @@ struct PackedCmf {
     bool f16_range;           // every |cmf| <= 2^14
 };
 
-// Packs cmf[81][3] into the B-operand layout of spectral_kernel: lane
-// 16 q + m of MFMA t holds cmf[q + 4 (8 t + v)][m] at v = 0..7 (zero past
+// Packs -cmf[81][3] into the B-operand layout of spectral_kernel: lane
+// 16 q + m of MFMA t holds -cmf[q + 4 (8 t + v)][m] at v = 0..7 (zero past
 // slot 20, wavelength 80 or colour 2), as the round-to-nearest hi part and
 // the rounded remainder (lo), once in BF16 and once in FP16.  The sums are
-// those of hi + lo, the values the MFMA multiplies.
+// those of the curves as the MFMA sees them, -(hi + lo).
 static PackedCmf pack_cmf(const float* cmf) {
     PackedCmf out;
     out.frag.assign(kCmfFrags, u32x4{0u, 0u, 0u, 0u});
@@ -756,7 +764,7 @@ static PackedCmf pack_cmf(const float* cmf) {
                     const int s = 8 * t + v;
                     const int k = q + 4 * s;
                     const bool used = s < kSpecSlots && k < kSpecLambdas && m < 3;
-                    const float x = used ? cmf[3 * k + m] : 0.f;
+                    const float x = used ? -cmf[3 * k + m] : 0.f;
                     double hx, lx;
                     if (f == 0) {
                         hv[v] = bf16_rne(x);
@@ -770,7 +778,7 @@ static PackedCmf pack_cmf(const float* cmf) {
                         lx = f16_value(lv[v]);
                     }
                     if (lv[v] & 0x7FFFu) out.has_lo[f] = 1;
-                    if (used) sum[f][m] += hx + lx;
+                    if (used) sum[f][m] -= hx + lx;
                 }
                 u32x4 h, l;
                 for (int p = 0; p < 4; ++p) {
