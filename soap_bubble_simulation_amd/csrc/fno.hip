@@ -32,8 +32,8 @@ using namespace sf::fno;
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kCin = 5;
-constexpr int kCout = 2;
+constexpr int kCin = kLiftIn;
+constexpr int kCout = kProjOut;
 constexpr int kSplitRows = 4096;          // rows per split-K chunk of the weight gradients
 
 template <class T>
@@ -527,12 +527,33 @@ struct Fno {
     // training: keep the pre-activations for the backward pass
     void forward(int B, bool training) {
         const long long R = static_cast<long long>(B) * N;
+        int row_threads = 256;
+        const RowKernel rk = row_kernel(&row_threads);
+        if (rk && !training) {
+            // Inference: the lift is folded into the layer-0 phi DFT and the
+            // layer-0 row kernel, and the projection into the last one, so
+            // neither x_0 nor x_L is written or read.
+            const int J = 2 * Mp;
+            const size_t lds = static_cast<size_t>(P * kCin + J * (kCin + 1)) * sizeof(float);
+            SF_REQUIRE(lds <= 64 * 1024 && P <= kLiftDftMaxP, SF_ERR_INTERNAL, "lift DFT tile too large");
+            hipLaunchKernelGGL(lift_dft_kernel<kCin>, dim3(static_cast<unsigned>(B * T)), dim3(256), lds, stream, in,
+                               Ephi, w + o_wl, w + o_bl, X1, P, J, C);
+            SF_HIP(hipGetLastError());
+            for (int l = 0; l < L; ++l) {
+                theta_stage(l, B);
+                const bool last = l == L - 1;
+                RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi, Ephi, last ? nullptr : xs[l + 1], nullptr,
+                           last ? nullptr : X1, last ? 0 : 1,
+                           l == 0 ? in : nullptr, w + o_wl, w + o_bl, w + o_q, w + o_qb, last ? out : nullptr};
+                hipLaunchKernelGGL(rk, dim3(static_cast<unsigned>(B * T)), dim3(row_threads), 0, stream, ra);
+                SF_HIP(hipGetLastError());
+            }
+            return;
+        }
         // lift (FP32, bias included)
         hipLaunchKernelGGL(lift_kernel<kCin>, dim3(blocks_for(R * ((C + 3) / 4), 256, 16384)), dim3(256), 0, stream, in,
                            w + o_wl, w + o_bl, xs[0], R, C);
         SF_HIP(hipGetLastError());
-        int row_threads = 256;
-        const RowKernel rk = row_kernel(&row_threads);
         if (rk) {
             const int MpC2 = 2 * Mp * C;
             gemm(Ephi, xs[0], X1, 2 * Mp, C, P, P, C, C, 0, static_cast<long long>(P) * C,

@@ -20,6 +20,11 @@ namespace fno {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Network input channels (h, surfactant and three geometry/parameter
+// features) and output channels (the two predicted increments).
+constexpr int kLiftIn = 5;
+constexpr int kProjOut = 2;
+
 __device__ inline unsigned short to_bf16(float x) {
     uint32_t u = __float_as_uint(x);
     u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (inputs are finite)
@@ -143,6 +148,10 @@ __device__ inline float gelu(float z) {      // tanh form
 //                                        the accumulators: the MFMA result
 //                                        layout D[4(l/16)+v][l%16] is the
 //                                        B-operand layout B[4(l/16)+v][l%16])
+// Inference options: with `in` set, x is not read but lifted on the fly from
+// the kLiftIn input features (x = b_l + in W_l, FP32); with `proj` set, the
+// projection proj[p][j] = q[j] + sum_o x'[p][o] Q[o][j] is formed from the
+// accumulators (FP32) and x' is stored only if x_out is set.
 // Block: WAVES waves, wave w owns rows p in [w P/WAVES, (w+1) P/WAVES).
 // RT = P / (16 WAVES) row tiles per wave, CT = C / 16 channel tiles, JT =
 // J / 16 tiles of the 2 Mp real/imaginary phi modes.
@@ -157,6 +166,12 @@ struct RowArgs {
     float* z_out;         // [rows][P][C] or null
     float* X1_out;        // [rows][J][C] or null
     int act;
+    const float* in;      // [rows][P][kLiftIn] or null: lift x from these
+    const float* Wl;      // [kLiftIn][C] lift weights
+    const float* bl;      // [C] lift bias
+    const float* Q;       // [C][kProjOut] projection weights
+    const float* qb;      // [kProjOut] projection bias
+    float* proj;          // [rows][P][kProjOut] or null
 };
 
 template <int RT, int CT, int JT, int WAVES>
@@ -164,6 +179,7 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
     constexpr int C = 16 * CT, J = 16 * JT, K = C + J, P = 16 * RT * WAVES, NT = 64 * WAVES;
     __shared__ unsigned short Bs[C][K + 4];       // [out channel][k]
     __shared__ float red[WAVES][J][C];
+    __shared__ __align__(16) float s_lift[(kLiftIn + 1) * C];   // W_l rows, then b_l
     const long long row = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int mr = lane & 15, kq = 4 * (lane >> 4);
@@ -172,6 +188,10 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
         const float v = k < C ? a.W[k * C + n] : a.Z[row * J * C + static_cast<long long>(k - C) * C + n];
         Bs[n][k] = to_bf16(v);
     }
+    const bool lift = a.in != nullptr;
+    if (lift)
+        for (int q = threadIdx.x; q < (kLiftIn + 1) * C; q += NT)
+            s_lift[q] = q < kLiftIn * C ? a.Wl[q] : a.bl[q - kLiftIn * C];
     __syncthreads();
     const float* x = a.x + row * P * C;
     f32x4 acc[RT][CT];
@@ -182,9 +202,28 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
         const int p = wave * RT * 16 + rt * 16 + mr;
+        float fin[kLiftIn];
+        if (lift) {
+            const float* ip = a.in + (row * P + p) * kLiftIn;
+#pragma unroll
+            for (int i = 0; i < kLiftIn; ++i) fin[i] = ip[i];
+        }
 #pragma unroll
         for (int ks = 0; ks < CT; ++ks) {
-            const float4 v = *reinterpret_cast<const float4*>(x + static_cast<long long>(p) * C + ks * 16 + kq);
+            float4 v;
+            if (lift) {
+                v = *reinterpret_cast<const float4*>(&s_lift[kLiftIn * C + ks * 16 + kq]);
+#pragma unroll
+                for (int i = 0; i < kLiftIn; ++i) {
+                    const float4 wv = *reinterpret_cast<const float4*>(&s_lift[i * C + ks * 16 + kq]);
+                    v.x = fmaf(fin[i], wv.x, v.x);
+                    v.y = fmaf(fin[i], wv.y, v.y);
+                    v.z = fmaf(fin[i], wv.z, v.z);
+                    v.w = fmaf(fin[i], wv.w, v.w);
+                }
+            } else {
+                v = *reinterpret_cast<const float4*>(x + static_cast<long long>(p) * C + ks * 16 + kq);
+            }
             const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
                                     static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
 #pragma unroll
@@ -206,7 +245,7 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
         }
     }
     // epilogue: bias, activation, stores
-    float* xo = a.x_out + row * P * C;
+    float* xo = a.x_out ? a.x_out + row * P * C : nullptr;
     float* zo = a.z_out ? a.z_out + row * P * C : nullptr;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -219,11 +258,43 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
                 const int p = wave * RT * 16 + rt * 16 + kq + v;
                 const float z = acc[rt][ct][v] + bo;
                 const float xv = a.act ? gelu(z) : z;
-                xo[static_cast<long long>(p) * C + o] = xv;
+                if (xo) xo[static_cast<long long>(p) * C + o] = xv;
                 if (zo) zo[static_cast<long long>(p) * C + o] = z;
                 acc[rt][ct][v] = xv;
             }
         }
+    if (a.proj) {
+        // Channel o = 16 ct + mr of rows p = base + kq + v sits in lane
+        // (mr, kq): per-lane partial sums over ct, then a sum over the 16
+        // lanes of the kq group; lane mr = 2 v + j stores output j of row v.
+        float qv[CT][kProjOut];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int j = 0; j < kProjOut; ++j) qv[ct][j] = a.Q[(ct * 16 + mr) * kProjOut + j];
+        float* po = a.proj + row * P * kProjOut;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            float mine = 0.f;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int j = 0; j < kProjOut; ++j) {
+                    float s = 0.f;
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) s = fmaf(acc[rt][ct][v], qv[ct][j], s);
+                    s += __shfl_xor(s, 8, 16);
+                    s += __shfl_xor(s, 4, 16);
+                    s += __shfl_xor(s, 2, 16);
+                    s += __shfl_xor(s, 1, 16);
+                    if (mr == kProjOut * v + j) mine = s + a.qb[j];
+                }
+            if (mr < 4 * kProjOut) {
+                const int p = wave * RT * 16 + rt * 16 + kq + mr / kProjOut;
+                po[static_cast<long long>(p) * kProjOut + mr % kProjOut] = mine;
+            }
+        }
+    }
     if (a.X1_out == nullptr) return;
     // phi DFT of the new activations: X1'[j][o] = sum_p Ephi[j][p] x'[p][o]
     f32x4 acc2[JT][CT];
@@ -327,6 +398,66 @@ __global__ __launch_bounds__(256) void lift_kernel(const float* __restrict__ in,
 #pragma unroll
         for (int i = 0; i < NIN; ++i) v = fmaf(in[r * NIN + i], W[i * C + c], v);
         x[k] = v;
+    }
+}
+
+// Layer-0 phi DFT straight from the input features: with the lift
+// x = in W_l + 1 b_l, X1 = Ephi x = (Ephi in) W_l + (Ephi 1) b_l per
+// (bubble, theta row), in FP32.  One block per row, P <= kLiftDftMaxP.
+// Lane l holds the inputs of columns p = l + 64 k; wave w forms the products
+// of modes j = w, w + waves, ... from one coalesced read of the Ephi row (all
+// loads of a row issued before use) and a butterfly over the wave.
+constexpr int kLiftDftMaxP = 512;
+
+template <int NIN>
+__global__ __launch_bounds__(256) void lift_dft_kernel(const float* __restrict__ in, const float* __restrict__ Ephi,
+                                                       const float* __restrict__ Wl, const float* __restrict__ bl,
+                                                       float* __restrict__ X1, int P, int J, int C) {
+    constexpr int KP = kLiftDftMaxP / 64;
+    extern __shared__ float lds[];
+    float* s_in = lds;                  // [P][NIN]
+    float* s_e = lds + P * NIN;         // [J][NIN + 1]
+    const long long row = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int t = threadIdx.x; t < P * NIN; t += blockDim.x) s_in[t] = in[row * P * NIN + t];
+    __syncthreads();
+    float fin[KP][NIN];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) fin[k][i] = p < P ? s_in[p * NIN + i] : 0.f;
+    }
+    for (int j = wave; j < J; j += nw) {
+        const float* e = Ephi + static_cast<long long>(j) * P;
+        float ev[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) ev[k] = lane + 64 * k < P ? e[lane + 64 * k] : 0.f;
+        float acc[NIN + 1];
+#pragma unroll
+        for (int i = 0; i <= NIN; ++i) acc[i] = 0.f;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+#pragma unroll
+            for (int i = 0; i < NIN; ++i) acc[i] = fmaf(ev[k], fin[k][i], acc[i]);
+            acc[NIN] += ev[k];
+        }
+#pragma unroll
+        for (int i = 0; i <= NIN; ++i)
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) acc[i] += __shfl_xor(acc[i], m, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int i = 0; i <= NIN; ++i) s_e[j * (NIN + 1) + i] = acc[i];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < J * C; t += blockDim.x) {
+        const int j = t / C, c = t - j * C;
+        const float* ej = s_e + j * (NIN + 1);
+        float s = ej[NIN] * bl[c];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) s = fmaf(ej[i], Wl[i * C + c], s);
+        X1[row * J * C + t] = s;
     }
 }
 
