@@ -276,6 +276,8 @@ struct Fno {
     float* Eth = nullptr;      // [2 Mt][T]
     float* Ath = nullptr;      // [2 T][2 Mt]
     float* Iphi = nullptr;     // [P][2 Mp]
+    unsigned short* Ephi16 = nullptr;   // Ephi and Iphi rounded to BF16 (the
+    unsigned short* Iphi16 = nullptr;   // fused layer kernel's MFMA operands)
 
     // activations (max batch Bm)
     float* in = nullptr;       // [Bm][N][kCin]
@@ -313,7 +315,7 @@ struct Fno {
 
     ~Fno() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, in, out, target, dout, ybuf, X1, G, Y, Z,
+        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, in, out, target, dout, ybuf, X1, G, Y, Z,
                       dxa, dxb, dz, dXs, part, loss_part, dscratch, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
                       sel_hi, sel_lo, hp_hi, hp_lo, d_win, d_take, rpart};
         for (void* p : ps) (void)hipFree(p);
@@ -407,6 +409,22 @@ struct Fno {
         SF_HIP(hipMemcpy(Eth, eth.data(), eth.size() * 4, hipMemcpyHostToDevice));
         SF_HIP(hipMemcpy(Ath, ath.data(), ath.size() * 4, hipMemcpyHostToDevice));
         SF_HIP(hipMemcpy(Iphi, iphi.data(), iphi.size() * 4, hipMemcpyHostToDevice));
+        // BF16 copies, rounded to nearest even as the kernels round floats.
+        auto bf16 = [](const std::vector<float>& v) {
+            std::vector<unsigned short> o(v.size());
+            for (size_t k = 0; k < v.size(); ++k) {
+                uint32_t u;
+                std::memcpy(&u, &v[k], 4);
+                u += 0x7FFFu + ((u >> 16) & 1u);
+                o[k] = static_cast<unsigned short>(u >> 16);
+            }
+            return o;
+        };
+        const std::vector<unsigned short> e16 = bf16(ephi), i16 = bf16(iphi);
+        Ephi16 = galloc<unsigned short>(e16.size());
+        Iphi16 = galloc<unsigned short>(i16.size());
+        SF_HIP(hipMemcpy(Ephi16, e16.data(), e16.size() * 2, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(Iphi16, i16.data(), i16.size() * 2, hipMemcpyHostToDevice));
     }
 
     void init_params(uint64_t seed) {
@@ -542,9 +560,11 @@ struct Fno {
             for (int l = 0; l < L; ++l) {
                 theta_stage(l, B);
                 const bool last = l == L - 1;
-                RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi, Ephi, last ? nullptr : xs[l + 1], nullptr,
+                // Activations between layers in BF16 (the precision the next
+                // layer's MFMA reads them at).
+                RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi16, Ephi16, last ? nullptr : xs[l + 1], nullptr,
                            last ? nullptr : X1, last ? 0 : 1,
-                           l == 0 ? in : nullptr, w + o_wl, w + o_bl, w + o_q, w + o_qb, last ? out : nullptr};
+                           l == 0 ? in : nullptr, w + o_wl, w + o_bl, w + o_q, w + o_qb, last ? out : nullptr, 1};
                 hipLaunchKernelGGL(rk, dim3(static_cast<unsigned>(B * T)), dim3(row_threads), 0, stream, ra);
                 SF_HIP(hipGetLastError());
             }
@@ -560,7 +580,7 @@ struct Fno {
                  static_cast<long long>(MpC2), B * T, 0, 0, 1.f, 0.f);
             for (int l = 0; l < L; ++l) {
                 theta_stage(l, B);
-                RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi, Ephi, xs[l + 1], training ? zs[l] : nullptr,
+                RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi16, Ephi16, xs[l + 1], training ? zs[l] : nullptr,
                            l < L - 1 ? X1 : nullptr, l < L - 1 ? 1 : 0};
                 hipLaunchKernelGGL(rk, dim3(static_cast<unsigned>(B * T)), dim3(row_threads), 0, stream, ra);
                 SF_HIP(hipGetLastError());

@@ -148,7 +148,9 @@ __device__ inline float gelu(float z) {      // tanh form
 //                                        the accumulators: the MFMA result
 //                                        layout D[4(l/16)+v][l%16] is the
 //                                        B-operand layout B[4(l/16)+v][l%16])
-// Inference options: with `in` set, x is not read but lifted on the fly from
+// Inference options: with x16 set, x and x_out hold BF16 activations (the
+// values the MFMA reads; a float is rounded the same way on load, so this
+// changes no result); with `in` set, x is not read but lifted on the fly from
 // the kLiftIn input features (x = b_l + in W_l, FP32); with `proj` set, the
 // projection proj[p][j] = q[j] + sum_o x'[p][o] Q[o][j] is formed from the
 // accumulators (FP32) and x' is stored only if x_out is set.
@@ -160,8 +162,8 @@ struct RowArgs {
     const float* Z;       // [rows][J][C]
     const float* W;       // [C][C]
     const float* bias;    // [C]
-    const float* Iphi;    // [P][J]
-    const float* Ephi;    // [J][P]
+    const unsigned short* Iphi;   // [P][J] BF16
+    const unsigned short* Ephi;   // [J][P] BF16
     float* x_out;         // [rows][P][C]
     float* z_out;         // [rows][P][C] or null
     float* X1_out;        // [rows][J][C] or null
@@ -172,6 +174,7 @@ struct RowArgs {
     const float* Q;       // [C][kProjOut] projection weights
     const float* qb;      // [kProjOut] projection bias
     float* proj;          // [rows][P][kProjOut] or null
+    int x16;              // x and x_out hold BF16 values (inference)
 };
 
 template <int RT, int CT, int JT, int WAVES>
@@ -210,22 +213,28 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
         }
 #pragma unroll
         for (int ks = 0; ks < CT; ++ks) {
-            float4 v;
-            if (lift) {
-                v = *reinterpret_cast<const float4*>(&s_lift[kLiftIn * C + ks * 16 + kq]);
-#pragma unroll
-                for (int i = 0; i < kLiftIn; ++i) {
-                    const float4 wv = *reinterpret_cast<const float4*>(&s_lift[i * C + ks * 16 + kq]);
-                    v.x = fmaf(fin[i], wv.x, v.x);
-                    v.y = fmaf(fin[i], wv.y, v.y);
-                    v.z = fmaf(fin[i], wv.z, v.z);
-                    v.w = fmaf(fin[i], wv.w, v.w);
-                }
+            bf16x4 A;
+            if (!lift && a.x16) {
+                A = *reinterpret_cast<const bf16x4*>(reinterpret_cast<const unsigned short*>(a.x) + row * P * C +
+                                                     static_cast<long long>(p) * C + ks * 16 + kq);
             } else {
-                v = *reinterpret_cast<const float4*>(x + static_cast<long long>(p) * C + ks * 16 + kq);
+                float4 v;
+                if (lift) {
+                    v = *reinterpret_cast<const float4*>(&s_lift[kLiftIn * C + ks * 16 + kq]);
+#pragma unroll
+                    for (int i = 0; i < kLiftIn; ++i) {
+                        const float4 wv = *reinterpret_cast<const float4*>(&s_lift[i * C + ks * 16 + kq]);
+                        v.x = fmaf(fin[i], wv.x, v.x);
+                        v.y = fmaf(fin[i], wv.y, v.y);
+                        v.z = fmaf(fin[i], wv.z, v.z);
+                        v.w = fmaf(fin[i], wv.w, v.w);
+                    }
+                } else {
+                    v = *reinterpret_cast<const float4*>(x + static_cast<long long>(p) * C + ks * 16 + kq);
+                }
+                A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
+                           static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
             }
-            const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
-                                    static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
                 const bf16x4 B = *reinterpret_cast<const bf16x4*>(&Bs[ct * 16 + mr][ks * 16 + kq]);
@@ -234,9 +243,7 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
         }
 #pragma unroll
         for (int js = 0; js < JT; ++js) {
-            const float4 v = *reinterpret_cast<const float4*>(a.Iphi + static_cast<long long>(p) * J + js * 16 + kq);
-            const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
-                                    static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
+            const bf16x4 A = *reinterpret_cast<const bf16x4*>(a.Iphi + static_cast<long long>(p) * J + js * 16 + kq);
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
                 const bf16x4 B = *reinterpret_cast<const bf16x4*>(&Bs[ct * 16 + mr][C + js * 16 + kq]);
@@ -245,7 +252,8 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
         }
     }
     // epilogue: bias, activation, stores
-    float* xo = a.x_out ? a.x_out + row * P * C : nullptr;
+    float* xo = (a.x_out && !a.x16) ? a.x_out + row * P * C : nullptr;
+    unsigned short* xo16 = (a.x_out && a.x16) ? reinterpret_cast<unsigned short*>(a.x_out) + row * P * C : nullptr;
     float* zo = a.z_out ? a.z_out + row * P * C : nullptr;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -259,6 +267,7 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
                 const float z = acc[rt][ct][v] + bo;
                 const float xv = a.act ? gelu(z) : z;
                 if (xo) xo[static_cast<long long>(p) * C + o] = xv;
+                if (xo16) xo16[static_cast<long long>(p) * C + o] = to_bf16(xv);
                 if (zo) zo[static_cast<long long>(p) * C + o] = z;
                 acc[rt][ct][v] = xv;
             }
@@ -312,9 +321,7 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
                             static_cast<short>(to_bf16(acc[rt][ct][2])), static_cast<short>(to_bf16(acc[rt][ct][3]))};
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
-            const float4 v = *reinterpret_cast<const float4*>(a.Ephi + static_cast<long long>(jt * 16 + mr) * P + p0 + kq);
-            const bf16x4 A = bf16x4{static_cast<short>(to_bf16(v.x)), static_cast<short>(to_bf16(v.y)),
-                                    static_cast<short>(to_bf16(v.z)), static_cast<short>(to_bf16(v.w))};
+            const bf16x4 A = *reinterpret_cast<const bf16x4*>(a.Ephi + static_cast<long long>(jt * 16 + mr) * P + p0 + kq);
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct)
                 acc2[jt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bx[ct], acc2[jt][ct], 0, 0, 0);
