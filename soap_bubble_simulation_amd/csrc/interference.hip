@@ -900,7 +900,10 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     }
     a.view = d->spec_view;
     const size_t supergroups = (d->n + 63) / 64;
-    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((supergroups + 3) / 4, 256 * 10));
+    // About two 64-cell groups per wave: the short blocks balance the tail
+    // (145.8 us at 2048 x 4096, against 154.5 us with 2560 grid-striding
+    // blocks and 149.5 us with one group per wave).
+    const unsigned blocks = static_cast<unsigned>(std::max<size_t>(1, (supergroups + 7) / 8));
     void (*kern)(const SpecArgs) = f ? spectral_kernel<true> : spectral_kernel<false>;
     if (d->spec_timer.enabled) {
         auto& ev = d->spec_timer.next(st);
