@@ -193,6 +193,20 @@ def test_spectral_small_grid(oracle):
     assert np.max(np.abs(want)) > 0.05
 
 
+@pytest.mark.parametrize("nt,npf", [(7, 13), (5, 30), (9, 66)])
+def test_spectral_ragged_grids(oracle, nt, npf):
+    """Cell counts that are not multiples of 4 or 64 (91, 150, 594 cells):
+    partial 64-cell groups and the scalar RGB stores."""
+    ctx, h0, p, cam, cmf = _spectral_case(nt, npf, 31)
+    cm = cmf.astype(np.float32)
+    got = spectral_rgb(ctx, cam, cm)
+    want = oracle.spectral_rgb(h0, p.bubble_radius, cam, 1.33, cm.astype(np.float64))
+    assert got.shape == want.shape == (3, nt, npf)
+    err = np.max(np.abs(got - want))
+    print("spectral %dx%d max |rgb - oracle| = %.3g" % (nt, npf, err))
+    assert err <= 1e-3
+
+
 def test_spectral_curves_not_exact_in_bf16(oracle):
     """Curves with a nonzero BF16 remainder take the third (hi x curve-lo) MFMA."""
     nt, npf = 64, 128
