@@ -7,7 +7,8 @@
 // merge_bubbles, foam.rs:365-399).
 //
 //   cluster_pair_kernel       all-pairs forces: lane = bubble i of a 64-bubble
-//                             tile, wave = one contiguous run of partners j;
+//                             tile, wave = one contiguous run of partners j
+//                             (read with scalar loads, wave-uniform);
 //                             FP32 pair force as the reference computes it,
 //                             compensated FP32 sums, one partial per
 //                             (block row, i)
@@ -24,7 +25,7 @@
 // thickness_nm), FP32 like the reference's glam Vec3/f32 fields.  Ids, aspect
 // ratio and refractive index stay on the host (ids are uploaded for the
 // connection records).  Pair work is N^2: at N = 4096 one step is 16.8 M pair
-// evaluations spread over 1024 blocks.
+// evaluations spread over 1024 blocks of 16 waves (runs of 16 partners).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -47,14 +48,16 @@ constexpr float kMaxSpeed = 0.1f;       // foam_dynamics.rs:182
 constexpr float kAdhesion = 2.0f;       // foam_dynamics.rs:147
 constexpr float kContactGap = 0.3f;     // foam.rs:342
 constexpr int kTile = 64;               // bubbles i per block (one per lane)
-constexpr int kRunsPerBlock = 4;        // partner runs per block (one per wave)
+constexpr int kRunsPerBlock = 4;        // partner runs per contact/connect block (one per wave)
+constexpr int kPairRuns = 16;           // partner runs per pair-kernel block (one per wave)
 
 struct Consts {
     float gdir[3];          // gravity.normalize_or_zero()
     float gmag;             // gravity.length()
     float density, vdw, k_rep, damping, drag_c;   // drag_c = 6 pi eta
     float surface_tension, coal_thr, dt;
-    int n, run, nruns;      // bubbles, partners per run, runs
+    int n, run, nruns;      // bubbles, partners per run, runs (contact kernels)
+    int prun, npruns;       // partners per run, runs (force kernel)
 };
 
 // compute_interaction_force (foam_dynamics.rs:116-154), force on a from the
@@ -102,9 +105,9 @@ __device__ inline void pair_force_add(float4 a, float4 b, const Consts& c, float
 }
 
 // Partner positions of a run are loaded once per 64, one per lane, and
-// broadcast with v_readlane: the inner loop reads no memory.  Every lane of
-// a wave takes part (lanes past the last bubble load a clamped index and
-// write nothing), so each broadcast reads a loaded value.
+// broadcast with v_readlane (the contact kernels).  Every lane of a wave
+// takes part (lanes past the last bubble load a clamped index and write
+// nothing), so each broadcast reads a loaded value.
 __device__ inline float4 lane_bcast(const float4& v, int t) {
     return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), t)),
                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), t)),
@@ -112,26 +115,23 @@ __device__ inline float4 lane_bcast(const float4& v, int t) {
                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), t)));
 }
 
-// One partial per (block row, bubble): the block's four runs are folded in
-// run order through LDS.
-__global__ __launch_bounds__(256) void cluster_pair_kernel(const float4* __restrict__ pr, Consts c,
-                                                           float* __restrict__ part) {
-    __shared__ float s_f[kRunsPerBlock][3][kTile];
+// One partial per (block row, bubble): the block's kPairRuns runs are folded
+// in run order through LDS.  The partner index is wave-uniform, so partner
+// records come through scalar loads and cost the vector ALU nothing.
+__global__ __launch_bounds__(64 * kPairRuns) void cluster_pair_kernel(const float4* __restrict__ pr, Consts c,
+                                                                     float* __restrict__ part) {
+    __shared__ float s_f[kPairRuns][3][kTile];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int run = blockIdx.y * kRunsPerBlock + w;
+    const int run = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.y) * kPairRuns + w);
     const int i = blockIdx.x * kTile + lane;
     // Within a run only partners closer than 3 (r_a + r_b) add a term, a
     // handful at cluster densities; plain FP32 sums, folded compensated.
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (run < c.nruns) {                 // whole wave
+    if (run < c.npruns) {                // whole wave
         const float4 a = pr[i < c.n ? i : c.n - 1];
-        const int j0 = run * c.run, j1 = min(c.n, j0 + c.run);
-        for (int jc = j0; jc < j1; jc += 64) {
-            const float4 mine = pr[min(jc + lane, c.n - 1)];
-            const int cnt = min(64, j1 - jc);
-            for (int t = 0; t < cnt; ++t) pair_force_add(a, lane_bcast(mine, t), c, fx, fy, fz);
-        }
+        const int j0 = run * c.prun, j1 = min(c.n, j0 + c.prun);
+        for (int j = j0; j < j1; ++j) pair_force_add(a, pr[j], c, fx, fy, fz);
     }
     s_f[w][0][lane] = fx;
     s_f[w][1][lane] = fy;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void cluster_pair_kernel(const float4* __restr
         float* dst = part + (static_cast<size_t>(blockIdx.y) * c.n + i) * 3;
         for (int d = 0; d < 3; ++d) {
             KSum k;
-            for (int r = 0; r < kRunsPerBlock; ++r) k.add(s_f[r][d][lane]);
+            for (int r = 0; r < kPairRuns; ++r) k.add(s_f[r][d][lane]);
             dst[d] = k.value();
         }
     }
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void cluster_integrate_kernel(const float4* __
     fx.add(c.gdir[0] * bm);
     fy.add(c.gdir[1] * bm);
     fz.add(c.gdir[2] * bm);
-    const int rows = (c.nruns + kRunsPerBlock - 1) / kRunsPerBlock;
+    const int rows = (c.npruns + kPairRuns - 1) / kPairRuns;
     for (int r = 0; r < rows; ++r) {
         const float* q = part + (static_cast<size_t>(r) * c.n + i) * 3;
         fx.add(q[0]); fy.add(q[1]); fz.add(q[2]);
@@ -438,6 +438,10 @@ struct Cluster {
         c.n = n;
         c.run = std::max(kTile, (n + 63) / 64);
         c.nruns = n > 0 ? (n + c.run - 1) / c.run : 0;
+        // Short force runs: enough waves to hide latency (16 per CU at
+        // n = 4096), folded 16 to a block.
+        c.prun = std::max(16, (n + 255) / 256);
+        c.npruns = n > 0 ? (n + c.prun - 1) / c.prun : 0;
         return c;
     }
 
@@ -466,7 +470,7 @@ struct Cluster {
     }
 
     void ensure_scratch(const Consts& c) {
-        const size_t np = static_cast<size_t>(c.nruns) * n * 3;
+        const size_t np = static_cast<size_t>((c.npruns + kPairRuns - 1) / kPairRuns) * n * 3;
         if (np > part_cap) {
             dfree(part);
             part = dnew<float>(np);
@@ -487,6 +491,10 @@ struct Cluster {
         return dim3((n + kTile - 1) / kTile, (c.nruns + kRunsPerBlock - 1) / kRunsPerBlock);
     }
 
+    dim3 force_grid(const Consts& c) const {
+        return dim3((n + kTile - 1) / kTile, (c.npruns + kPairRuns - 1) / kPairRuns);
+    }
+
     // compute_forces into force_out (device, [n][3]) and, when out state is
     // given, the integrated state into the other buffers.
     void forces_and_integrate(float dt, float* force_dev, bool integrate) {
@@ -495,10 +503,10 @@ struct Cluster {
         ensure_scratch(c);
         if (pair_timer.enabled) {
             auto& ev = pair_timer.next(stream);
-            hipExtLaunchKernelGGL(cluster_pair_kernel, pair_grid(c), dim3(256), 0, stream, ev.first, ev.second, 0,
-                                  pr[cur], c, part);
+            hipExtLaunchKernelGGL(cluster_pair_kernel, force_grid(c), dim3(64 * kPairRuns), 0, stream, ev.first,
+                                  ev.second, 0, pr[cur], c, part);
         } else {
-            hipLaunchKernelGGL(cluster_pair_kernel, pair_grid(c), dim3(256), 0, stream, pr[cur], c, part);
+            hipLaunchKernelGGL(cluster_pair_kernel, force_grid(c), dim3(64 * kPairRuns), 0, stream, pr[cur], c, part);
         }
         SF_HIP(hipGetLastError());
         hipLaunchKernelGGL(cluster_integrate_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pr[cur], vt[cur],
