@@ -474,8 +474,9 @@ def main():
     traffic, pmc = pmc_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                # the float4 kernel the context selects: 16-row bands at 2048 x 4096, 8-row blocks at 128 x 256
-                "kernel": "drainage_band_kernel" if n_theta >= 1024 else "drainage_vec4_kernel",
+                # the kernel the context selects: 16-row float4 bands at 2048 x 4096, the scalar kernel
+                # with one row per block at 128 x 256 (grids of up to 262144 cells)
+                "kernel": "drainage_band_kernel" if n_theta >= 1024 else "drainage_step_kernel",
                 "avg_launch_ms": drain_ms,
                 "launches_timed": drain_n,
                 "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells}
@@ -507,10 +508,26 @@ def main():
         "state": {"min_h": stats.min_thickness, "max_h": stats.max_thickness,
                   "mass_m3": stats.mass, "time_s": stats.current_time, "finite": finite},
     }
+    if args.workload == "c1":
+        # The C1 configuration also builds the 2-D interference LUT: 1024
+        # thicknesses (0-1000 nm) x 90 angles at 650/532/450 nm, host-timed
+        # including the copy back.
+        lam = (C.c_double * 3)(650.0, 532.0, 450.0)
+        lut = np.empty((1024, 90, 3), np.float32)
+        L.check(lib.sf_lut_2d(1.33, lam, 0.0, 1000.0, 1024, 90, 1, L.fptr(lut)))
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L.check(lib.sf_lut_2d(1.33, lam, 0.0, 1000.0, 1024, 90, 1, L.fptr(lut)))
+        out["kernels"]["lut_2d_1024x90_ms"] = (time.perf_counter() - t0) * 1e3 / reps
     if not args.no_cpu_baseline:
         import oracle as O
         out["cpu_baseline"] = cpu_baseline(O, p, h0, n_theta, n_phi, dt, cam, cmf, spectral,
                                            args.cpu_seconds)
+        if args.workload == "c1":
+            t0 = time.perf_counter()
+            O.lut_2d(1.33, 0.0, 1000.0, 1024, 90)
+            out["cpu_baseline"]["lut_2d_1024x90_ms"] = (time.perf_counter() - t0) * 1e3
     print(json.dumps(out))
 
 

@@ -237,14 +237,26 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     dphi = 2.0 * kPi / static_cast<double>(np);
     SF_HIP(hipGetDevice(&device));
     SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    vec4 = (np % 4 == 0) && std::getenv("SOAPFILM_SCALAR_STENCIL") == nullptr;
+    // Kernel choice.  Grids of up to kSmallGrid cells are latency-bound: the
+    // scalar kernel with one row per block gives the most waves (6.5 us per
+    // step at 128 x 256 against 30 us for 8-row float4 blocks).  Larger grids
+    // take the float4 kernels: 16-row bands (the band-streaming kernel) when
+    // that still gives two blocks per CU, else 4-row blocks.  SOAPFILM_STENCIL
+    // (scalar | vec4) and SOAPFILM_ROWS_PER_BLOCK override the choice (tests).
+    constexpr size_t kSmallGrid = 262144;
+    const char* force = std::getenv("SOAPFILM_STENCIL");
+    const bool small = n <= kSmallGrid;
+    if (force && std::strcmp(force, "scalar") == 0)
+        vec4 = false;
+    else if (force && std::strcmp(force, "vec4") == 0)
+        vec4 = (np % 4 == 0);
+    else
+        vec4 = (np % 4 == 0) && !small;
     const int cols_per_wave = vec4 ? 256 : 64;
     wx = static_cast<int>(std::min<size_t>(4, cdiv(np, cols_per_wave)));
     block = 64 * wx;
     ntiles = static_cast<int>(cdiv(np, static_cast<size_t>(cols_per_wave) * wx));
-    // float4 path: 16-row bands (the band-streaming kernel) when that still
-    // gives two blocks per CU, else the unrolled 8-row kernel.
-    rows_per_block = vec4 ? (static_cast<size_t>(ntiles) * cdiv(nt - 2, 16) >= 512 ? 16 : 8) : 16;
+    rows_per_block = vec4 ? (static_cast<size_t>(ntiles) * cdiv(nt - 2, 16) >= 512 ? 16 : 4) : (small ? 1 : 16);
     if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK")) rows_per_block = std::max(1, std::atoi(e));
     if (vec4) rows_per_block = rows_per_block <= 4 ? 4 : (rows_per_block <= 8 ? 8 : std::min(rows_per_block, kBandMax));
     for (int k = 0; k < 2; ++k) {

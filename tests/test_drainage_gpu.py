@@ -68,8 +68,24 @@ def test_single_step_parity_default_grid(oracle):
 @pytest.mark.parametrize("nt,npf", [(64, 128), (40, 96), (37, 74), (9, 6), (3, 4), (20, 1030),
                                      (130, 2052)])
 def test_grid_shapes_parity(oracle, nt, npf):
-    """Scalar (n_phi % 4 != 0) and float4 kernels, partial waves and tiles,
-    tiny grids with both rings in one row band."""
+    """Scalar (n_phi % 4 != 0 and small grids) and float4 kernels, partial
+    waves and tiles, tiny grids with both rings in one row band."""
+    _shape_parity(oracle, nt, npf)
+
+
+@pytest.mark.parametrize("stencil,rows", [("scalar", 1), ("scalar", 3), ("scalar", 16), ("vec4", 4), ("vec4", 8),
+                                          ("vec4", 16)])
+@pytest.mark.parametrize("nt,npf", [(40, 96), (130, 2052)])
+def test_stencil_kernels_parity(oracle, monkeypatch, stencil, rows, nt, npf):
+    """Every stencil kernel (scalar with 1-16 rows per block, float4 with 4-
+    and 8-row blocks, 16-row bands), forced through the selection overrides,
+    on a small and a multi-tile grid."""
+    monkeypatch.setenv("SOAPFILM_STENCIL", stencil)
+    monkeypatch.setenv("SOAPFILM_ROWS_PER_BLOCK", str(rows))
+    _shape_parity(oracle, nt, npf)
+
+
+def _shape_parity(oracle, nt, npf):
     h0 = W.noise_field(nt, npf, seed=nt * 1000 + npf, rel=0.05)
     p = cpu_params(flags=L.FLAG_SURFACTANT | L.FLAG_DIAG)
     g0 = W.ff_exact(np.clip(0.5 + 0.05 * np.random.default_rng(1).standard_normal((nt, npf)), 0, 1))
