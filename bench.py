@@ -292,12 +292,14 @@ def bench_surrogate(args, L, W, n_bubbles):
     model = FnoSurrogate(n_theta=T, n_phi=P, max_batch=16)
     K = model.config.window
 
-    def make(nb, seed):
+    def make(nb, seed, shapes=None):
         d = W.foam_batch(nb, seed=seed, lat=(T, T))
         bubbles = [FoamBubble(**{**x, "n_phi": P}) for x in d]
         fm = FoamDrainage(foam_params(), bubbles)
         h = W.foam_noise(fm.offsets, [T * P] * nb, [b.h0 for b in bubbles], seed=seed + 1)
         fm.set_fields(W.ff_exact(h))
+        if shapes is not None:
+            fm.set_tile_shapes(shapes)
         return fm
 
     # online training on plain-solver trajectories of 16 random bubbles; a new
@@ -319,6 +321,9 @@ def bench_surrogate(args, L, W, n_bubbles):
     steps = windows * K
 
     ref = make(n_bubbles, 77)
+    # The plain solver at its fastest tile shapes (the state is kept); the
+    # surrogate runs step with the same shapes.
+    shapes = ref.autotune(trial_steps=5)
     h0, _ = ref.fields()
     ref.synchronize()
     t1 = time.perf_counter()
@@ -332,7 +337,7 @@ def bench_surrogate(args, L, W, n_bubbles):
     runs = {}
     for name, pred in [("network", L.PREDICT_NETWORK), ("identity", L.PREDICT_IDENTITY),
                        ("extrapolate", L.PREDICT_EXTRAPOLATE)]:
-        fm = make(n_bubbles, 77)
+        fm = make(n_bubbles, 77, shapes)
         fm.synchronize()
         t2 = time.perf_counter()
         st = model.advance_foam(fm, windows, tol=args.tol, predictor=pred)
@@ -360,7 +365,8 @@ def bench_surrogate(args, L, W, n_bubbles):
                                "plain-step judge per window (tolerance %g) and fallback, %d bubble(s) at 256x512, "
                                "%d steps; trained online for %d windows on plain-solver trajectories"
                                % (args.workload, args.tol, n_bubbles, steps, args.train_windows),
-                   "bubbles": n_bubbles, "cells": cells, "window": K, "windows": windows},
+                   "bubbles": n_bubbles, "cells": cells, "window": K, "windows": windows,
+                   "tile_shapes": list(shapes)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": achieved / 2500.0, "traffic": None,
                      "kernel": "network inference phase (fno_row_kernel + gemm_kernel, host-timed)",

@@ -355,16 +355,18 @@ int sf_foam_synchronize(sf_foam* foam);
 /* Replays 16-step HIP graphs for long step counts (default on). */
 int sf_foam_use_graph(sf_foam* foam, int32_t enable);
 /* Tile shapes: a wave streams a band of rows of one column chunk.
- * index:            0   1   2   3   4   5   6   7
- * columns per lane: 1   1   2   2   2   4   4   4
- * rows per band:    16  32  16  32  64  16  32  64
- * Times every shape on each bubble-size bucket (n_phi <= 96, <= 192, <= 384,
- * > 384) for trial_steps steps on a copy of the state, keeps the fastest per
- * bucket and restores the state.  chosen[4] receives the shape per bucket,
- * trial_ms[4][8] the average step-kernel time of each trial (-1 for buckets
- * without bubbles); either may be NULL. */
+ * index:            0   1   2   3   4   5   6   7   8   9
+ * columns per lane: 1   1   2   2   2   4   4   4   1   1
+ * rows per band:    16  32  16  32  64  16  32  64  4   2
+ * Foams of up to 2^19 cells start with shape 9, up to 2^23 with shape 8,
+ * larger ones with 1, 3, 6, 6 per bucket.  Times every shape (0-7 only for
+ * foams above 2^23 cells) on each bubble-size bucket (n_phi <= 96, <= 192,
+ * <= 384, > 384) for trial_steps steps on a copy of the state, keeps the
+ * fastest per bucket and restores the state.  chosen[4] receives the shape
+ * per bucket, trial_ms[4][10] the average step-kernel time of each trial (-1
+ * for buckets without bubbles or shapes not tried); either may be NULL. */
 int sf_foam_autotune(sf_foam* foam, int32_t trial_steps, int32_t* chosen, double* trial_ms);
-/* Sets the tile shape index (0-7, as above) of each of the 4 buckets. */
+/* Sets the tile shape index (0-9, as above) of each of the 4 buckets. */
 int sf_foam_set_tile_shapes(sf_foam* foam, const int32_t* shapes);
 /* dt[n_bubbles]: each bubble's time step. */
 int sf_foam_bubble_dt(sf_foam* foam, double* dt);

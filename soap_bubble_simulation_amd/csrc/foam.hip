@@ -80,10 +80,15 @@ struct FoamArgs {
 };
 
 // Tile shapes: columns per lane x rows per item (a band of rows is streamed
-// top to bottom by one wave, so taller bands re-read fewer halo rows).
-constexpr int kShapes = 8;
-constexpr int kShapeCpl[kShapes] = {1, 1, 2, 2, 2, 4, 4, 4};
-constexpr int kShapeBand[kShapes] = {16, 32, 16, 32, 64, 16, 32, 64};
+// top to bottom by one wave, so taller bands re-read fewer halo rows; the
+// short bands of shapes 8 and 9 give small foams enough waves).
+constexpr int kShapes = 10;
+constexpr int kShapeCpl[kShapes] = {1, 1, 2, 2, 2, 4, 4, 4, 1, 1};
+constexpr int kShapeBand[kShapes] = {16, 32, 16, 32, 64, 16, 32, 64, 4, 2};
+// Foams above this many cells are bandwidth-bound: the autotuner leaves out
+// the short-band shapes there, whose extra halo reads a bucket timed on its
+// own (latency-bound) does not show.
+constexpr int64_t kShortBandMaxCells = int64_t{1} << 23;
 constexpr int kPaths = 4;     // 1, 2, 4 (float4) and 4 (two float2) columns per lane
 
 // One row of a lane's columns plus its west and east neighbours.
@@ -784,7 +789,8 @@ struct Foam {
             if (!present) continue;
             double best = INFINITY;
             int best_v = variant_of_bucket[bk];
-            for (int v = 0; v < kShapes; ++v) {
+            const int nshapes = total > kShortBandMaxCells ? 8 : kShapes;
+            for (int v = 0; v < nshapes; ++v) {
                 // Only this bucket's bubbles take part in the trial.
                 std::vector<FoamBubble> saved = bub;
                 int save[4];
@@ -931,6 +937,14 @@ extern "C" int sf_foam_create(const sf_params* params, int32_t n, const sf_bubbl
             }
             f->time.assign(n, 0.0);
             f->active.assign(n, 1);
+            // Small foams are latency-bound: short bands give them waves
+            // (one 256 x 512 bubble: 4.4 us per step with 2-row bands against
+            // 22.7 us with the 32-row default; 16 such bubbles: 11.1 against
+            // 25.6 us with 4-row bands).  sf_foam_autotune refines per bucket.
+            if (f->total <= (int64_t{1} << 19))
+                std::fill(f->variant_of_bucket, f->variant_of_bucket + 4, 9);
+            else if (f->total <= kShortBandMaxCells)
+                std::fill(f->variant_of_bucket, f->variant_of_bucket + 4, 8);
             f->layout_items();
             f->upload(hh.data(), ll.data(), gg.data());
             f->resolve_cfl();

@@ -19,7 +19,7 @@ import numpy as np
 from . import _lib as L
 
 # Tile shapes (columns per lane, rows per band) by index, as in sf_foam_autotune.
-TILE_SHAPES = ((1, 16), (1, 32), (2, 16), (2, 32), (2, 64), (4, 16), (4, 32), (4, 64))
+TILE_SHAPES = ((1, 16), (1, 32), (2, 16), (2, 32), (2, 64), (4, 16), (4, 32), (4, 64), (1, 4), (1, 2))
 
 
 @dataclass

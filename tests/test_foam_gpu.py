@@ -54,7 +54,7 @@ def seeded_fields(fm, seed, rel=0.05):
     return h, g
 
 
-@pytest.mark.parametrize("shape_idx", range(8))
+@pytest.mark.parametrize("shape_idx", range(10))
 def test_foam_trajectory_matches_oracle(oracle, shape_idx):
     fm, p = make_foam(SHAPES, seed=10 + shape_idx)
     fm.set_tile_shapes([shape_idx] * 4)
@@ -119,7 +119,7 @@ def test_foam_autotune_keeps_state_and_shapes_agree():
     fm.step(5)
     before = fm.fields()
     chosen = fm.autotune(trial_steps=3)
-    assert all(0 <= c < 8 for c in chosen)
+    assert all(0 <= c < 10 for c in chosen)
     after = fm.fields()
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     assert fm.info()["variants"] == chosen
