@@ -175,9 +175,28 @@ __global__ __launch_bounds__(256) void cluster_integrate_kernel(const float4* __
     fy.add(c.gdir[1] * bm);
     fz.add(c.gdir[2] * bm);
     const int rows = (c.npruns + kPairRuns - 1) / kPairRuns;
-    for (int r = 0; r < rows; ++r) {
-        const float* q = part + (static_cast<size_t>(r) * c.n + i) * 3;
-        fx.add(q[0]); fy.add(q[1]); fz.add(q[2]);
+    // Partials in row order; eight rows' loads are issued before their adds
+    // so that the loads overlap (the order of the sum is unchanged).
+    int r = 0;
+    for (; r + 8 <= rows; r += 8) {
+        float q[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float* src = part + (static_cast<size_t>(r + k) * c.n + i) * 3;
+            q[k][0] = src[0];
+            q[k][1] = src[1];
+            q[k][2] = src[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            fx.add(q[k][0]);
+            fy.add(q[k][1]);
+            fz.add(q[k][2]);
+        }
+    }
+    for (; r < rows; ++r) {
+        const float* src = part + (static_cast<size_t>(r) * c.n + i) * 3;
+        fx.add(src[0]); fy.add(src[1]); fz.add(src[2]);
     }
     fx.add(-v.x * dc);
     fy.add(-v.y * dc);
