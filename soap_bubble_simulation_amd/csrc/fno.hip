@@ -536,8 +536,13 @@ struct Fno {
         const long long tot = static_cast<long long>(B) * Mt * Mp * C;
         hipLaunchKernelGGL(theta_combine_kernel, dim3(blocks_for(tot)), dim3(256), 0, stream, G, spec[l], Mt, Mp, C,
                            tot);
-        hipLaunchKernelGGL(mode_mix_kernel, dim3(Mt * Mp), dim3(256), 0, stream, spec[l], w + o_r[l], Y, B, Mt, Mp,
-                           C);
+        // Weights in LDS (2 C C floats, at most 32 KiB); the inputs too when
+        // the batch fits beside them.
+        const size_t mm_r = static_cast<size_t>(2 * C * C) * sizeof(float);
+        const size_t mm_x = static_cast<size_t>(B) * 2 * C * sizeof(float);
+        const int stage_x = mm_r + mm_x <= 64 * 1024 ? 1 : 0;
+        hipLaunchKernelGGL(mode_mix_kernel, dim3(Mt * Mp), dim3(256), stage_x ? mm_r + mm_x : mm_r, stream, spec[l],
+                           w + o_r[l], Y, B, Mt, Mp, C, stage_x);
         gemm(Ath, Y, Z, 2 * T, Mp * C, 2 * Mt, 2 * Mt, Mp * C, Mp * C, 0, static_cast<long long>(2 * Mt) * Mp * C,
              static_cast<long long>(2 * T) * Mp * C, B, 0, 0, 1.f, 0.f);
     }

@@ -615,16 +615,30 @@ __global__ void theta_combine_adjoint_kernel(const float* dXs, float* dG, int Mt
 // Spectral mixing per retained mode (k, m): Y[b][k][m][o] = sum_c Xs[b][k][m][c] R[k][m][c][o]
 // (complex).  Xs: [B][Mt][Mp][2][C]; R: [Mt][Mp][2][C][C]; Y written in the
 // layout of the inverse theta GEMM: [B][2][Mt][Mp][C].  One block per mode.
-__global__ void mode_mix_kernel(const float* Xs, const float* R, float* Y, int Bn, int Mt, int Mp, int C) {
+// The mode's weights (2 C C) and, when stage_x is set, the Bn input vectors
+// (2 C each) are staged in dynamic LDS first, so the products read no global
+// memory.
+__global__ void mode_mix_kernel(const float* Xs, const float* R, float* Y, int Bn, int Mt, int Mp, int C,
+                                int stage_x) {
+    extern __shared__ float mm_lds[];
+    float* sR = mm_lds;                        // [2][C][C]
+    float* sX = mm_lds + 2 * C * C;            // [Bn][2][C]
     const int mode = blockIdx.x;               // k * Mp + m
     const float* Rm = R + static_cast<long long>(mode) * 2 * C * C;
+    for (int t = threadIdx.x; t < 2 * C * C; t += blockDim.x) sR[t] = Rm[t];
+    if (stage_x)
+        for (int t = threadIdx.x; t < Bn * 2 * C; t += blockDim.x) {
+            const int b = t / (2 * C), e = t - b * 2 * C;
+            sX[t] = Xs[(static_cast<long long>(b) * Mt * Mp + mode) * 2 * C + e];
+        }
+    __syncthreads();
     for (int q = threadIdx.x; q < Bn * C; q += blockDim.x) {
         const int b = q / C, o = q % C;
-        const float* x = Xs + (static_cast<long long>(b) * Mt * Mp + mode) * 2 * C;
+        const float* x = stage_x ? sX + b * 2 * C : Xs + (static_cast<long long>(b) * Mt * Mp + mode) * 2 * C;
         float re = 0.f, im = 0.f;
         for (int c = 0; c < C; ++c) {
             const float xr = x[c], xi = x[C + c];
-            const float rr = Rm[c * C + o], ri = Rm[C * C + c * C + o];
+            const float rr = sR[c * C + o], ri = sR[C * C + c * C + o];
             re += xr * rr - xi * ri;
             im += xr * ri + xi * rr;
         }
