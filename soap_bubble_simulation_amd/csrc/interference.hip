@@ -938,7 +938,12 @@ extern "C" int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, doub
         SF_REQUIRE(std::isfinite(dt), SF_ERR_INVALID, "dt must be finite");
         Drainage* d = reinterpret_cast<Drainage*>(ctx);
         if (n_steps > 0) dt = d->resolve_dt(dt);
-        if (n_steps > 0) spectral_prepare(d, cmf);
+        if (n_steps > 0) {
+            spectral_prepare(d, cmf);
+            // Curves and operand format are checked before any step is queued.
+            SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
+            (void)spectral_format(d);
+        }
         // The pole rows stay lazy (ring partials): the spectral kernel takes
         // their ring mean itself, and the next step reads the partials.
         for (int s = 0; s < n_steps; ++s) {

@@ -268,6 +268,14 @@ def test_spectral_fp16_range():
     L.check(lib.sf_spectral_set_operand(ctx.handle, L.SPECTRAL_OPERAND_FP16))
     with pytest.raises(L.SoapFilmError, match="FP16"):
         spectral_rgb(ctx, cam, cm)
+    # The stepping call refuses before it queues any step.
+    t0 = ctx.stats().current_time
+    camera = L.Camera()
+    for k in range(3):
+        camera.position[k] = cam[k]
+    assert lib.sf_drainage_step_spectral(ctx.handle, 3, 0.0, C.byref(camera), 1.33, L.fptr(cm)) != 0
+    assert "FP16" in L.last_error()
+    assert ctx.stats().current_time == t0
     assert lib.sf_spectral_set_operand(ctx.handle, 7) != 0
 
 
