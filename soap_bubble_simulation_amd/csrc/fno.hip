@@ -278,6 +278,8 @@ struct Fno {
     float* Iphi = nullptr;     // [P][2 Mp]
     unsigned short* Ephi16 = nullptr;   // Ephi and Iphi rounded to BF16 (the
     unsigned short* Iphi16 = nullptr;   // fused layer kernel's MFMA operands)
+    unsigned short* Eth16 = nullptr;    // Eth and Ath rounded to BF16 (the fused
+    unsigned short* Ath16 = nullptr;    // inference theta stage's MFMA operands)
 
     // activations (max batch Bm)
     float* in = nullptr;       // [Bm][N][kCin]
@@ -315,7 +317,7 @@ struct Fno {
 
     ~Fno() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, in, out, target, dout, ybuf, X1, G, Y, Z,
+        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, Eth16, Ath16, in, out, target, dout, ybuf, X1, G, Y, Z,
                       dxa, dxb, dz, dXs, part, loss_part, dscratch, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
                       sel_hi, sel_lo, hp_hi, hp_lo, d_win, d_take, rpart};
         for (void* p : ps) (void)hipFree(p);
@@ -425,6 +427,11 @@ struct Fno {
         Iphi16 = galloc<unsigned short>(i16.size());
         SF_HIP(hipMemcpy(Ephi16, e16.data(), e16.size() * 2, hipMemcpyHostToDevice));
         SF_HIP(hipMemcpy(Iphi16, i16.data(), i16.size() * 2, hipMemcpyHostToDevice));
+        const std::vector<unsigned short> et16 = bf16(eth), at16 = bf16(ath);
+        Eth16 = galloc<unsigned short>(et16.size());
+        Ath16 = galloc<unsigned short>(at16.size());
+        SF_HIP(hipMemcpy(Eth16, et16.data(), et16.size() * 2, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(Ath16, at16.data(), at16.size() * 2, hipMemcpyHostToDevice));
     }
 
     void init_params(uint64_t seed) {
@@ -547,6 +554,20 @@ struct Fno {
              static_cast<long long>(2 * T) * Mp * C, B, 0, 0, 1.f, 0.f);
     }
 
+    // Inference theta stage of layer l: the fused kernel where its shape
+    // constraints and LDS tile allow, else the general sequence.
+    void theta_stage_inference(int l, int B) {
+        const size_t lds = static_cast<size_t>(2 * C) * (T + 8) * 2 + static_cast<size_t>(2 * Mt) * (2 * C + 1) * 4 +
+                           static_cast<size_t>(C) * (2 * Mt + 8) * 2;
+        if (C % 16 || (2 * Mt) % 16 || T % 16 || lds > 64 * 1024) {
+            theta_stage(l, B);
+            return;
+        }
+        hipLaunchKernelGGL(theta_fused_kernel, dim3(static_cast<unsigned>(Mp), static_cast<unsigned>(B)), dim3(512), lds,
+                           stream, X1, Eth16, w + o_r[l], Ath16, Z, T, Mt, Mp, C);
+        SF_HIP(hipGetLastError());
+    }
+
     // training: keep the pre-activations for the backward pass
     void forward(int B, bool training) {
         const long long R = static_cast<long long>(B) * N;
@@ -563,7 +584,7 @@ struct Fno {
                                Ephi, w + o_wl, w + o_bl, X1, P, J, C);
             SF_HIP(hipGetLastError());
             for (int l = 0; l < L; ++l) {
-                theta_stage(l, B);
+                theta_stage_inference(l, B);
                 const bool last = l == L - 1;
                 // Activations between layers in BF16 (the precision the next
                 // layer's MFMA reads them at).

@@ -649,6 +649,96 @@ __global__ void mode_mix_kernel(const float* Xs, const float* R, float* Y, int B
     }
 }
 
+// Fused theta stage of one layer for inference, one block (8 waves) per
+// (phi mode m, bubble b): the theta DFT of the m-th phi coefficients of all
+// T rows, the complex combination, the per-mode channel mixing and the
+// inverse theta DFT, with the intermediate spectrum in LDS.  It computes the
+// same products as the gemm_kernel / theta_combine_kernel / mode_mix_kernel /
+// gemm_kernel sequence, in the same order and precision (BF16 operands of
+// 16x16x16 MFMAs accumulated in FP32 over increasing k; FP32 combination and
+// mixing), so the results match that sequence.
+//   X1  [B T][J][C]     phi DFT of the layer input, J = 2 Mp (re rows, then im)
+//   Eth [2 Mt][T] BF16  forward theta DFT (cos rows, then -sin rows)
+//   R   [Mt Mp][2][C][C] mode weights (re, im)
+//   Ath [2 T][2 Mt] BF16 inverse theta DFT
+//   Z   [B T][J][C]     output (the columns j = m and Mp + m of this block)
+// Dynamic LDS: X slice [2C][T + 8] BF16, G [2 Mt][2C + 1] FP32, Y [C][2 Mt + 8]
+// BF16.  Requires C, 2 Mt and T multiples of 16.
+__global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restrict__ X1,
+                                                          const unsigned short* __restrict__ Eth,
+                                                          const float* __restrict__ R,
+                                                          const unsigned short* __restrict__ Ath,
+                                                          float* __restrict__ Z, int T, int Mt, int Mp, int C) {
+    extern __shared__ __align__(16) unsigned char th_lds[];
+    const int m = blockIdx.x;
+    const long long b = blockIdx.y;
+    const int J = 2 * Mp, C2 = 2 * C, M2 = 2 * Mt;
+    const int xs_ld = T + 8, g_ld = C2 + 1, y_ld = M2 + 8;
+    unsigned short* sx = reinterpret_cast<unsigned short*>(th_lds);                        // [C2][xs_ld]
+    float* sg = reinterpret_cast<float*>(th_lds + static_cast<size_t>(C2) * xs_ld * 2);    // [M2][g_ld]
+    unsigned short* sy = reinterpret_cast<unsigned short*>(sg + static_cast<size_t>(M2) * g_ld);   // [C][y_ld]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int mr = lane & 15, kq = 4 * (lane >> 4);
+
+    // 1. the m-th coefficients of every row, transposed to [q][t], q = ri C + c
+    for (int idx = tid; idx < T * C2; idx += blockDim.x) {
+        const int t = idx / C2, q = idx - t * C2;
+        const int ri = q / C, c = q - ri * C;
+        sx[q * xs_ld + t] = to_bf16(X1[((b * T + t) * J + ri * Mp + m) * C + c]);
+    }
+    __syncthreads();
+
+    // 2. G[p][q] = sum_t Eth[p][t] X[t][q]
+    const int gt_p = M2 / 16, gt_q = C2 / 16;
+    for (int tile = wave; tile < gt_p * gt_q; tile += nwaves) {
+        const int pt = tile / gt_q, qt = tile - pt * gt_q;
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < T; k0 += 16) {
+            const bf16x4 A = *reinterpret_cast<const bf16x4*>(Eth + static_cast<long long>(pt * 16 + mr) * T + k0 + kq);
+            const bf16x4 Bf = *reinterpret_cast<const bf16x4*>(sx + (qt * 16 + mr) * xs_ld + k0 + kq);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bf, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sg[(pt * 16 + kq + v) * g_ld + qt * 16 + mr] = acc[v];
+    }
+    __syncthreads();
+
+    // 3-4. spectrum of mode (k, m) (theta_combine_kernel) mixed over channels
+    // (mode_mix_kernel); Y stored transposed [o][q], q = plane Mt + k
+    for (int idx = tid; idx < Mt * C; idx += blockDim.x) {
+        const int k = idx / C, o = idx - k * C;
+        const float* Rm = R + static_cast<long long>(k * Mp + m) * 2 * C * C;
+        float re = 0.f, im = 0.f;
+        for (int c = 0; c < C; ++c) {
+            const float xr = sg[k * g_ld + c] - sg[(Mt + k) * g_ld + C + c];
+            const float xi = sg[k * g_ld + C + c] + sg[(Mt + k) * g_ld + c];
+            const float rr = Rm[c * C + o], rim = Rm[C * C + c * C + o];
+            re += xr * rr - xi * rim;
+            im += xr * rim + xi * rr;
+        }
+        sy[o * y_ld + k] = to_bf16(re);
+        sy[o * y_ld + Mt + k] = to_bf16(im);
+    }
+    __syncthreads();
+
+    // 5. Z[r][o] = sum_q Ath[r][q] Y[q][o], r = 2 t + ri
+    const int zt_r = (2 * T) / 16, zt_o = C / 16;
+    for (int tile = wave; tile < zt_r * zt_o; tile += nwaves) {
+        const int rt = tile / zt_o, ot = tile - rt * zt_o;
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < M2; k0 += 16) {
+            const bf16x4 A = *reinterpret_cast<const bf16x4*>(Ath + static_cast<long long>(rt * 16 + mr) * M2 + k0 + kq);
+            const bf16x4 Bf = *reinterpret_cast<const bf16x4*>(sy + (ot * 16 + mr) * y_ld + k0 + kq);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bf, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = rt * 16 + kq + v, t = r >> 1, ri = r & 1;
+            Z[((b * T + t) * J + ri * Mp + m) * C + ot * 16 + mr] = acc[v];
+        }
+    }
+}
+
 // Backward of mode_mix_kernel: dXs (overwritten) and dR (accumulated).
 __global__ void mode_mix_grad_kernel(const float* Xs, const float* R, const float* dY, float* dXs, float* dR, int Bn,
                                      int Mt, int Mp, int C) {
