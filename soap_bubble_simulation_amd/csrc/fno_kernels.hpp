@@ -693,10 +693,20 @@ __global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restric
     for (int tile = wave; tile < gt_p * gt_q; tile += nwaves) {
         const int pt = tile / gt_q, qt = tile - pt * gt_q;
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < T; k0 += 16) {
-            const bf16x4 A = *reinterpret_cast<const bf16x4*>(Eth + static_cast<long long>(pt * 16 + mr) * T + k0 + kq);
-            const bf16x4 Bf = *reinterpret_cast<const bf16x4*>(sx + (qt * 16 + mr) * xs_ld + k0 + kq);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bf, acc, 0, 0, 0);
+        const unsigned short* erow = Eth + static_cast<long long>(pt * 16 + mr) * T + kq;
+        // Eight k-steps' A fragments are loaded before their MFMAs, so the
+        // global loads overlap instead of each waiting in turn.
+        for (int k0 = 0; k0 < T; k0 += 128) {
+            bf16x4 Af[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                Af[s] = k0 + 16 * s < T ? *reinterpret_cast<const bf16x4*>(erow + k0 + 16 * s) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                if (k0 + 16 * s >= T) break;
+                const bf16x4 Bf = *reinterpret_cast<const bf16x4*>(sx + (qt * 16 + mr) * xs_ld + k0 + 16 * s + kq);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Af[s], Bf, acc, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) sg[(pt * 16 + kq + v) * g_ld + qt * 16 + mr] = acc[v];
@@ -709,12 +719,23 @@ __global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restric
         const int k = idx / C, o = idx - k * C;
         const float* Rm = R + static_cast<long long>(k * Mp + m) * 2 * C * C;
         float re = 0.f, im = 0.f;
-        for (int c = 0; c < C; ++c) {
-            const float xr = sg[k * g_ld + c] - sg[(Mt + k) * g_ld + C + c];
-            const float xi = sg[k * g_ld + C + c] + sg[(Mt + k) * g_ld + c];
-            const float rr = Rm[c * C + o], rim = Rm[C * C + c * C + o];
-            re += xr * rr - xi * rim;
-            im += xr * rim + xi * rr;
+        // Eight channels' weights are loaded before their products (C is a
+        // multiple of 16).
+        for (int c0 = 0; c0 < C; c0 += 8) {
+            float rr[8], rim[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                rr[s] = Rm[(c0 + s) * C + o];
+                rim[s] = Rm[C * C + (c0 + s) * C + o];
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int c = c0 + s;
+                const float xr = sg[k * g_ld + c] - sg[(Mt + k) * g_ld + C + c];
+                const float xi = sg[k * g_ld + C + c] + sg[(Mt + k) * g_ld + c];
+                re += xr * rr[s] - xi * rim[s];
+                im += xr * rim[s] + xi * rr[s];
+            }
         }
         sy[o * y_ld + k] = to_bf16(re);
         sy[o * y_ld + Mt + k] = to_bf16(im);
@@ -726,10 +747,18 @@ __global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restric
     for (int tile = wave; tile < zt_r * zt_o; tile += nwaves) {
         const int rt = tile / zt_o, ot = tile - rt * zt_o;
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < M2; k0 += 16) {
-            const bf16x4 A = *reinterpret_cast<const bf16x4*>(Ath + static_cast<long long>(rt * 16 + mr) * M2 + k0 + kq);
-            const bf16x4 Bf = *reinterpret_cast<const bf16x4*>(sy + (ot * 16 + mr) * y_ld + k0 + kq);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bf, acc, 0, 0, 0);
+        const unsigned short* arow = Ath + static_cast<long long>(rt * 16 + mr) * M2 + kq;
+        for (int k0 = 0; k0 < M2; k0 += 128) {
+            bf16x4 Af[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                Af[s] = k0 + 16 * s < M2 ? *reinterpret_cast<const bf16x4*>(arow + k0 + 16 * s) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                if (k0 + 16 * s >= M2) break;
+                const bf16x4 Bf = *reinterpret_cast<const bf16x4*>(sy + (ot * 16 + mr) * y_ld + k0 + 16 * s + kq);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Af[s], Bf, acc, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
