@@ -680,11 +680,26 @@ __global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restric
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int mr = lane & 15, kq = 4 * (lane >> 4);
 
-    // 1. the m-th coefficients of every row, transposed to [q][t], q = ri C + c
-    for (int idx = tid; idx < T * C2; idx += blockDim.x) {
-        const int t = idx / C2, q = idx - t * C2;
-        const int ri = q / C, c = q - ri * C;
-        sx[q * xs_ld + t] = to_bf16(X1[((b * T + t) * J + ri * Mp + m) * C + c]);
+    // 1. the m-th coefficients of every row, transposed to [q][t], q = ri C + c;
+    // eight loads per thread are in flight before their stores
+    const int nx = T * C2, step = static_cast<int>(blockDim.x);
+    for (int idx0 = tid; idx0 < nx; idx0 += 8 * step) {
+        float v[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int idx = idx0 + s * step;
+            const int t = idx / C2, q = idx - t * C2;
+            const int ri = q / C, c = q - ri * C;
+            v[s] = idx < nx ? X1[((b * T + t) * J + ri * Mp + m) * C + c] : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int idx = idx0 + s * step;
+            if (idx < nx) {
+                const int t = idx / C2, q = idx - t * C2;
+                sx[q * xs_ld + t] = to_bf16(v[s]);
+            }
+        }
     }
     __syncthreads();
 
