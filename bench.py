@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3", "c4", "c5", "cluster"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-drainage", "c1", "c3", "c4", "c5", "cluster", "frame"])
     ap.add_argument("--train-windows", type=int, default=300,
                     help="c4/c5: online training windows before the timed surrogate run")
     ap.add_argument("--tol", type=float, default=1e-3, help="c4/c5: judge tolerance")
@@ -219,7 +219,11 @@ def bench_foam(args, L, W):
 
 
 CLUSTER_FLOP_PER_PAIR = 13.0     # delta 3, |delta|^2 5, sqrt 1, direction 3, r_a + r_b 1
-FP32_VALU_PEAK_TFLOPS = 78.6     # MI355X FP32 vector without packed math
+FP32_VALU_PEAK_TFLOPS = 78.6
+# Consumers of the thickness field: the caustic map reads hi and lo and writes
+# one FP32 intensity; the LUT render reads hi and lo and writes one RGBA8 pixel.
+CAUSTIC_BYTES_PER_CELL = 12.0
+RENDER_BYTES_PER_CELL = 12.0     # MI355X FP32 vector without packed math
 
 
 def bench_cluster(args, L, W):
@@ -275,6 +279,78 @@ def bench_cluster(args, L, W):
         out["cpu_baseline"] = {"value": m * (m - 1) / 2.0 * steps / el, "unit": "pair-interactions/s",
                                "cores": 1, "kind": "port",
                                "sample": "%d FP64 numpy oracle steps of the first %d bubbles" % (steps, m)}
+    print(json.dumps(out))
+
+
+def bench_frame(args, L, W):
+    """Consumers of the thickness field on the 2048x4096 single-bubble state:
+    the caustic-intensity map (caustics_compute.wgsl) and the frame rendered
+    through the RGBA8 LUT (bubble.wgsl:425-439).  One step = one caustic map
+    of the whole grid, device-resident, back to back; the rendered frame is
+    timed per call with its LUT build and the copy of the frame to the host."""
+    from soap_bubble_simulation_amd.drainage import DrainageContext
+    from soap_bubble_simulation_amd.export import render_frame
+    from soap_bubble_simulation_amd.caustics import default_params
+
+    n_theta, n_phi = 2048, 4096
+    cells = n_theta * n_phi
+    h0 = W.smooth_bubble(n_theta, n_phi, seed=3)
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    p.flags = L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG
+    ctx = DrainageContext(p, n_theta, n_phi, 500e-9)
+    ctx.set_fields(h0, np.full_like(h0, 0.5))
+    ctx.step(8, ctx.stats().dt_cfl * p.cfl_safety)
+    cam = W.camera(p.bubble_radius)
+    cp = default_params()
+    lib = L.lib()
+    steps = min(args.steps, 2000)
+    for _ in range(max(1, min(args.warmup, 100))):
+        L.check(lib.sf_caustics(ctx.handle, C.byref(cp), None))
+    L.check(lib.sf_drainage_synchronize(ctx.handle))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.check(lib.sf_caustics(ctx.handle, C.byref(cp), None))
+    L.check(lib.sf_drainage_synchronize(ctx.handle))
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1e3 / steps
+    algo = CAUSTIC_BYTES_PER_CELL * cells
+    achieved = algo / (ms * 1e-3) / 1e9
+    frame = render_frame(ctx, camera=cam, refractive_index=1.33, intensity=4.0)
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        frame = render_frame(ctx, camera=cam, refractive_index=1.33, intensity=4.0)
+    render_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {
+        "metric": "caustic-map cells/s (caustics_compute.wgsl on the drainage state)", "value": cells / (ms * 1e-3),
+        "unit": "cells/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "caustic map and LUT-rendered RGBA8 frame of the single-bubble 2048x4096 "
+                               "float-float thickness state, CausticParams defaults, camera (2.2R, 1.4R, 3.1R)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "caustics_vec4_kernel",
+                     "avg_launch_ms": ms, "algorithmic_bytes_per_launch": algo,
+                     "timing": "wall clock over back-to-back launches on the context's stream"},
+        "render": {"ms_per_frame_incl_lut_and_copy": render_ms,
+                   "frame_bytes": int(frame.nbytes), "kernel": "lut_render_kernel",
+                   "algorithmic_bytes_per_cell": RENDER_BYTES_PER_CELL},
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+        from soap_bubble_simulation_amd.interference import interference_lut_rgba8
+        h, _ = ctx.fields()
+        hf = h.astype(np.float32)
+        t0 = time.perf_counter()
+        O.caustics(hf)
+        c_el = time.perf_counter() - t0
+        lut = np.asarray(interference_lut_rgba8(1.33, 1.0), np.uint8).reshape(64, 256, 4)
+        t0 = time.perf_counter()
+        O.render_lut(h, p.bubble_radius, cam, lut, intensity=4.0)
+        r_el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cells / c_el, "unit": "cells/s", "cores": 1, "kind": "port",
+                               "sample": "one caustic map of the full 2048x4096 grid by the C oracle",
+                               "render_ms_per_frame": r_el * 1e3}
     print(json.dumps(out))
 
 
@@ -413,6 +489,9 @@ def main():
         return
     if args.workload == "cluster":
         bench_cluster(args, L, W)
+        return
+    if args.workload == "frame":
+        bench_frame(args, L, W)
         return
     if args.workload in ("c4", "c5"):
         bench_surrogate(args, L, W, 1 if args.workload == "c4" else (args.bubbles or 1024))

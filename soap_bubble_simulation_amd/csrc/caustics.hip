@@ -10,7 +10,8 @@
 //   branch = 1.5 if det < -branch_threshold else 1,
 //   refraction = 1 + |grad| / 2      (caustics_compute.wgsl:89-114).
 // FP32 like the shader, with its operation order and no contraction, so the
-// branch decision matches any FP32 evaluation of the same expressions.
+// branch decision matches any FP32 evaluation of the same expressions; the
+// power runs on the hardware log2/exp2 (caustic_pow).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -18,6 +19,8 @@
 #include "drainage.hpp"
 
 namespace sf {
+
+constexpr unsigned kCausticXcds = 8;   // MI355X: 8 accelerator complex dies
 
 struct CausticArgs {
     const float* hi;
@@ -35,16 +38,26 @@ __device__ inline float th_at(const CausticArgs& a, int x, int y) {
     return a.lo ? a.hi[k] + a.lo[k] : a.hi[k];
 }
 
-__global__ __launch_bounds__(256) void caustics_kernel(const CausticArgs a) {
+// I^s for the map's I >= 0 as exp2(s * log2 I) on the hardware log2 and exp2
+// (v_log_f32, v_exp_f32: about one ulp each), the product split exactly with
+// an FMA and its rounding error folded back to first order, so the result
+// stays within about 1e-6 relative of the correctly rounded power; the shader
+// evaluates pow() the same way.  s == 0 gives 1 (pow(x, 0) for every x).
+__device__ inline float caustic_pow(float x, float s) {
+    if (s == 0.0f) return 1.0f;
+    const float l = __builtin_amdgcn_logf(x);
+    const float p = s * l;
+    const float pe = fmaf(s, l, -p);
+    const float r = __builtin_amdgcn_exp2f(p);
+    // l = -inf (x = 0) or r = inf leave pe or the correction non-finite
+    return __builtin_isfinite(pe) && __builtin_isfinite(r) ? fmaf(r, pe * 0.6931471805599453f, r) : r;
+}
+
+// The intensity of one cell from its 3 x 3 neighbourhood (u: row above, d:
+// row below; l/r: west/east), in the shader's operation order.
+__device__ inline float caustic_cell(const CausticArgs& a, float c, float l, float r, float u, float d, float ul,
+                                     float ur, float dl, float dr) {
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= a.np) return;
-    const float c = th_at(a, x, y);
-    const float l = th_at(a, x - 1, y), r = th_at(a, x + 1, y);
-    const float u = th_at(a, x, y - 1), d = th_at(a, x, y + 1);
-    const float ul = th_at(a, x - 1, y - 1), ur = th_at(a, x + 1, y - 1);
-    const float dl = th_at(a, x - 1, y + 1), dr = th_at(a, x + 1, y + 1);
     const float gx = (r - l) / 2.0f;
     const float gy = (d - u) / 2.0f;
     const float lap = (l + r + u + d - 4.0f * c) / 1.0f;
@@ -57,12 +70,81 @@ __global__ __launch_bounds__(256) void caustics_kernel(const CausticArgs a) {
     const float branch = det < -a.threshold ? 1.5f : 1.0f;
     const float refraction = 1.0f + sqrtf(gx * gx + gy * gy) * 0.5f;
     float I = focusing * branch * refraction;
-    I = powf(I, a.sharpness) * a.intensity;
-    a.out[static_cast<size_t>(y) * a.np + x] = fminf(fmaxf(I, 0.0f), 10.0f);
+    I = caustic_pow(I, a.sharpness) * a.intensity;
+    return fminf(fmaxf(I, 0.0f), 10.0f);
+}
+
+// Any width: one cell per thread, nine gathered loads.
+__global__ __launch_bounds__(256) void caustics_kernel(const CausticArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= a.np) return;
+    a.out[static_cast<size_t>(y) * a.np + x] =
+        caustic_cell(a, th_at(a, x, y), th_at(a, x - 1, y), th_at(a, x + 1, y), th_at(a, x, y - 1),
+                     th_at(a, x, y + 1), th_at(a, x - 1, y - 1), th_at(a, x + 1, y - 1), th_at(a, x - 1, y + 1),
+                     th_at(a, x + 1, y + 1));
+}
+
+// Thickness of four consecutive cells of row y from column c (c % 4 == 0).
+__device__ inline float4 th4_at(const CausticArgs& a, int y, int c) {
+    const size_t k = static_cast<size_t>(y) * a.np + c;
+    float4 t = *reinterpret_cast<const float4*>(a.hi + k);
+    if (a.lo) {
+        const float4 l = *reinterpret_cast<const float4*>(a.lo + k);
+        t.x += l.x; t.y += l.y; t.z += l.z; t.w += l.w;
+    }
+    return t;
+}
+
+__device__ inline float th1_at(const CausticArgs& a, int y, int c) {
+    const size_t k = static_cast<size_t>(y) * a.np + c;
+    return a.lo ? a.hi[k] + a.lo[k] : a.hi[k];
+}
+
+// Widths divisible by 4: a lane owns four consecutive cells of one row and
+// loads them as float4 from the rows above, at and below; the west and east
+// neighbours come from the adjacent lanes, and only the wave's first lane
+// and its last lane in the row load theirs (phi wrapped).  A block is four
+// waves, 1024 columns of one row; blocks are renumbered so that each XCD
+// walks a contiguous run of rows and the rows it shares with its neighbours
+// stay in its own L2.
+__global__ __launch_bounds__(256) void caustics_vec4_kernel(const CausticArgs a) {
+    const int lane = threadIdx.x & 63;
+    const unsigned nblk = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+    const unsigned per_xcd = (nblk + kCausticXcds - 1) / kCausticXcds;
+    unsigned v = (b % kCausticXcds) * per_xcd + b / kCausticXcds;
+    if (nblk % kCausticXcds != 0) v = b;
+    const int bx = static_cast<int>(v % gridDim.x), y = static_cast<int>(v / gridDim.x);
+    const int wave0 = (bx * 4 + static_cast<int>(threadIdx.x >> 6)) * 256;
+    if (wave0 >= a.np) return;                       // the whole wave is past the row's end
+    const int c0 = wave0 + 4 * lane;
+    const bool valid = c0 < a.np;
+    const int cc = valid ? c0 : a.np - 4;
+    const int cw = cc == 0 ? a.np - 1 : cc - 1;
+    const int ce = cc + 4 == a.np ? 0 : cc + 4;
+    const bool load_w = lane == 0;
+    const bool load_e = lane == 63 || cc + 4 >= a.np;
+    const int yu = max(y - 1, 0), yd = min(y + 1, a.nt - 1);
+    const float4 U = th4_at(a, yu, cc), M = th4_at(a, y, cc), D = th4_at(a, yd, cc);
+    float uw = __shfl_up(U.w, 1), mw = __shfl_up(M.w, 1), dw = __shfl_up(D.w, 1);
+    float ue = __shfl_down(U.x, 1), me = __shfl_down(M.x, 1), de = __shfl_down(D.x, 1);
+    if (load_w) { uw = th1_at(a, yu, cw); mw = th1_at(a, y, cw); dw = th1_at(a, yd, cw); }
+    if (load_e) { ue = th1_at(a, yu, ce); me = th1_at(a, y, ce); de = th1_at(a, yd, ce); }
+    if (!valid) return;
+    float4 o;
+    o.x = caustic_cell(a, M.x, mw, M.y, U.x, D.x, uw, U.y, dw, D.y);
+    o.y = caustic_cell(a, M.y, M.x, M.z, U.y, D.y, U.x, U.z, D.x, D.z);
+    o.z = caustic_cell(a, M.z, M.y, M.w, U.z, D.z, U.y, U.w, D.y, D.w);
+    o.w = caustic_cell(a, M.w, M.z, me, U.w, D.w, U.z, ue, D.z, de);
+    *reinterpret_cast<float4*>(a.out + static_cast<size_t>(y) * a.np + c0) = o;
 }
 
 static void launch_caustics(const CausticArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(caustics_kernel, dim3((a.np + 255) / 256, a.nt), dim3(256), 0, s, a);
+    if (a.np % 4 == 0)
+        hipLaunchKernelGGL(caustics_vec4_kernel, dim3((a.np + 1023) / 1024, a.nt), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(caustics_kernel, dim3((a.np + 255) / 256, a.nt), dim3(256), 0, s, a);
     SF_HIP(hipGetLastError());
 }
 

@@ -994,14 +994,22 @@ struct LutRenderArgs {
     float cx, cy, cz, cam2, radius, intensity;
 };
 
-__device__ inline float lut_texel(const uint8_t* lut, int i, int j, int c) {
-    return static_cast<float>(lut[(static_cast<size_t>(j) * kLutT + i) * 4 + c]) * (1.f / 255.f);
+__device__ inline uchar4 lut_texel4(const uint8_t* lut, int i, int j) {
+    return reinterpret_cast<const uchar4*>(lut)[static_cast<size_t>(j) * kLutT + i];
 }
 
+__device__ inline float lut_chan(uchar4 t, int c) {
+    const uint8_t b = c == 0 ? t.x : (c == 1 ? t.y : t.z);
+    return static_cast<float>(b) * (1.f / 255.f);
+}
+
+// One pixel per thread on a (column block, row) grid; each of the four
+// bilinear texels is one 32-bit load.
 __global__ __launch_bounds__(256) void lut_render_kernel(const LutRenderArgs a) {
-    const size_t cell = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (cell >= a.n) return;
-    const int i = static_cast<int>(cell / a.np), j = static_cast<int>(cell % a.np);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= a.np) return;
+    const size_t cell = static_cast<size_t>(i) * a.np + j;
     const float4 rw = a.rows[i];
     const float2 cl = a.cols[j];
     const float nx = rw.x * cl.y, ny = rw.y, nz = rw.x * cl.x;
@@ -1016,11 +1024,14 @@ __global__ __launch_bounds__(256) void lut_render_kernel(const LutRenderArgs a) 
     const float fx = x - x0, fy = y - y0;
     const int i0 = min(max(static_cast<int>(x0), 0), kLutT - 1), i1 = min(max(static_cast<int>(x0) + 1, 0), kLutT - 1);
     const int j0 = min(max(static_cast<int>(y0), 0), kLutA - 1), j1 = min(max(static_cast<int>(y0) + 1, 0), kLutA - 1);
+    const uchar4 t00 = lut_texel4(a.lut, i0, j0), t10 = lut_texel4(a.lut, i1, j0);
+    const uchar4 t01 = lut_texel4(a.lut, i0, j1), t11 = lut_texel4(a.lut, i1, j1);
     uchar4 px;
     uint8_t ch[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float top = lut_texel(a.lut, i0, j0, c) * (1.f - fx) + lut_texel(a.lut, i1, j0, c) * fx;
-        const float bot = lut_texel(a.lut, i0, j1, c) * (1.f - fx) + lut_texel(a.lut, i1, j1, c) * fx;
+        const float top = lut_chan(t00, c) * (1.f - fx) + lut_chan(t10, c) * fx;
+        const float bot = lut_chan(t01, c) * (1.f - fx) + lut_chan(t11, c) * fx;
         const float val = fminf(fmaxf((top * (1.f - fy) + bot * fy) * a.intensity, 0.f), 1.f);
         ch[c] = static_cast<uint8_t>(val * 255.f + 0.5f);
     }
@@ -1067,8 +1078,8 @@ extern "C" int sf_render_lut_rgba8(sf_drainage* ctx, const uint8_t* lut_rgba8, d
                                     c.position[2] * c.position[2]);
         a.radius = static_cast<float>(d->p.bubble_radius);
         a.intensity = intensity;
-        hipLaunchKernelGGL(lut_render_kernel, dim3(static_cast<unsigned>((d->n + 255) / 256)), dim3(256), 0,
-                           d->stream, a);
+        hipLaunchKernelGGL(lut_render_kernel, dim3(static_cast<unsigned>((d->np + 255) / 256), d->nt), dim3(256),
+                           0, d->stream, a);
         SF_HIP(hipGetLastError());
         SF_HIP(hipMemcpyAsync(rgba_out, out.p, d->n * 4, hipMemcpyDeviceToHost, d->stream));
         SF_HIP(hipStreamSynchronize(d->stream));
