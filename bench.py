@@ -51,6 +51,7 @@ PMC_SUMMARY = {
     "c2": os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json"),
     "c2-drainage": os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json"),
     "c3": os.path.join(ROOT, "profiles", "foam_step_4096_bubbles_pmc.json"),
+    "frame": os.path.join(ROOT, "profiles", "caustics_render_2048x4096_pmc.json"),
 }
 
 
@@ -316,6 +317,7 @@ def bench_frame(args, L, W):
     ms = elapsed * 1e3 / steps
     algo = CAUSTIC_BYTES_PER_CELL * cells
     achieved = algo / (ms * 1e-3) / 1e9
+    traffic, _ = pmc_traffic("frame")
     frame = render_frame(ctx, camera=cam, refractive_index=1.33, intensity=4.0)
     reps = 10
     t0 = time.perf_counter()
@@ -329,7 +331,7 @@ def bench_frame(args, L, W):
         "config": {"workload": "caustic map and LUT-rendered RGBA8 frame of the single-bubble 2048x4096 "
                                "float-float thickness state, CausticParams defaults, camera (2.2R, 1.4R, 3.1R)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "caustics_vec4_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "caustics_vec4_kernel",
                      "avg_launch_ms": ms, "algorithmic_bytes_per_launch": algo,
                      "timing": "wall clock over back-to-back launches on the context's stream"},
         "render": {"ms_per_frame_incl_lut_and_copy": render_ms,
