@@ -66,9 +66,12 @@ __device__ inline float caustic_cell(const CausticArgs& a, float c, float l, flo
     const float dxy = (dr - dl - ur + ul) / 4.0f;
     const float det = d2x * d2y - dxy * dxy;
     const float curv = -lap * a.focal;
-    const float focusing = 1.0f / fmaxf(fabsf(1.0f - curv), 0.01f);
+    // v_rcp_f32 and v_sqrt_f32 (about one ulp) in place of the correctly
+    // rounded library forms: their error is below the power's, and the
+    // branch decision does not depend on them.
+    const float focusing = __builtin_amdgcn_rcpf(fmaxf(fabsf(1.0f - curv), 0.01f));
     const float branch = det < -a.threshold ? 1.5f : 1.0f;
-    const float refraction = 1.0f + sqrtf(gx * gx + gy * gy) * 0.5f;
+    const float refraction = 1.0f + __builtin_amdgcn_sqrtf(gx * gx + gy * gy) * 0.5f;
     float I = focusing * branch * refraction;
     I = caustic_pow(I, a.sharpness) * a.intensity;
     return fminf(fmaxf(I, 0.0f), 10.0f);
@@ -85,20 +88,21 @@ __global__ __launch_bounds__(256) void caustics_kernel(const CausticArgs a) {
                      th_at(a, x + 1, y + 1));
 }
 
-// Thickness of four consecutive cells of row y from column c (c % 4 == 0).
-__device__ inline float4 th4_at(const CausticArgs& a, int y, int c) {
-    const size_t k = static_cast<size_t>(y) * a.np + c;
-    float4 t = *reinterpret_cast<const float4*>(a.hi + k);
-    if (a.lo) {
-        const float4 l = *reinterpret_cast<const float4*>(a.lo + k);
+// Thickness of four consecutive cells from column c (c % 4 == 0) of a row
+// whose hi and lo start at rh and rl (rl null: thickness = hi); the row
+// pointers are wave-uniform, so the loads take a scalar base and a 32-bit
+// lane offset.
+__device__ inline float4 th4_row(const float* rh, const float* rl, int c) {
+    float4 t = *reinterpret_cast<const float4*>(rh + c);
+    if (rl) {
+        const float4 l = *reinterpret_cast<const float4*>(rl + c);
         t.x += l.x; t.y += l.y; t.z += l.z; t.w += l.w;
     }
     return t;
 }
 
-__device__ inline float th1_at(const CausticArgs& a, int y, int c) {
-    const size_t k = static_cast<size_t>(y) * a.np + c;
-    return a.lo ? a.hi[k] + a.lo[k] : a.hi[k];
+__device__ inline float th1_row(const float* rh, const float* rl, int c) {
+    return rl ? rh[c] + rl[c] : rh[c];
 }
 
 // Widths divisible by 4: a lane owns four consecutive cells of one row and
@@ -126,11 +130,15 @@ __global__ __launch_bounds__(256) void caustics_vec4_kernel(const CausticArgs a)
     const bool load_w = lane == 0;
     const bool load_e = lane == 63 || cc + 4 >= a.np;
     const int yu = max(y - 1, 0), yd = min(y + 1, a.nt - 1);
-    const float4 U = th4_at(a, yu, cc), M = th4_at(a, y, cc), D = th4_at(a, yd, cc);
+    const size_t np = static_cast<size_t>(a.np);
+    const float *hu = a.hi + yu * np, *hm = a.hi + y * np, *hd = a.hi + yd * np;
+    const float *lu = a.lo ? a.lo + yu * np : nullptr, *lm = a.lo ? a.lo + y * np : nullptr,
+                *ld = a.lo ? a.lo + yd * np : nullptr;
+    const float4 U = th4_row(hu, lu, cc), M = th4_row(hm, lm, cc), D = th4_row(hd, ld, cc);
     float uw = __shfl_up(U.w, 1), mw = __shfl_up(M.w, 1), dw = __shfl_up(D.w, 1);
     float ue = __shfl_down(U.x, 1), me = __shfl_down(M.x, 1), de = __shfl_down(D.x, 1);
-    if (load_w) { uw = th1_at(a, yu, cw); mw = th1_at(a, y, cw); dw = th1_at(a, yd, cw); }
-    if (load_e) { ue = th1_at(a, yu, ce); me = th1_at(a, y, ce); de = th1_at(a, yd, ce); }
+    if (load_w) { uw = th1_row(hu, lu, cw); mw = th1_row(hm, lm, cw); dw = th1_row(hd, ld, cw); }
+    if (load_e) { ue = th1_row(hu, lu, ce); me = th1_row(hm, lm, ce); de = th1_row(hd, ld, ce); }
     if (!valid) return;
     float4 o;
     o.x = caustic_cell(a, M.x, mw, M.y, U.x, D.x, uw, U.y, dw, D.y);

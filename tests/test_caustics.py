@@ -1,7 +1,7 @@
 """Caustic-intensity stencil (caustics_compute.wgsl:27-126): the CPU
 restatement is pinned by closed forms, then the HIP kernel is checked against
 it (branch mask bit-exact, intensity within 2e-6 relative: the kernel's power
-runs on the hardware log2/exp2, measured within 5e-7 of the FP64 pow)."""
+runs on the hardware log2/exp2, measured within 7e-7 of the FP64 pow)."""
 import numpy as np
 import pytest
 
