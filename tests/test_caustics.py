@@ -97,3 +97,25 @@ def test_gpu_caustics_sharpness(sharpness):
     assert np.allclose(m, mo, rtol=2e-6, atol=0)
     if sharpness == 0.0:
         assert np.all(m == np.float32(p.caustic_intensity))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(40, 1028), (33, 70), (16, 4)])
+def test_gpu_caustics_of_a_context_with_gradients(shape):
+    # O(1) thickness values on a drainage context: the float-float state's
+    # hi + lo feeds the stencil; widths with a partly filled last wave
+    # (1028), with no float4 path (70) and a single lane (4)
+    from soap_bubble_simulation_amd import _lib as L
+    from soap_bubble_simulation_amd.drainage import DrainageContext
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    nt, npf = shape
+    ctx = DrainageContext(p, nt, npf, 500e-9)
+    rng = np.random.default_rng(nt + npf)
+    h0 = rng.uniform(0.2, 2.0, shape)
+    ctx.set_fields(h0, np.full_like(h0, 0.5))
+    m = ctx.caustics()
+    h, _ = ctx.fields()
+    mo, bo = O.caustics(h.astype(np.float32))
+    assert np.allclose(m, mo, rtol=2e-6, atol=0)
+    assert bo.any() and (~bo.astype(bool)).any()
