@@ -995,7 +995,7 @@ struct LutRenderArgs {
 };
 
 __device__ inline uchar4 lut_texel4(const uint8_t* lut, int i, int j) {
-    return reinterpret_cast<const uchar4*>(lut)[static_cast<size_t>(j) * kLutT + i];
+    return reinterpret_cast<const uchar4*>(lut)[j * kLutT + i];      // 32-bit offset from the LUT base
 }
 
 __device__ inline float lut_chan(uchar4 t, int c) {
@@ -1009,14 +1009,17 @@ __global__ __launch_bounds__(256) void lut_render_kernel(const LutRenderArgs a) 
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     if (j >= a.np) return;
-    const size_t cell = static_cast<size_t>(i) * a.np + j;
+    // row bases are uniform in the block: the per-lane offset stays 32-bit
+    const size_t row = static_cast<size_t>(i) * a.np;
+    const float* rh = a.hi + row;
+    const float* rl = a.lo + row;
     const float4 rw = a.rows[i];
     const float2 cl = a.cols[j];
     const float nx = rw.x * cl.y, ny = rw.y, nz = rw.x * cl.x;
     const float cn = a.cx * nx + a.cy * ny + a.cz * nz;
     const float len2 = a.cam2 - 2.f * a.radius * cn + a.radius * a.radius;
     const float cos_v = fminf(fabsf(cn - a.radius) * rsqrtf(len2), 1.f);
-    const float t_nm = (a.hi[cell] + a.lo[cell]) * 1e9f;
+    const float t_nm = (rh[j] + rl[j]) * 1e9f;
     const float u = fminf(fmaxf(t_nm / kLutMaxNm, 0.f), 1.f);
     const float v = fminf(fmaxf(cos_v, 0.f), 1.f);
     const float x = u * static_cast<float>(kLutT) - 0.5f, y = v * static_cast<float>(kLutA) - 0.5f;
@@ -1039,7 +1042,7 @@ __global__ __launch_bounds__(256) void lut_render_kernel(const LutRenderArgs a) 
     px.y = ch[1];
     px.z = ch[2];
     px.w = 255;
-    reinterpret_cast<uchar4*>(a.out)[cell] = px;
+    (reinterpret_cast<uchar4*>(a.out) + row)[j] = px;
 }
 
 extern "C" int sf_render_lut_rgba8(sf_drainage* ctx, const uint8_t* lut_rgba8, double refractive_index,
