@@ -539,6 +539,13 @@ int sf_fno_get_scales(sf_fno* model, double* scales);
 int sf_fno_set_scales(sf_fno* model, const double* scales);
 /* Host-array forward pass: in[batch][N][5] -> out[batch][N][2]. */
 int sf_fno_forward(sf_fno* model, int32_t batch, const float* in, float* out);
+/* Kernel paths of this model's shape: the fused per-row layer kernel (training
+ * and inference), the lift folded into the first phi DFT and the fused theta
+ * stage (inference only); shapes outside them run the general batched GEMMs. */
+#define SF_FNO_PATH_FUSED_ROWS   (1 << 0)
+#define SF_FNO_PATH_FUSED_THETA  (1 << 1)
+#define SF_FNO_PATH_LIFT_DFT     (1 << 2)
+int sf_fno_kernel_paths(sf_fno* model, int32_t* paths);
 /* Loss mean((out - target)^2) and its gradient w.r.t. every parameter. */
 int sf_fno_gradient(sf_fno* model, int32_t batch, const float* in, const float* target, double* loss,
                     float* grad);

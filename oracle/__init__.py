@@ -104,6 +104,7 @@ def lib():
             "orc_interference_color": (None, [C.c_double, dp, C.c_double, C.c_double, C.c_int, dp]),
             "orc_lut_1d": (None, [C.c_double, dp, C.c_double, C.c_double, C.c_int, C.c_double, dp]),
             "orc_lut_2d": (None, [C.c_double, dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp]),
+            "orc_lut_airy_terms": (None, [C.c_float, C.c_float, C.c_float, fp, fp, fp, fp]),
             "orc_lut_thin_film": (None, [C.c_float, C.c_float, C.c_float, C.c_float, fp]),
             "orc_lut_rgb_f32": (None, [C.c_float, C.c_float, fp]),
             "orc_lut_rgba8": (None, [C.c_float, C.c_float, u8p]),
@@ -273,6 +274,15 @@ def lut_2d(n_film, tmin, tmax, nt, na, gamma=True, lambdas=RGB_NM):
     out = np.zeros((nt, na, 3), np.float64)
     lib().orc_lut_2d(n_film, _dp(lam), tmin, tmax, nt, na, int(gamma), _dp(out))
     return out
+
+
+def lut_airy_terms(t_nm, cos_theta, n_film):
+    """Reflectance stage of the runtime LUT at one (thickness, angle):
+    (cos_t, R, F, a[7]) in FP32, a[k] the Airy reflectance at 400..700 nm."""
+    ct, r, f = C.c_float(), C.c_float(), C.c_float()
+    a = (C.c_float * 7)()
+    lib().orc_lut_airy_terms(t_nm, cos_theta, n_film, C.byref(ct), C.byref(r), C.byref(f), a)
+    return np.float32(ct.value), np.float32(r.value), np.float32(f.value), np.array(a[:], np.float32)
 
 
 def lut_thin_film(t_nm, cos_theta, n_film, intensity=1.0):

@@ -104,6 +104,8 @@ void   orc_lut_2d(double n_film, const double lambdas[3], double tmin, double tm
                   int nt, int na, int gamma, double* out /* nt*na*3 */);
 
 /* ---- FP32 runtime LUT (interference_lut.rs) ------------------------------ */
+void   orc_lut_airy_terms(float t_nm, float cos_theta, float n_film, float* cos_t, float* r, float* f,
+                          float a[7]);
 void   orc_lut_thin_film(float t_nm, float cos_theta, float n_film, float intensity, float rgb[3]);
 void   orc_lut_rgb_f32(float n_film, float intensity, float* out /* 64*256*3 */);
 void   orc_lut_rgba8(float n_film, float intensity, uint8_t* out /* 64*256*4 */);

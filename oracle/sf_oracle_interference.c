@@ -182,22 +182,40 @@ static void lut_cie(float w, float out[3])
     out[2] = maxf_(z, 0.0f);
 }
 
-/* thin_film_interference (interference_lut.rs:45-93) + xyz_to_rgb (158-163) */
-void orc_lut_thin_film(float t_nm, float cos_theta, float n_film, float intensity, float rgb[3])
+static const float LUT_W[7] = {400.0f, 450.0f, 500.0f, 550.0f, 600.0f, 650.0f, 700.0f};
+
+/* The reflectance stage of thin_film_interference (interference_lut.rs:54-72):
+ * refraction cosine, unpolarised Fresnel reflectance R, the coefficient of
+ * finesse F of airy_interference and the Airy reflectance a[k] at the seven
+ * LUT wavelengths. */
+void orc_lut_airy_terms(float t_nm, float cos_theta, float n_film, float* cos_t_out, float* r_out,
+                        float* f_out, float a[7])
 {
-    static const float W[7] = {400.0f, 450.0f, 500.0f, 550.0f, 600.0f, 650.0f, 700.0f};
     float cos_t = lut_snell(cos_theta, n_film);
     float fres = lut_fresnel(cos_theta, cos_t, 1.0f, n_film);
     float optical_path = 2.0f * n_film * t_nm * cos_t;
+    for (int k = 0; k < 7; ++k) {
+        float phase = 2.0f * PI_F * optical_path / LUT_W[k] + PI_F;
+        a[k] = lut_airy(phase, fres);
+    }
+    float one_minus_r = maxf_(1.0f - fres, 0.001f);
+    if (cos_t_out) *cos_t_out = cos_t;
+    if (r_out) *r_out = fres;
+    if (f_out) *f_out = 4.0f * fres / (one_minus_r * one_minus_r);
+}
+
+/* thin_film_interference (interference_lut.rs:45-93) + xyz_to_rgb (158-163) */
+void orc_lut_thin_film(float t_nm, float cos_theta, float n_film, float intensity, float rgb[3])
+{
+    float a[7];
+    orc_lut_airy_terms(t_nm, cos_theta, n_film, 0, 0, 0, a);
     float xyz[3] = {0.0f, 0.0f, 0.0f};
     for (int k = 0; k < 7; ++k) {
-        float phase = 2.0f * PI_F * optical_path / W[k] + PI_F;
-        float a = lut_airy(phase, fres);
         float cie[3];
-        lut_cie(W[k], cie);
-        xyz[0] += cie[0] * a;
-        xyz[1] += cie[1] * a;
-        xyz[2] += cie[2] * a;
+        lut_cie(LUT_W[k], cie);
+        xyz[0] += cie[0] * a[k];
+        xyz[1] += cie[1] * a[k];
+        xyz[2] += cie[2] * a[k];
     }
     xyz[0] /= 7.0f;
     xyz[1] /= 7.0f;

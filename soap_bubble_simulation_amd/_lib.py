@@ -31,6 +31,9 @@ SPECTRAL_OPERAND_FP16 = 2
 PREDICT_NETWORK = 0
 PREDICT_IDENTITY = 1
 PREDICT_EXTRAPOLATE = 2
+FNO_PATH_FUSED_ROWS = 1 << 0
+FNO_PATH_FUSED_THETA = 1 << 1
+FNO_PATH_LIFT_DFT = 1 << 2
 
 
 class SoapFilmError(RuntimeError):
@@ -248,6 +251,7 @@ SIGNATURES = {
     "sf_fno_set_scales": (C.c_int, [_vp, _dp]),
     "sf_fno_forward": (C.c_int, [_vp, C.c_int32, _fp, _fp]),
     "sf_fno_gradient": (C.c_int, [_vp, C.c_int32, _fp, _fp, _dp, _fp]),
+    "sf_fno_kernel_paths": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_fno_train_step": (C.c_int, [_vp, C.c_int32, _fp, _fp, C.c_double, _dp]),
     "sf_fno_train_foam": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_double, _dp]),
     "sf_fno_advance_foam": (C.c_int, [_vp, _vp, C.POINTER(Params), C.c_int32, C.c_double, C.c_int32,

@@ -556,10 +556,27 @@ struct Fno {
 
     // Inference theta stage of layer l: the fused kernel where its shape
     // constraints and LDS tile allow, else the general sequence.
+    size_t theta_fused_lds() const {
+        return static_cast<size_t>(2 * C) * (T + 8) * 2 + static_cast<size_t>(2 * Mt) * (2 * C + 1) * 4 +
+               static_cast<size_t>(C) * (2 * Mt + 8) * 2;
+    }
+    bool theta_fused_ok() const { return !(C % 16 || (2 * Mt) % 16 || T % 16 || theta_fused_lds() > 64 * 1024); }
+
+    // Kernel paths the forward passes take at this shape (SF_FNO_PATH_* bits).
+    int paths() const {
+        int row_threads = 0, p = 0;
+        if (row_kernel(&row_threads)) {
+            p |= SF_FNO_PATH_FUSED_ROWS;
+            if (static_cast<size_t>(P * kCin + 2 * Mp * (kCin + 1)) * sizeof(float) <= 64 * 1024 && P <= kLiftDftMaxP)
+                p |= SF_FNO_PATH_LIFT_DFT;
+            if (theta_fused_ok()) p |= SF_FNO_PATH_FUSED_THETA;
+        }
+        return p;
+    }
+
     void theta_stage_inference(int l, int B) {
-        const size_t lds = static_cast<size_t>(2 * C) * (T + 8) * 2 + static_cast<size_t>(2 * Mt) * (2 * C + 1) * 4 +
-                           static_cast<size_t>(C) * (2 * Mt + 8) * 2;
-        if (C % 16 || (2 * Mt) % 16 || T % 16 || lds > 64 * 1024) {
+        const size_t lds = theta_fused_lds();
+        if (!theta_fused_ok()) {
             theta_stage(l, B);
             return;
         }
@@ -1080,6 +1097,13 @@ extern "C" int sf_fno_forward(sf_fno* m, int32_t batch, const float* in, float* 
         f->forward(batch, false);
         SF_HIP(hipMemcpyAsync(out, f->out, R * kCout * 4, hipMemcpyDeviceToHost, f->stream));
         SF_HIP(hipStreamSynchronize(f->stream));
+    });
+}
+
+extern "C" int sf_fno_kernel_paths(sf_fno* m, int32_t* paths) {
+    return guard([&] {
+        SF_REQUIRE(m && paths, SF_ERR_INVALID, "NULL argument");
+        *paths = FN(m)->paths();
     });
 }
 

@@ -80,6 +80,12 @@ class FnoSurrogate:
         L.check(self._lib.sf_fno_forward(self._h, b, L.fptr(x), L.fptr(out)))
         return out
 
+    def kernel_paths(self) -> int:
+        """FNO_PATH_* bits: which fused kernels this shape's passes run."""
+        p = C.c_int32()
+        L.check(self._lib.sf_fno_kernel_paths(self._h, C.byref(p)))
+        return p.value
+
     def gradient(self, x: np.ndarray, target: np.ndarray):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(target, np.float32)

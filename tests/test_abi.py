@@ -97,6 +97,20 @@ def test_cpp_api_compiles():
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.gpu
+def test_cpp_api_runs_reference_unit_tests():
+    """The C++ mirror of drainage.rs:616-720, interference.rs:612-792 and
+    interference_lut.rs:165-205 (tests/cpp/test_api.cpp), built against
+    libsoapfilm.so through soapfilm.hpp, runs on the GPU and prints OK."""
+    from soap_bubble_simulation_amd import build as B
+    if not os.path.exists(B.CPP_TEST_BIN):
+        B.build_cpp_test()
+    r = subprocess.run([B.CPP_TEST_BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "OK"
+
+
 def test_foam_rejects_unsupported_flags():
     p = L.Params()
     L.check(L.lib().sf_params_cpu_default(p))      # carries SF_FLAG_TFE

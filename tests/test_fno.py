@@ -273,3 +273,80 @@ def test_judge_accepts_and_tracks_plain_solver(trained):
     print("accepted %d / %d, rel-L2 h %.3e, increment %.3e" % (
         st.accepted, st.bubble_windows, err, np.linalg.norm(h_s - h_r) / np.linalg.norm(h_r - h_start)))
     assert err <= 1e-2          # <= 1 % relative L2 on h
+
+
+# ---- the benchmark network shape (256 x 512, width 32, 16 x 16 modes, 4 layers) ----
+
+C4_CFG = dict(n_theta=256, n_phi=512, width=32, modes_theta=16, modes_phi=16, layers=4, window=16, max_batch=2)
+
+
+def scaled_params(s, seed, spectral_std=0.1):
+    """Parameters at fan-in scales, spectral weights large enough that the
+    spectral path carries about as much of the output as the pointwise one."""
+    rng = np.random.default_rng(seed)
+    C = s.C
+    parts = [rng.normal(0, 1 / np.sqrt(R.N_IN), R.N_IN * C), rng.normal(0, 0.1, C)]
+    for _ in range(s.L):
+        parts += [rng.normal(0, 1 / np.sqrt(C), C * C), rng.normal(0, 0.1, C),
+                  rng.normal(0, spectral_std, s.Mt * s.Mp * 2 * C * C)]
+    parts += [rng.normal(0, 1 / np.sqrt(C), C * R.N_OUT), rng.normal(0, 0.1, R.N_OUT)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch", [1, 2])
+def test_gpu_benchmark_shape_matches_reference(batch):
+    """The network of the surrogate benchmarks: inference forward (lift
+    folded into the first phi DFT, fused theta stage at 32 theta modes,
+    fused row kernel at n_phi 512 and width 32) and the training forward +
+    backward, against the FP64 reference."""
+    from soap_bubble_simulation_amd import _lib as L
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+    m = FnoSurrogate(**C4_CFG)
+    paths = m.kernel_paths()
+    assert paths == L.FNO_PATH_FUSED_ROWS | L.FNO_PATH_FUSED_THETA | L.FNO_PATH_LIFT_DFT, paths
+    s = R.Shapes.of(m.config)
+    f32 = scaled_params(s, seed=20 + batch)
+    m.set_params(f32)
+    rng = np.random.default_rng(30 + batch)
+    x32 = rng.normal(0, 1, (batch, s.T, s.P, R.N_IN)).astype(np.float32)
+    t32 = rng.normal(0, 0.3, (batch, s.T, s.P, R.N_OUT)).astype(np.float32)
+    out = m.forward(x32)
+    ref, _, _ = R.forward(f32.astype(np.float64), x32.astype(np.float64), s)
+    print("benchmark shape, batch %d: forward rel-L2 %.2e" % (batch, rel(out, ref)))
+    assert rel(out, ref) <= 2e-2, rel(out, ref)
+    loss, g = m.gradient(x32, t32)
+    rl, rg, _ = R.loss_and_grad(f32.astype(np.float64), x32.astype(np.float64), t32.astype(np.float64), s)
+    assert abs(loss - rl) <= 1e-2 * rl
+    worst = 0.0
+    for name, sl in group_slices(s):
+        worst = max(worst, rel(g[sl], rg[sl]))
+        assert rel(g[sl], rg[sl]) <= 2e-2, (name, rel(g[sl], rg[sl]))
+    print("benchmark shape, batch %d: worst gradient group rel-L2 %.2e" % (batch, worst))
+    m.close()
+
+
+@pytest.mark.gpu
+def test_judged_foam_at_benchmark_shape():
+    """The batched judge at the surrogate benchmark's network and grid
+    (64 bubbles of 256 x 512, four 16-bubble chunks): every bubble-window is
+    either accepted or falls back, and with a tolerance no prediction meets
+    the foam ends bit-identical to the plain solver."""
+    from soap_bubble_simulation_amd import _lib as L
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+    cfg = dict(C4_CFG, max_batch=16)
+    m = FnoSurrogate(**cfg)
+    m.set_scales(0.05, 1e-3)
+    grid = dict(n_theta=256, n_phi=512)
+    fm, ref = small_foam(64, 77, **grid), small_foam(64, 77, **grid)
+    st = m.advance_foam(fm, 2, tol=1e-30, predictor=L.PREDICT_NETWORK)
+    ref.step(2 * cfg["window"])
+    assert st.windows == 2 and st.bubble_windows == 128
+    assert st.fallbacks == 128 and st.accepted == 0
+    assert np.array_equal(fm.fields()[0], ref.fields()[0])
+    st = m.advance_foam(fm, 2, tol=1e-3, predictor=L.PREDICT_IDENTITY)
+    assert st.accepted + st.fallbacks == st.bubble_windows == 128
+    assert st.max_residual >= 0.0
+    fm.close()
+    ref.close()
+    m.close()

@@ -213,3 +213,55 @@ def test_foam_set_state_roundtrip():
     assert np.array_equal(fm.bubble_dt(), dt0)
     rb = fm.resolved_bubbles()
     assert [b.dt for b in rb] == list(dt0)
+
+
+def test_full_size_foam_closed_form():
+    """The 4096-bubble benchmark foam (latitudes log-uniform 32-512, 3.8e8
+    cells, 9 GB of state) with h diffusion off and uniform films: every row
+    of every bubble follows the forward-Euler recursion of the gravity sink
+    h <- h - dt c h^3 sin(theta_i) (poles: ring means), rows stay uniform
+    along phi bit for bit, and the fused FP64 film volume and CFL estimate
+    equal their closed forms on those rows."""
+    desc = W.foam_batch(4096, seed=3)
+    p = foam_params()
+    p.diffusion_coeff = 0.0
+    rho_g = p.density * p.gravity
+    bubbles = []
+    for d in desc:
+        c = rho_g / (3.0 * d["viscosity"])
+        # 0.2 % relative thinning per step at the equator
+        bubbles.append(FoamBubble(d["n_theta"], d["n_phi"], d["bubble_radius"], d["viscosity"], d["h0"],
+                                  dt=2e-3 / (c * d["h0"] ** 2)))
+    fm = FoamDrainage(p, bubbles)
+    assert fm.info()["cells"] > 3.5e8
+    steps = 48
+    fm.step(steps)
+    h, g = fm.fields()
+    st = fm.stats()
+    D = max(p.diffusion_coeff, p.surfactant_diffusion)
+    worst_h = worst_m = worst_c = 0.0
+    for k, (b, hk, gk) in enumerate(zip(bubbles, fm.split(h), fm.split(g))):
+        nt, npf = b.n_theta, b.n_phi
+        assert np.all(hk == hk[:, :1]), k
+        assert np.all(gk == gk[:, :1]), k
+        th = np.arange(nt) * np.pi / (nt - 1)
+        c = rho_g / (3.0 * b.viscosity)
+        row = np.full(nt, b.h0)
+        for _ in range(steps):
+            row = row - b.dt * c * row ** 3 * np.sin(th)
+            row[0], row[-1] = row[1], row[-2]
+        worst_h = max(worst_h, float(np.max(np.abs(hk[:, 0] / row - 1.0))))
+        dth, dph = np.pi / (nt - 1), 2 * np.pi / npf
+        lo_b = np.where(np.arange(nt) == 0, 0.0, th - dth / 2)
+        hi_b = np.where(np.arange(nt) == nt - 1, np.pi, th + dth / 2)
+        area = b.bubble_radius ** 2 * (np.cos(lo_b) - np.cos(hi_b)) * dph
+        mass = float(np.sum(hk[:, 0] * npf * area))
+        worst_m = max(worst_m, abs(st["mass"][k] / mass - 1.0))
+        s = np.sin(th[1:-1])
+        lam = 4 * D * (1 / dth ** 2 + 1 / (s * s * dph * dph)) / b.bubble_radius ** 2 + 3 * c * hk[1:-1, 0] ** 2 * s
+        worst_c = max(worst_c, abs(st["dt_cfl"][k] / (2.0 / np.max(lam)) - 1.0))
+    print("4096-bubble foam: max rel dev of rows %.2e, mass %.2e, CFL %.2e" % (worst_h, worst_m, worst_c))
+    assert worst_h <= 1e-7
+    assert worst_m <= 1e-12
+    assert worst_c <= 1e-12
+    fm.close()
