@@ -150,6 +150,22 @@ int sf_drainage_get_thickness_batch(sf_drainage* ctx, int32_t n, const double* t
 
 /* min/max (drainage.rs:180-193), FP64 mass and CFL, rupture count, time. */
 int sf_drainage_stats(sf_drainage* ctx, sf_stats* out);
+/* Allocation-free statistics of a caller-held device field.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * h = h_hi + h_lo (float-float; h_lo may be NULL for an FP32 field), row-major
+ * [n_theta][n_phi] on the current device, with the grid convention and the
+ * params of sf_drainage_create.  Writes *out — device memory — with what
+ * sf_drainage_stats reports for such a field (min, max, FP64 film volume,
+ * FP64 CFL step, rupture count, has_critical_region; time, fronts and steps
+ * 0), asynchronously on `stream` (NULL: the null stream).  Calls no device
+ * allocator: its row tables and partial sums live in one fixed library array
+ * per device (about 0.6 MB, held until the process ends), so n_theta is at
+ * most SF_NOALLOC_MAX_THETA (larger grids fail with SF_ERR_INVALID), and
+ * calls that overlap on one device race on that array. */
+#define SF_NOALLOC_MAX_THETA 16384
+int sf_field_stats_noalloc(const sf_params* params, int32_t n_theta, int32_t n_phi, const float* h_hi,
+                           const float* h_lo, void* stream, sf_stats* out);
 
 /* DrainageSimulator::reset (drainage.rs:575-583) / GPUDrainageSimulator::reset
  * (gpu_drainage.rs:415-432): uniform h0 and gamma0, time 0, fronts cleared. */

@@ -31,6 +31,7 @@ SPECTRAL_OPERAND_FP16 = 2
 PREDICT_NETWORK = 0
 PREDICT_IDENTITY = 1
 PREDICT_EXTRAPOLATE = 2
+NOALLOC_MAX_THETA = 16384
 FNO_PATH_FUSED_ROWS = 1 << 0
 FNO_PATH_FUSED_THETA = 1 << 1
 FNO_PATH_LIFT_DFT = 1 << 2
@@ -185,6 +186,7 @@ SIGNATURES = {
     "sf_drainage_get_thickness": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
     "sf_drainage_get_thickness_batch": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp]),
     "sf_drainage_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
+    "sf_field_stats_noalloc": (C.c_int, [C.POINTER(Params), C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]),
     "sf_drainage_reset": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sf_drainage_set_marginal_regeneration": (C.c_int, [_vp, C.c_int32]),
     "sf_drainage_set_tfe_thickness_ratio": (C.c_int, [_vp, C.c_double]),

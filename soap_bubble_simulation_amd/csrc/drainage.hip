@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -47,7 +48,9 @@ struct StatsPart {
     unsigned long long pad;
 };
 
-// Per-block min / max / FP64 mass / FP64 CFL lambda / rupture count.
+// Per-block min / max / FP64 mass / FP64 CFL lambda / rupture count of
+// h = hi + lo (lo may be null: an FP32 field).  Every block writes its own
+// slot of `part`, so the partials need no clearing between calls.
 __global__ __launch_bounds__(256) void stats_partial_kernel(const float* hi, const float* lo,
                                                             const RowD* rowd, int nt, int np,
                                                             float hc0, float hc1, float hc2,
@@ -58,7 +61,7 @@ __global__ __launch_bounds__(256) void stats_partial_kernel(const float* hi, con
     for (size_t c = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; c < n;
          c += static_cast<size_t>(gridDim.x) * blockDim.x) {
         const int i = static_cast<int>(c / np);
-        const float h = hi[c], l = lo[c];
+        const float h = hi[c], l = lo ? lo[c] : 0.f;
         const double v = static_cast<double>(h) + static_cast<double>(l);
         mn = fmin(mn, v);
         mx = fmax(mx, v);
@@ -1173,5 +1176,120 @@ extern "C" int sf_spectral_get_operand(sf_drainage* ctx, int32_t* kind) {
             *kind = (d->cmf_f16_range && d->cmf_f16_bound) ? SF_SPECTRAL_OPERAND_FP16 : SF_SPECTRAL_OPERAND_BF16_PAIR;
         else
             *kind = d->spec_operand;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free field statistics (sf_field_stats_noalloc).
+//
+// The kernels of sf_drainage_stats, run on a caller-held device field with
+// their per-row FP64 tables and per-block partials in `scratch`, a fixed
+// array that every device the library runs on holds until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address comes from hipGetSymbolAddress, once per device, and only
+// that address is handed to kernels.
+
+namespace {
+
+constexpr int kNoallocMaxTheta = SF_NOALLOC_MAX_THETA;
+constexpr size_t kNoallocRowBytes = static_cast<size_t>(kNoallocMaxTheta) * sizeof(RowD);
+constexpr size_t kNoallocPartBytes = Drainage::kStatsBlocks * sizeof(StatsPart);
+constexpr size_t kNoallocBytes = kNoallocRowBytes + kNoallocPartBytes + sizeof(StatsPart);
+static_assert(kNoallocRowBytes % 256 == 0 && kNoallocPartBytes % 256 == 0, "scratch regions stay aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call there).
+unsigned char* device_scratch() {
+    int dev = 0;
+    SF_HIP(hipGetDevice(&dev));
+    SF_REQUIRE(dev >= 0 && dev < kMaxDevices, SF_ERR_DEVICE, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SF_REQUIRE(p != nullptr, SF_ERR_DEVICE, "no device address for the statistics scratch array");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// The FP64 row table of Drainage::upload_tables (cell area, CFL grid term,
+// |sin theta|), one thread per row.
+__global__ void noalloc_rows_kernel(RowD* rowd, int nt, double dtheta, double dphi, double R2, double D,
+                                    double sin_guard) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const double theta = static_cast<double>(i) * dtheta;
+    const double s = sin(theta);
+    const double ss = fabs(s) < sin_guard ? copysign(sin_guard, s) : s;
+    const double lo_b = i == 0 ? 0.0 : theta - 0.5 * dtheta;
+    const double hi_b = i == nt - 1 ? kPi : theta + 0.5 * dtheta;
+    RowD r;
+    r.area = R2 * (cos(lo_b) - cos(hi_b)) * dphi;
+    r.lam_grid = 4.0 * D * (1.0 / (dtheta * dtheta) + 1.0 / (ss * ss * dphi * dphi)) / R2;
+    r.sin_abs = fabs(s);
+    r.pad = 0.0;
+    rowd[i] = r;
+}
+
+// sf_stats of the folded partials, as sf_drainage_stats fills it for a field
+// with no context (time, fronts and steps 0).
+__global__ void noalloc_emit_kernel(const StatsPart* r, double h_crit, sf_stats* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const StatsPart s = *r;
+    sf_stats o;
+    o.min_thickness = s.mn;
+    o.max_thickness = s.mx;
+    o.mass = s.mass;
+    o.dt_cfl = s.lam > 0.0 ? 2.0 / s.lam : INFINITY;
+    o.current_time = 0.0;
+    o.n_ruptured = static_cast<int64_t>(s.nrupt);
+    o.n_tfe = 0;
+    o.has_critical_region = s.mn < h_crit ? 1 : 0;
+    o.steps = 0;
+    *out = o;
+}
+
+}  // namespace
+
+extern "C" int sf_field_stats_noalloc(const sf_params* params, int32_t n_theta, int32_t n_phi, const float* h_hi,
+                                      const float* h_lo, void* stream, sf_stats* out) {
+    return guard([&] {
+        check_params(params);
+        SF_REQUIRE(h_hi != nullptr && out != nullptr, SF_ERR_INVALID, "NULL argument");
+        SF_REQUIRE(n_theta >= 3 && n_phi >= 1, SF_ERR_INVALID, "grid %d x %d: need n_theta >= 3 and n_phi >= 1",
+                   n_theta, n_phi);
+        SF_REQUIRE(n_theta <= kNoallocMaxTheta, SF_ERR_INVALID,
+                   "n_theta %d exceeds SF_NOALLOC_MAX_THETA (%d), the row capacity of the fixed statistics scratch",
+                   n_theta, kNoallocMaxTheta);
+        SF_REQUIRE(static_cast<int64_t>(n_theta) * n_phi < (int64_t(1) << 40), SF_ERR_INVALID, "grid too large");
+        require_device();
+        unsigned char* base = device_scratch();
+        RowD* rowd = reinterpret_cast<RowD*>(base);
+        StatsPart* part = reinterpret_cast<StatsPart*>(base + kNoallocRowBytes);
+        StatsPart* folded = reinterpret_cast<StatsPart*>(base + kNoallocRowBytes + kNoallocPartBytes);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const double dtheta = kPi / static_cast<double>(n_theta - 1);
+        const double dphi = 2.0 * kPi / static_cast<double>(n_phi);
+        double D = params->diffusion_coeff;
+        if ((params->flags & SF_FLAG_SURFACTANT) && params->surfactant_diffusion > D) D = params->surfactant_diffusion;
+        hipLaunchKernelGGL(noalloc_rows_kernel, dim3(cdiv(n_theta, 256)), dim3(256), 0, s, rowd, n_theta, dtheta,
+                           dphi, params->bubble_radius * params->bubble_radius, D, params->sin_guard);
+        const size_t n = static_cast<size_t>(n_theta) * n_phi;
+        const size_t blocks = std::min<size_t>(Drainage::kStatsBlocks, std::max<size_t>(1, cdiv(n, 256)));
+        const Float3 hc = split3(params->critical_thickness);
+        const double coef3 = 3.0 * (params->density * params->gravity / (3.0 * params->viscosity));
+        hipLaunchKernelGGL(stats_partial_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, h_hi, h_lo,
+                           rowd, n_theta, n_phi, hc.c0, hc.c1, hc.c2, coef3, part);
+        hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(64), 0, s, part, static_cast<int>(blocks), folded);
+        hipLaunchKernelGGL(noalloc_emit_kernel, dim3(1), dim3(64), 0, s, folded, params->critical_thickness, out);
+        SF_HIP(hipGetLastError());
     });
 }

@@ -1,7 +1,8 @@
 """C ABI checks that need no GPU: the library loads, exports every entry
 point declared in include/soapfilm.h, the ctypes mirrors match the C struct
 layouts, defaults match the reference, and compute entry points refuse to
-run (SF_ERR_DEVICE) when no GPU is present instead of falling back to CPU."""
+run (SF_ERR_DEVICE) when no GPU is present instead of falling back to CPU.
+One GPU test runs the C++ API test binary (tests/cpp/test_api.cpp)."""
 import ctypes as C
 import os
 import re
@@ -155,3 +156,19 @@ def test_cluster_argument_validation():
         b[0].radius = 0.01
         assert L.lib().sf_cluster_create(C.byref(p), 1, b, C.byref(h)) == L.SF_ERR_DEVICE
         assert not h.value
+
+
+def test_noalloc_stats_argument_checks():
+    """sf_field_stats_noalloc refuses grids beyond its fixed scratch and NULL
+    pointers before touching a device, and needs a GPU otherwise."""
+    p = L.Params()
+    L.check(L.lib().sf_params_cpu_default(p))
+    fake = C.c_void_p(256)          # never dereferenced: the calls fail first
+    f = L.lib().sf_field_stats_noalloc
+    assert f(C.byref(p), L.NOALLOC_MAX_THETA + 1, 8, fake, None, None, fake) == L.SF_ERR_INVALID
+    assert "SF_NOALLOC_MAX_THETA" in L.last_error()
+    assert f(C.byref(p), 64, 128, None, None, None, fake) == L.SF_ERR_INVALID
+    assert f(C.byref(p), 64, 128, fake, None, None, None) == L.SF_ERR_INVALID
+    assert f(C.byref(p), 2, 128, fake, None, None, fake) == L.SF_ERR_INVALID
+    if not L.device_available():
+        assert f(C.byref(p), 64, 128, fake, None, None, fake) == L.SF_ERR_DEVICE
