@@ -70,6 +70,10 @@ def parse():
     ap.add_argument("--tile-shapes", default=None,
                     help="c3: tile shape index per size bucket, e.g. 0,0,0,0 (skips autotuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="c2: skip the reduced surrogate run reported beside the headline")
+    ap.add_argument("--surrogate-bubbles", type=int, default=64,
+                    help="c2: bubbles of the reduced surrogate run (256x512 each)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the float4 copy kernel (counter calibration)")
@@ -86,20 +90,53 @@ def copy_probe(L, nbytes=256 << 20, iters=20):
 
 
 def pmc_traffic(workload):
-    """HBM bytes per step-kernel launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE, WRITE_SIZE per dispatch, scaled by the copy probe)."""
+    """HBM bytes per step-kernel launch, read from the committed rocprofv3
+    --pmc summary of the same kernel and workload (FETCH_SIZE, WRITE_SIZE per
+    dispatch, scaled by what the counters read for a copy of known size),
+    and the file it was read from.  The counters are not collected by this
+    run: rocprofv3 --pmc needs a run of its own."""
     try:
-        with open(PMC_SUMMARY[workload]) as f:
+        path = PMC_SUMMARY[workload]
+        with open(path) as f:
             d = json.load(f)
-        return float(d["traffic_bytes_per_launch"]), d
+        return float(d["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
     except (OSError, ValueError, KeyError):
         return None, None
 
 
-def cpu_baseline(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s):
-    """FP64 oracle on the host: full-grid drainage steps plus (when the
-    workload renders) the spectral RGB of sampled cells, extrapolated per
-    cell-step.  Single thread (the reference loop is sequential)."""
+def traffic_source(src):
+    return None if src is None else ("committed rocprofv3 --pmc summary %s (FETCH_SIZE, WRITE_SIZE; "
+                                     "not measured in this run)" % src)
+
+
+def host_info():
+    """CPU model and counts of the host the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def baseline_threads():
+    """Host threads of the all-cores CPU baseline: OMP_NUM_THREADS when set,
+    else the CPUs this process may run on, at most 16 (one GPU's share of
+    the host)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() else len(os.sched_getaffinity(0))
+    return max(1, min(16, n))
+
+
+def _cpu_rate(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s, threads):
+    """FP64 oracle on the host with `threads` threads: full-grid drainage
+    steps plus (when the workload renders) the spectral RGB of sampled
+    cells, extrapolated per cell-step."""
+    O.set_threads(threads)
     q = O.Params()
     for name, _ in O.Params._fields_:
         setattr(q, name, getattr(params, name))
@@ -118,7 +155,7 @@ def cpu_baseline(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s
     if spectral:
         h, _ = sim.fields()
         rng = np.random.default_rng(0)
-        m = 20000
+        m = 20000 * threads
         t1 = time.perf_counter()
         done = 0
         while done < 10 * m and (time.perf_counter() - t1) < budget_s * 0.4:
@@ -127,16 +164,27 @@ def cpu_baseline(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s
             done += m
         t_spec = (time.perf_counter() - t1) / done
         sample += " + 81-wavelength FP64 spectral RGB of %d sampled cells" % done
-    per = t_drain + t_spec
-    return {"value": 1.0 / per, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": sample + ", 1 thread, extrapolated per cell-step"}
+    O.set_threads(1)
+    return 1.0 / (t_drain + t_spec), sample
 
 
-def foam_cpu_baseline(O, params, fm, h, budget_s):
-    """FP64 oracle stepping randomly drawn bubbles of the foam (their own
-    grids, radii, viscosities and dt) on one host thread."""
+def cpu_baseline(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s):
+    """The FP64 oracle timed twice on the host: with one thread (the
+    reference loop is sequential) and with OpenMP over theta rows on the
+    host's cores; ``value`` is the all-cores rate."""
+    threads = baseline_threads()
+    v1, s1 = _cpu_rate(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s * 0.5, 1)
+    vn, sn = _cpu_rate(O, params, h0, n_theta, n_phi, dt, cam, cmf, spectral, budget_s * 0.5, threads)
+    return {"value": vn, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": sn + ", %d threads (OpenMP over theta rows), extrapolated per cell-step" % threads,
+            "single_thread": {"value": v1, "cores": 1, "sample": s1 + ", 1 thread"},
+            "host": host_info()}
+
+
+def _foam_cpu_rate(O, params, fm, h, budget_s, threads):
     rng = np.random.default_rng(1)
     dts = fm.bubble_dt()
+    O.set_threads(threads)
     t0 = time.perf_counter()
     cell_steps = 0
     t_steps = 0.0
@@ -156,9 +204,21 @@ def foam_cpu_baseline(O, params, fm, h, budget_s):
         t_steps += time.perf_counter() - t1
         cell_steps += n * b.n_theta * b.n_phi
         picked += 1
-    return {"value": cell_steps / t_steps, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle, 20 steps each of %d randomly drawn foam bubbles (%d cell-steps), "
-                      "1 thread" % (picked, cell_steps)}
+    O.set_threads(1)
+    return cell_steps / t_steps, ("FP64 oracle, 20 steps each of %d randomly drawn foam bubbles (%d cell-steps)"
+                                  % (picked, cell_steps))
+
+
+def foam_cpu_baseline(O, params, fm, h, budget_s):
+    """FP64 oracle stepping randomly drawn bubbles of the foam (their own
+    grids, radii, viscosities and dt), once with one host thread and once
+    with OpenMP over each bubble's theta rows; ``value`` is the latter."""
+    threads = baseline_threads()
+    v1, s1 = _foam_cpu_rate(O, params, fm, h, budget_s * 0.5, 1)
+    vn, sn = _foam_cpu_rate(O, params, fm, h, budget_s * 0.5, threads)
+    return {"value": vn, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": sn + ", %d threads" % threads,
+            "single_thread": {"value": v1, "cores": 1, "sample": s1 + ", 1 thread"}, "host": host_info()}
 
 
 def bench_foam(args, L, W):
@@ -206,6 +266,7 @@ def bench_foam(args, L, W):
                                                                      for row in trial]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("c3")[0],
+                     "traffic_source": traffic_source(pmc_traffic("c3")[1]),
                      "kernel": "foam_step_kernel",
                      "avg_launch_ms": k_ms, "launches_timed": k_n,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells},
@@ -317,7 +378,7 @@ def bench_frame(args, L, W):
     ms = elapsed * 1e3 / steps
     algo = CAUSTIC_BYTES_PER_CELL * cells
     achieved = algo / (ms * 1e-3) / 1e9
-    traffic, _ = pmc_traffic("frame")
+    traffic, tsrc = pmc_traffic("frame")
     frame = render_frame(ctx, camera=cam, refractive_index=1.33, intensity=4.0)
     reps = 10
     t0 = time.perf_counter()
@@ -331,7 +392,8 @@ def bench_frame(args, L, W):
         "config": {"workload": "caustic map and LUT-rendered RGBA8 frame of the single-bubble 2048x4096 "
                                "float-float thickness state, CausticParams defaults, camera (2.2R, 1.4R, 3.1R)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "caustics_vec4_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source(tsrc),
+                     "kernel": "caustics_vec4_kernel",
                      "avg_launch_ms": ms, "algorithmic_bytes_per_launch": algo,
                      "timing": "wall clock over back-to-back launches on the context's stream"},
         "render": {"ms_per_frame_incl_lut_and_copy": render_ms,
@@ -368,9 +430,12 @@ def fno_window_flops(cfg, cells):
     return cells * (cfg.layers * per_cell_layer + 2 * 5 * C + 2 * C * 2)
 
 
-def bench_surrogate(args, L, W, n_bubbles):
-    """Surrogate + judge (c4: one bubble, c5: a batch of bubbles) against the
-    plain solver on identical seeded inputs, with the no-network baselines."""
+def surrogate_runs(L, W, n_bubbles, steps, train_windows, tol, open_loop=True):
+    """Surrogate + judge on a foam of n_bubbles 256x512 bubbles against the
+    plain solver on identical seeded inputs, with the no-network baselines
+    (identity, linear extrapolation) judged the same way, and the network and
+    extrapolation open loop (every window accepted).  The network is trained
+    online first.  Returns (model, runs, plain, info)."""
     from soap_bubble_simulation_amd.foam import FoamBubble, FoamDrainage, foam_params
     from soap_bubble_simulation_amd.surrogate import FnoSurrogate
 
@@ -389,21 +454,23 @@ def bench_surrogate(args, L, W, n_bubbles):
         return fm
 
     # online training on plain-solver trajectories of 16 random bubbles; a new
-    # trajectory (new bubbles and initial films) every 62 windows (~1000 steps)
+    # trajectory (new bubbles and initial films) every 62 windows (~1000
+    # steps), each trajectory one train_foam call so that its windows carry
+    # the previous window's increment
     t0 = time.perf_counter()
     losses = []
     seed = 1000
-    train = None
-    for w in range(args.train_windows):
-        if w % 62 == 0:
-            if train is not None:
-                train.close()
-            train = make(16, seed)
-            seed += 11
-        lr = 1e-3 if w < 0.75 * args.train_windows else 2e-4
-        losses.append(float(model.train_foam(train, 1, lr)[0]))
+    w = 0
+    while w < train_windows:
+        n = min(62, train_windows - w)
+        train = make(16, seed)
+        seed += 11
+        lr = 1e-3 if w < 0.75 * train_windows else 2e-4
+        losses.extend(float(v) for v in model.train_foam(train, n, lr))
+        train.close()
+        w += n
     train_s = time.perf_counter() - t0
-    windows = max(1, args.steps // K)
+    windows = max(1, steps // K)
     steps = windows * K
 
     ref = make(n_bubbles, 77)
@@ -417,21 +484,25 @@ def bench_surrogate(args, L, W, n_bubbles):
     ref.synchronize()
     plain_s = time.perf_counter() - t1
     h_ref, _ = ref.fields()
+    ref.close()
     cells = n_bubbles * T * P
     inc_norm = float(np.linalg.norm(h_ref - h0))
 
+    cases = [("network", L.PREDICT_NETWORK, tol), ("identity", L.PREDICT_IDENTITY, tol),
+             ("extrapolate", L.PREDICT_EXTRAPOLATE, tol)]
+    if open_loop:
+        cases += [("network_open_loop", L.PREDICT_NETWORK, 1e9), ("extrapolate_open_loop", L.PREDICT_EXTRAPOLATE, 1e9)]
     runs = {}
-    for name, pred in [("network", L.PREDICT_NETWORK), ("identity", L.PREDICT_IDENTITY),
-                       ("extrapolate", L.PREDICT_EXTRAPOLATE)]:
+    for name, pred, t in cases:
         fm = make(n_bubbles, 77, shapes)
         fm.synchronize()
         t2 = time.perf_counter()
-        st = model.advance_foam(fm, windows, tol=args.tol, predictor=pred)
+        st = model.advance_foam(fm, windows, tol=t, predictor=pred)
         fm.synchronize()
         wall = time.perf_counter() - t2
         h, _ = fm.fields()
         runs[name] = {
-            "cell_steps_per_s": cells * steps / wall, "wall_s": wall,
+            "cell_steps_per_s": cells * steps / wall, "wall_s": wall, "tol": t,
             "speedup_vs_plain": plain_s / wall,
             "accepted": int(st.accepted), "fallbacks": int(st.fallbacks),
             "max_residual": st.max_residual,
@@ -440,7 +511,22 @@ def bench_surrogate(args, L, W, n_bubbles):
             "network_ms": st.surrogate_ms, "judge_ms": st.judge_ms, "fallback_ms": st.fallback_ms,
         }
         fm.close()
+    plain = {"cell_steps_per_s": cells * steps / plain_s, "wall_s": plain_s}
+    info = {"bubbles": n_bubbles, "cells": cells, "window": K, "windows": windows, "steps": steps,
+            "tile_shapes": list(shapes),
+            "training": {"windows": train_windows, "seconds": train_s,
+                         "first_loss": losses[0] if losses else None,
+                         "last_losses_mean": float(np.mean(losses[-20:])) if losses else None,
+                         "scales": model.scales()}}
+    return model, runs, plain, info
+
+
+def bench_surrogate(args, L, W, n_bubbles):
+    """Surrogate + judge (c4: one bubble, c5: a batch of bubbles) against the
+    plain solver on identical seeded inputs, with the no-network baselines."""
+    model, runs, plain, info = surrogate_runs(L, W, n_bubbles, args.steps, args.train_windows, args.tol)
     net = runs["network"]
+    cells, windows, steps = info["cells"], info["windows"], info["steps"]
     flops = fno_window_flops(model.config, cells) * windows
     achieved = flops / (net["network_ms"] * 1e-3) / 1e12 if net["network_ms"] > 0 else 0.0
     out = {
@@ -451,18 +537,17 @@ def bench_surrogate(args, L, W, n_bubbles):
                                "plain-step judge per window (tolerance %g) and fallback, %d bubble(s) at 256x512, "
                                "%d steps; trained online for %d windows on plain-solver trajectories"
                                % (args.workload, args.tol, n_bubbles, steps, args.train_windows),
-                   "bubbles": n_bubbles, "cells": cells, "window": K, "windows": windows,
-                   "tile_shapes": list(shapes)},
+                   "bubbles": n_bubbles, "cells": cells, "window": info["window"], "windows": windows,
+                   "tile_shapes": info["tile_shapes"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": achieved / 2500.0, "traffic": None,
-                     "kernel": "network inference phase (fno_row_kernel + gemm_kernel, host-timed)",
+                     "kernel": "network inference phase (lift_dft_kernel, theta_fused_kernel, fno_row_kernel; "
+                               "host-timed)",
                      "flops_per_window": fno_window_flops(model.config, cells)},
-        "plain": {"cell_steps_per_s": cells * steps / plain_s, "wall_s": plain_s},
-        "surrogate_vs_plain": net["cell_steps_per_s"] / (cells * steps / plain_s),
+        "plain": plain,
+        "surrogate_vs_plain": net["cell_steps_per_s"] / plain["cell_steps_per_s"],
         "runs": runs,
-        "training": {"windows": args.train_windows, "seconds": train_s, "first_loss": losses[0] if losses else None,
-                     "last_losses_mean": float(np.mean(losses[-20:])) if losses else None,
-                     "scales": model.scales()},
+        "training": info["training"],
     }
     print(json.dumps(out))
 
@@ -536,39 +621,43 @@ def main():
     run(max(1, args.warmup))
     L.check(lib.sf_drainage_synchronize(ctx.handle))
 
-    def timed_pass():
+    def timed_pass(steps):
         L.check(lib.sf_drainage_synchronize(ctx.handle))
         t0 = time.perf_counter()
         done = 0
-        while done < args.steps:
-            n = min(500, args.steps - done)
+        while done < steps:
+            n = min(500, steps - done)
             run(n)
             done += n
         L.check(lib.sf_drainage_synchronize(ctx.handle))
         return time.perf_counter() - t0
 
     # Kernel-timing pass: a HIP event pair brackets every launch on the
-    # context's stream (average kernel durations for the roofline).
+    # context's stream (average kernel durations for the roofline).  It runs
+    # at least 500 steps: from an idle GPU the first ~100 steps run at lower
+    # clocks (20 steps after 5 warm-up steps: 0.189 ms per step against
+    # 0.172 ms from 200 steps on), and the kernel averages are steady-state
+    # figures.
     ctx.kernel_timing(True)
     ms_spec = C.c_double()
     n_spec = C.c_int64()
     L.check(lib.sf_spectral_kernel_timing(ctx.handle, 1 if spectral else 0, C.byref(ms_spec),
                                           C.byref(n_spec)))
-    timed_pass()
+    timed_pass(max(args.steps, 500))
     drain_ms, drain_n = ctx.kernel_timing(False)
     L.check(lib.sf_spectral_kernel_timing(ctx.handle, 0, C.byref(ms_spec), C.byref(n_spec)))
 
     # Headline pass: exactly K steps, no per-launch events, synchronised on
     # both sides.
-    elapsed = timed_pass()
+    elapsed = timed_pass(args.steps)
 
     cells = n_theta * n_phi
     value = cells * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
     achieved = BYTES_PER_CELL_STEP * cells / (drain_ms * 1e-3) / 1e9
-    traffic, pmc = pmc_traffic(args.workload)
+    traffic, tsrc = pmc_traffic(args.workload) if args.workload in PMC_SUMMARY else (None, None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source(tsrc),
                 # the kernel the context selects: 16-row float4 bands at 2048 x 4096, the scalar kernel
                 # with one row per block at 128 x 256 (grids of up to 262144 cells)
                 "kernel": "drainage_band_kernel" if n_theta >= 1024 else "drainage_step_kernel",
@@ -625,6 +714,20 @@ def main():
         for _ in range(reps):
             L.check(lib.sf_lut_2d(1.33, lam, 0.0, 1000.0, 1024, 90, 1, L.fptr(lut)))
         out["kernels"]["lut_2d_1024x90_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+    if spectral and not args.no_surrogate:
+        # The surrogate half of the metric: a reduced run of the batched
+        # surrogate benchmark (the full one is --workload c5).
+        model, runs, plain, info = surrogate_runs(L, W, args.surrogate_bubbles, 640, 300, 1e-3, open_loop=True)
+        out["surrogate"] = {
+            "workload": "%d bubbles at 256x512, %d steps (%d windows of %d), judge tolerance 1e-3, trained online "
+                        "for %d windows; full run: --workload c5" % (info["bubbles"], info["steps"], info["windows"],
+                                                                     info["window"], info["training"]["windows"]),
+            "plain_cell_steps_per_s": plain["cell_steps_per_s"],
+            "network_vs_plain": runs["network"]["cell_steps_per_s"] / plain["cell_steps_per_s"],
+            "runs": {k: {x: v[x] for x in ("cell_steps_per_s", "accepted", "fallbacks", "rel_l2_h",
+                                           "rel_l2_increment")} for k, v in runs.items()},
+        }
+        model.close()
     if not args.no_cpu_baseline:
         import oracle as O
         out["cpu_baseline"] = cpu_baseline(O, p, h0, n_theta, n_phi, dt, cam, cmf, spectral,

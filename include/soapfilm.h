@@ -514,10 +514,16 @@ int sf_cluster_thickness_from_foam(sf_cluster* c, sf_foam* foam);
  *   ||h_J - h(t+K)||_2 <= tol ||h(t+K)||_2 and its state becomes h_J,
  *   otherwise it falls back to K plain steps from h(t).
  * Input channels per cell: relative thickness perturbation, surfactant,
- * sin(theta), the window's gravity-sink number K dt rho g h_ref^2 / (3 eta)
- * and diffusion number K dt D / (R^2 dtheta^2) (h_ref = the bubble's mean h).
- * Outputs: (h(t+j) - h(t)) / (h_ref s_y) for j = K-1, K.  Tensors are
- * channel-last [batch][n_theta][n_phi][channels], FP32. */
+ * sin(theta), the window's gravity-sink number K dt rho g h_ref^2 / (3 eta),
+ * its diffusion number K dt D / (R^2 dtheta^2) (h_ref = the bubble's mean h),
+ * the previous window's increment d = h(t) - h(t-K) (zero on the first
+ * window of a train or advance call), dt D lap(d), its change under one
+ * explicit diffusion step, and one explicit plain step of h,
+ * dt (D lap(h) - c h^3 sin theta) (zero on the pole rows); the last three are
+ * divided by the bubble's increment scale y = rms(d), or h_ref s_y when
+ * there is no previous window.  Outputs: corrections to linear extrapolation
+ * of that increment, (h(t+j) - h(t) - (j/K) d) / y for j = K-1, K.
+ * Tensors are channel-last [batch][n_theta][n_phi][channels], FP32. */
 typedef struct sf_fno_config {
     int32_t n_theta, n_phi;   /* grid of every bubble; n_theta*n_phi % 4096 == 0 */
     int32_t width;            /* channels (divides 256, <= 64)                  */
@@ -544,7 +550,7 @@ int sf_fno_config_default(sf_fno_config* config);
 int sf_fno_create(const sf_fno_config* config, sf_fno** out);
 void sf_fno_destroy(sf_fno* model);
 int sf_fno_num_params(sf_fno* model, int64_t* n);
-/* Flat FP32 parameters: lift W[5][width], b[width]; per layer W[width][width],
+/* Flat FP32 parameters: lift W[8][width], b[width]; per layer W[width][width],
  * b[width], R[2 modes_theta][modes_phi][2][width][width] (re, im); projection
  * Q[width][2], q[2]. */
 int sf_fno_get_params(sf_fno* model, float* params);
@@ -553,7 +559,7 @@ int sf_fno_set_params(sf_fno* model, const float* params);
  * first training window when unset). */
 int sf_fno_get_scales(sf_fno* model, double* scales);
 int sf_fno_set_scales(sf_fno* model, const double* scales);
-/* Host-array forward pass: in[batch][N][5] -> out[batch][N][2]. */
+/* Host-array forward pass: in[batch][N][8] -> out[batch][N][2]. */
 int sf_fno_forward(sf_fno* model, int32_t batch, const float* in, float* out);
 /* Kernel paths of this model's shape: the fused per-row layer kernel (training
  * and inference), the lift folded into the first phi DFT and the fused theta
@@ -568,8 +574,10 @@ int sf_fno_gradient(sf_fno* model, int32_t batch, const float* in, const float* 
 /* One Adam step (lr, beta 0.9 / 0.999) on a host batch; returns the loss. */
 int sf_fno_train_step(sf_fno* model, int32_t batch, const float* in, const float* target, double lr,
                       double* loss);
-/* `windows` training windows generated online by the foam's plain solver
- * (every bubble n_theta x n_phi); losses[windows] (may be NULL). */
+/* `windows` consecutive training windows generated online by the foam's
+ * plain solver (every bubble n_theta x n_phi); losses[windows] (may be NULL).
+ * The previous-increment input is carried between the windows of one call
+ * only: a call's first window sees zero. */
 int sf_fno_train_foam(sf_fno* model, sf_foam* foam, const sf_params* params, int32_t windows, double lr,
                       double* losses);
 /* Predictors of sf_fno_advance_foam: the network, or the no-network
@@ -583,7 +591,7 @@ int sf_fno_train_foam(sf_fno* model, sf_foam* foam, const sf_params* params, int
 int sf_fno_advance_foam(sf_fno* model, sf_foam* foam, const sf_params* params, int32_t windows, double tol,
                         int32_t predictor, sf_fno_stats* stats);
 /* Input features of bubbles [first, first + count) of the foam's current
- * state into in_host[count][N][5]. */
+ * state into in_host[count][N][8] (previous-increment channels zero). */
 int sf_fno_features(sf_fno* model, sf_foam* foam, const sf_params* params, int32_t first, int32_t count,
                     float* in_host);
 

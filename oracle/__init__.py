@@ -104,6 +104,7 @@ def lib():
             "orc_interference_color": (None, [C.c_double, dp, C.c_double, C.c_double, C.c_int, dp]),
             "orc_lut_1d": (None, [C.c_double, dp, C.c_double, C.c_double, C.c_int, C.c_double, dp]),
             "orc_lut_2d": (None, [C.c_double, dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp]),
+            "orc_set_threads": (None, [C.c_int]),
             "orc_lut_airy_terms": (None, [C.c_float, C.c_float, C.c_float, fp, fp, fp, fp]),
             "orc_lut_thin_film": (None, [C.c_float, C.c_float, C.c_float, C.c_float, fp]),
             "orc_lut_rgb_f32": (None, [C.c_float, C.c_float, fp]),
@@ -123,6 +124,12 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Host threads of the oracle's interior rows and sampled spectral cells
+    (1 by default: the reference's loop is sequential)."""
+    lib().orc_set_threads(int(n))
 
 
 def _dp(a: np.ndarray):

@@ -9,7 +9,7 @@ multiplies by explicit truncated DFT matrices, and the backward pass is
 written out by hand.  Its gradient is pinned by FP64 central differences
 in ``tests/test_fno.py``.
 
-Parameter layout (flat, as ``sf_fno_get_params``): lift W[5][C], b[C];
+Parameter layout (flat, as ``sf_fno_get_params``): lift W[8][C], b[C];
 per layer W[C][C], b[C], R[2 Mt'][Mp][2][C][C] (re, im; theta modes
 k = 0..Mt'-1, -Mt'..-1); projection Q[C][2], q[2].
 """
@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import numpy as np
 
-N_IN = 5
+N_IN = 8
 N_OUT = 2
 K0 = 0.7978845608028654
 K1 = 0.044715
@@ -104,7 +104,7 @@ def spectral_adjoint(dy, X, R, s: Shapes):
 
 
 def forward(flat, x_in, s: Shapes):
-    """x_in [B,T,P,5] -> out [B,T,P,2] and the cache of the backward pass."""
+    """x_in [B,T,P,N_IN] -> out [B,T,P,2] and the cache of the backward pass."""
     p = unpack(flat, s)
     x = np.asarray(x_in, np.float64) @ p["Wl"] + p["bl"]
     cache = {"x": [x], "z": [], "X": []}

@@ -104,6 +104,9 @@ void   orc_lut_2d(double n_film, const double lambdas[3], double tmin, double tm
                   int nt, int na, int gamma, double* out /* nt*na*3 */);
 
 /* ---- FP32 runtime LUT (interference_lut.rs) ------------------------------ */
+/* Host threads of orc_step's interior rows and orc_spectral_cells (default 1). */
+extern int orc_threads;
+void   orc_set_threads(int n);
 void   orc_lut_airy_terms(float t_nm, float cos_theta, float n_film, float* cos_t, float* r, float* f,
                           float a[7]);
 void   orc_lut_thin_film(float t_nm, float cos_theta, float n_film, float intensity, float rgb[3]);

@@ -188,6 +188,12 @@ static void update_marginal_regeneration(orc_sim* s, double dt)
 }
 
 /* Ring mean of one row (drainage.rs:484-488 / 495-499), sequential FP64 sum. */
+/* Threads of the interior loop and of the sampled spectral cells; 1 (the
+ * reference's sequential loop) unless a CPU baseline sets more. */
+int orc_threads = 1;
+
+void orc_set_threads(int n) { orc_threads = n < 1 ? 1 : n; }
+
 static double ring_mean(const double* row, int np)
 {
     double sum = 0.0;
@@ -209,11 +215,22 @@ void orc_step(orc_sim* s, double dt)
     const double R = p->bubble_radius;
     const size_t n = (size_t)nt * np;
 
-    memcpy(s->sh, s->h, n * sizeof(double));
-    memcpy(s->sg, s->g, n * sizeof(double));
     const double* sh = s->sh;
     const double* sg = s->sg;
+    (void)n;
 
+    /* Rows are independent once the scratch copy is complete: with more than
+     * one thread (orc_set_threads) the copy and the interior rows are split
+     * over threads, with the barrier of the first loop between them.  The
+     * arithmetic of each cell is unchanged. */
+#pragma omp parallel num_threads(orc_threads) if (orc_threads > 1)
+    {
+#pragma omp for schedule(static)
+    for (int i = 0; i < nt; ++i) {
+        memcpy(s->sh + (size_t)i * np, s->h + (size_t)i * np, (size_t)np * sizeof(double));
+        memcpy(s->sg + (size_t)i * np, s->g + (size_t)i * np, (size_t)np * sizeof(double));
+    }
+#pragma omp for schedule(static)
     for (int i = 1; i < nt - 1; ++i) {
         double theta = (double)i * dtheta;
         double sin_theta = sin(theta);
@@ -277,6 +294,7 @@ void orc_step(orc_sim* s, double dt)
                 s->g[c] = new_c;
             }
         }
+    }
     }
 
     /* update_poles (480-503) or copy-through (wgsl 85-89). */

@@ -336,6 +336,7 @@ void orc_spectral_cells(const double* h, int n_theta, int n_phi, const int64_t* 
 {
     const double dtheta = ORC_PI / (double)(n_theta - 1);
     const double dphi = 2.0 * ORC_PI / (double)n_phi;
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1)
     for (int k = 0; k < n_cells; ++k) {
         int64_t c = cells[k];
         int i = (int)(c / n_phi), j = (int)(c % n_phi);

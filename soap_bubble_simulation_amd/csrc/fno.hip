@@ -2,8 +2,19 @@
 //
 // The network maps a bubble's state at step t to its film thickness after
 // K - 1 and K drainage steps (K = window).  Input channels per cell:
-// relative thickness perturbation, surfactant, sin(theta), and two
-// per-bubble numbers of the window (gravity sink and diffusion number).
+// relative thickness perturbation, surfactant, sin(theta), two per-bubble
+// numbers of the window (gravity sink and diffusion number), the previous
+// window's increment d = h(t) - h(t - K) (zero on the first window of a call),
+// its change under one explicit diffusion step, dt D lap(d), and one explicit
+// plain step of h, dt (D lap(h) - c h^3 sin(theta)): local information (how
+// fast d decays; the pole-ring checkerboard that flips sign every step at the
+// CFL limit) that the network's pointwise and low-mode spectral paths cannot
+// form from values at one cell.
+// The outputs are corrections to linear extrapolation of that increment:
+// h(t + j) = h(t) + f_j (h(t) - h(t - K)) + out_j y, f_j = j / K, with y the
+// bubble's rms previous increment (h_ref s_y on a call's first window), the
+// scale of the increment channels and targets too: the network's relative
+// error then shrinks with the increments as the film settles.
 // Architecture: pointwise lift to `width` channels, `layers` Fourier layers
 //   x <- act(W x + b + iDFT(R . DFT_trunc(x)))   (act = GELU except the last)
 // with modes k in [-modes_theta, modes_theta) along theta and m in
@@ -57,6 +68,9 @@ struct BubbleWin {
     long long off;        // first cell in the foam layout
     float s_sink;         // K dt rho g / (3 eta) h_ref^2   (relative thinning per window at sin = 1)
     float s_diff;         // K dt D / (R^2 dtheta^2)
+    float yscale;         // m: scale of this window's increments (the rms of the
+                          // previous window's increment, else h_ref s_y)
+    float pad;
 };
 
 // Per-bubble reductions run in two passes: blockIdx.y slices a bubble's cells
@@ -100,6 +114,25 @@ __global__ __launch_bounds__(256) void bubble_mean_kernel(const float* hi, const
     if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.x) * gridDim.y + blockIdx.y] = t;
 }
 
+// Partial sums of (h - h_p)^2 over a slice of each bubble (compensated FP32):
+// the size of the previous window's increment.
+__global__ __launch_bounds__(256) void bubble_incsq_kernel(const float* hi, const float* lo, const float* hp_hi,
+                                                           const float* hp_lo, const BubbleWin* win, int n_cells,
+                                                           float* part) {
+    const BubbleWin w = win[blockIdx.x];
+    int c0, c1;
+    slice_of(n_cells, c0, c1);
+    Kahan s;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const long long o = w.off + c;
+        const float d = (hi[o] - hp_hi[o]) + (lo[o] - hp_lo[o]);
+        s.add(d * d);
+    }
+    __shared__ float red[256];
+    const float t = block_sum(s.value(), red);
+    if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.x) * gridDim.y + blockIdx.y] = t;
+}
+
 // out[b * k + j] = scale * sum over s of part[(b * S + s) * k + j].
 __global__ void fold_partials_kernel(const float* part, int nb, int S, int k, float scale, float* out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,11 +147,14 @@ struct FeatArgs {
     const float* hi;
     const float* lo;
     const float* g;
+    const float* hp_hi;   // state at the previous window's start, or null
+    const float* hp_lo;
     const BubbleWin* win;
     const float* href;    // per bubble of the chunk
     int first, nb, nt, np;
     float inv_sx;         // 1 / scale of the relative perturbation
     float sink_scale, diff_scale;
+    float inv_window;     // 1 / K
     float* in;            // [nb][nt * np][kCin]
 };
 
@@ -139,16 +175,69 @@ __global__ void features_kernel(const FeatArgs a) {
         float* o = a.in + q * kCin;
         o[0] = static_cast<float>(r) * a.inv_sx;
         o[1] = (a.g[w.off + c] - 0.5f) * 10.f;
-        o[2] = sinf(dth * static_cast<float>(i));
+        const float sin_i = sinf(dth * static_cast<float>(i));
+        o[2] = sin_i;
         o[3] = w.s_sink * a.sink_scale;
         o[4] = w.s_diff * a.diff_scale;
+        // previous window's increment d, scaled as the targets, one
+        // diffusion step of it, dt D lap(d) = (s_diff / K) dth^2 lap~(d), and
+        // one explicit plain step of h itself, dt (D lap(h) - c h^3 sin),
+        // lap~ the spherical Laplacian in radians (drainage.rs:352-372); h
+        // differences are taken on the float-float pairs
+        const float* hh = a.hi + w.off;
+        const float* hl = a.lo + w.off;
+        const bool interior = i > 0 && i < a.nt - 1;
+        float lap_w = 0.f, lap_s = 0.f, lap_c = 0.f, lap_p = 0.f;   // stencil weights
+        long long kn = c, ks = c, kw = c, ke = c;
+        if (interior) {
+            const int j = static_cast<int>(c - static_cast<long long>(i) * a.np);
+            const long long rowc = static_cast<long long>(i) * a.np;
+            kn = c - a.np;
+            ks = c + a.np;
+            kw = rowc + (j == 0 ? a.np - 1 : j - 1);
+            ke = rowc + (j == a.np - 1 ? 0 : j + 1);
+            const float st = sin_i, ct = cosf(dth * static_cast<float>(i));
+            const float dph = 2.f * static_cast<float>(kPi) / static_cast<float>(a.np);
+            const float rp = dth / (st * dph);
+            lap_c = 0.5f * dth * (ct / st);     // times (south - north)
+            lap_p = rp * rp;                    // times (east + west - 2 centre)
+            lap_s = 1.f;
+            lap_w = w.s_diff * a.inv_window;    // dt D / (R^2 dth^2)
+        }
+        auto lap_of = [&](auto f) {
+            const float fc = f(c), fn = f(kn), fs = f(ks);
+            return lap_s * ((fs - fc) + (fn - fc)) + lap_c * (fs - fn) + lap_p * ((f(ke) - fc) + (f(kw) - fc));
+        };
+        float dp = 0.f, ddp = 0.f;
+        if (a.hp_hi) {
+            const float* ph = a.hp_hi + w.off;
+            const float* pl = a.hp_lo + w.off;
+            auto inc = [&](long long k) { return (hh[k] - ph[k]) + (hl[k] - pl[k]); };
+            dp = inc(c);
+            if (interior) ddp = lap_w * lap_of(inc);
+        }
+        float step1 = 0.f;
+        if (interior) {
+            // h_k - h_c of the pairs: the hi difference is exact for neighbours within a factor 2
+            auto hd = [&](long long k) { return (hh[k] - hh[c]) + (hl[k] - hl[c]); };
+            const float diff = lap_s * (hd(ks) + hd(kn)) + lap_c * (hd(ks) - hd(kn)) + lap_p * (hd(ke) + hd(kw));
+            const float hrel = 1.f + r;
+            step1 = lap_w * diff - w.s_sink * a.inv_window * hrel * hrel * hrel * hr * sin_i;
+        }
+        const float iy = 1.f / w.yscale;
+        o[5] = dp * iy;
+        o[6] = ddp * iy;
+        o[7] = step1 * iy;
     }
 }
 
-// target[b][c][j] = (h_j - h_t) / (h_ref s_y), j = 0 (step K - 1), 1 (step K).
+// target[b][c][j] = (h_j - h_t - f_j (h_t - h_p)) / yscale_b, j = 0 (step
+// K - 1, f = (K - 1)/K), 1 (step K, f = 1); h_p is the state at the previous
+// window's start (hi_p null: no previous window, the term is zero).
 __global__ void targets_kernel(const float* hi_t, const float* lo_t, const float* hi_a, const float* lo_a,
-                               const float* hi_b, const float* lo_b, const BubbleWin* win, const float* href,
-                               int first, int nb, long long n_cells, float inv_sy, float* target) {
+                               const float* hi_b, const float* lo_b, const float* hi_p, const float* lo_p,
+                               float fa, float fb, const BubbleWin* win, const float* href, int first, int nb,
+                               long long n_cells, float* target) {
     const long long total = n_cells * nb;
     for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
          q += static_cast<long long>(gridDim.x) * blockDim.x) {
@@ -156,31 +245,35 @@ __global__ void targets_kernel(const float* hi_t, const float* lo_t, const float
         const long long c = q - static_cast<long long>(b) * n_cells;
         const long long o = win[first + b].off + c;
         // Increments of float-float pairs: the hi differences are exact.
-        const float da = (hi_a[o] - hi_t[o]) + (lo_a[o] - lo_t[o]);
-        const float db = (hi_b[o] - hi_t[o]) + (lo_b[o] - lo_t[o]);
-        const float s = 1.f / href[b];
-        target[q * kCout + 0] = (da * s) * inv_sy;
-        target[q * kCout + 1] = (db * s) * inv_sy;
+        const float dp = hi_p ? (hi_t[o] - hi_p[o]) + (lo_t[o] - lo_p[o]) : 0.f;
+        const float da = ((hi_a[o] - hi_t[o]) + (lo_a[o] - lo_t[o])) - fa * dp;
+        const float db = ((hi_b[o] - hi_t[o]) + (lo_b[o] - lo_t[o])) - fb * dp;
+        const float s = 1.f / win[first + b].yscale;
+        target[q * kCout + 0] = da * s;
+        target[q * kCout + 1] = db * s;
     }
 }
 
 // Predicted states as float-float arrays in the foam layout:
-// (hiA, loA) = h_t + out[.., 0] h_ref s_y,  (hiB, loB) = h_t + out[.., 1] h_ref s_y.
-__global__ void predict_states_kernel(const float* hi_t, const float* lo_t, const float* out, const BubbleWin* win,
-                                      const float* href, int first, int nb, long long n_cells, float sy, float* hiA,
-                                      float* loA, float* hiB, float* loB) {
+// (hiA, loA) = h_t + fa (h_t - h_p) + out[.., 0] yscale_b,
+// (hiB, loB) = h_t + fb (h_t - h_p) + out[.., 1] yscale_b  (hi_p null: no h_p term).
+__global__ void predict_states_kernel(const float* hi_t, const float* lo_t, const float* hi_p, const float* lo_p,
+                                      float fa, float fb, const float* out, const BubbleWin* win, const float* href,
+                                      int first, int nb, long long n_cells, float* hiA, float* loA,
+                                      float* hiB, float* loB) {
     const long long total = n_cells * nb;
     for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
          q += static_cast<long long>(gridDim.x) * blockDim.x) {
         const int b = static_cast<int>(q / n_cells);
         const long long c = q - static_cast<long long>(b) * n_cells;
         const long long o = win[first + b].off + c;
-        const float scale = href[b] * sy;
+        const float scale = win[first + b].yscale;
         const float h = hi_t[o], l = lo_t[o];
+        const float dp = hi_p ? (h - hi_p[o]) + (l - lo_p[o]) : 0.f;
         hiA[o] = h;
-        loA[o] = l + out[q * kCout + 0] * scale;
+        loA[o] = (l + fa * dp) + out[q * kCout + 0] * scale;
         hiB[o] = h;
-        loB[o] = l + out[q * kCout + 1] * scale;
+        loB[o] = (l + fb * dp) + out[q * kCout + 1] * scale;
     }
 }
 
@@ -764,7 +857,8 @@ struct Fno {
     }
 
     // Window numbers of each bubble at mean thickness h_ref (host copy of the means).
-    void upload_windows(const FoamInfo& fi, const std::vector<double>& href, const sf_params& prm) {
+    void upload_windows(const FoamInfo& fi, const std::vector<double>& href, const std::vector<double>& yscale,
+                        const sf_params& prm) {
         std::vector<BubbleWin> wv(fi.nb);
         const double dth = kPi / (T - 1);
         const double K = cfg.window;
@@ -775,8 +869,35 @@ struct Fno {
             wv[b].off = fi.offs[b];
             wv[b].s_sink = static_cast<float>(K * bb.dt * c * href[b] * href[b]);
             wv[b].s_diff = static_cast<float>(K * bb.dt * prm.diffusion_coeff / (R2 * dth * dth));
+            wv[b].yscale = static_cast<float>(yscale[b]);
+            wv[b].pad = 0.f;
         }
         SF_HIP(hipMemcpyAsync(d_win, wv.data(), fi.nb * sizeof(BubbleWin), hipMemcpyHostToDevice, stream));
+    }
+
+    // Increment scale of each bubble for this window: the rms of its previous
+    // window's increment h - h_p (after bubble_means, which leaves the bubble
+    // offsets in d_win), else h_ref s_y.
+    std::vector<double> increment_scales(const FoamInfo& fi, const std::vector<double>& href, const float* hi,
+                                         const float* lo, const float* hp_hi_, const float* hp_lo_) {
+        std::vector<double> y(fi.nb);
+        for (int b = 0; b < fi.nb; ++b) y[b] = href[b] * sy;
+        if (!hp_hi_) return y;
+        const int S = slices();
+        hipLaunchKernelGGL(bubble_incsq_kernel, dim3(fi.nb, S), dim3(256), 0, stream, hi, lo, hp_hi_, hp_lo_, d_win,
+                           static_cast<int>(N), rpart);
+        SF_HIP(hipGetLastError());
+        hipLaunchKernelGGL(fold_partials_kernel, dim3((fi.nb + 255) / 256), dim3(256), 0, stream, rpart, fi.nb, S, 1,
+                           static_cast<float>(1.0 / static_cast<double>(N)), dscratch);
+        SF_HIP(hipGetLastError());
+        std::vector<float> m(fi.nb);
+        SF_HIP(hipMemcpyAsync(m.data(), dscratch, fi.nb * sizeof(float), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int b = 0; b < fi.nb; ++b) {
+            const double r = std::sqrt(static_cast<double>(m[b]));
+            if (std::isfinite(r) && r > 1e-6 * href[b] * sy) y[b] = r;
+        }
+        return y;
     }
 
     std::vector<double> bubble_means(const FoamInfo& fi, const float* hi, const float* lo) {
@@ -797,8 +918,12 @@ struct Fno {
     }
 
     // h_ref of a chunk on the device (dscratch) and the chunk's input features.
+    // Extrapolation factors of the two outputs (steps K - 1 and K).
+    float fa() const { return static_cast<float>(cfg.window - 1) / static_cast<float>(cfg.window); }
+    float fb() const { return 1.f; }
+
     void build_features(const FoamInfo& fi, const std::vector<double>& href, int first, int nb, const float* hi,
-                        const float* lo, const float* g) {
+                        const float* lo, const float* g, const float* hp_hi, const float* hp_lo) {
         // The means are FP32 values carried in doubles on the host; the FP32
         // copy of the whole window lives in a member buffer (rewritten with the
         // same values for every chunk, and only after the window's last
@@ -810,6 +935,8 @@ struct Fno {
         a.hi = hi;
         a.lo = lo;
         a.g = g;
+        a.hp_hi = hp_hi;
+        a.hp_lo = hp_lo;
         a.win = d_win;
         a.href = dscratch;
         a.first = first;
@@ -817,6 +944,7 @@ struct Fno {
         a.nt = T;
         a.np = P;
         a.inv_sx = static_cast<float>(1.0 / sx);
+        a.inv_window = 1.f / static_cast<float>(cfg.window);
         a.sink_scale = 1e6f;
         a.diff_scale = 1.f;
         a.in = in;
@@ -853,7 +981,9 @@ struct Fno {
 
     // One training window on every bubble of the foam: the plain solver
     // advances K steps; each chunk of <= Bm bubbles takes one Adam step.
-    double train_window(sf_foam* f, const sf_params& prm, double lr) {
+    // have_prev: the foam's state at the previous window's start is in hp
+    // (the windows of one call follow each other).
+    double train_window(sf_foam* f, const sf_params& prm, double lr, bool have_prev) {
         FoamInfo fi = foam_info(f);
         ensure_foam_buffers(fi, prm);
         const float *hi, *lo, *g;
@@ -880,15 +1010,18 @@ struct Fno {
             SF_HIP(hipMemcpy(d.data(), hb_lo, fi.total * 4, hipMemcpyDeviceToHost));
             init_scales(fi, href, a, b, c, d);
         }
-        upload_windows(fi, href, prm);
+        upload_windows(fi, href, increment_scales(fi, href, ht_hi, ht_lo, have_prev ? hp_hi : nullptr,
+                                                  have_prev ? hp_lo : nullptr), prm);
         double total_loss = 0.0;
         int chunks = 0;
         for (int first = 0; first < fi.nb; first += Bm) {
             const int nb = std::min(Bm, fi.nb - first);
-            build_features(fi, href, first, nb, ht_hi, ht_lo, g_t);
+            build_features(fi, href, first, nb, ht_hi, ht_lo, g_t, have_prev ? hp_hi : nullptr,
+                           have_prev ? hp_lo : nullptr);
             hipLaunchKernelGGL(targets_kernel, dim3(blocks_for(static_cast<long long>(nb) * N, 256, 16384)),
-                               dim3(256), 0, stream, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo, d_win, dscratch, first,
-                               nb, N, static_cast<float>(1.0 / sy), target);
+                               dim3(256), 0, stream, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
+                               have_prev ? hp_hi : nullptr, have_prev ? hp_lo : nullptr, fa(), fb(), d_win, dscratch,
+                               first, nb, N, target);
             SF_HIP(hipGetLastError());
             forward(nb, true);
             total_loss += loss(nb);
@@ -896,6 +1029,7 @@ struct Fno {
             adam(lr);
             ++chunks;
         }
+        copy_state(ht_hi, ht_lo, hp_hi, hp_lo, fi.total);
         SF_HIP(hipStreamSynchronize(stream));
         return total_loss / std::max(1, chunks);
     }
@@ -923,15 +1057,18 @@ struct Fno {
             copy_state(hi, lo, ht_hi, ht_lo, fi.total);
             if (predictor == SF_FNO_PREDICT_NETWORK) {
                 const std::vector<double> href = bubble_means(fi, ht_hi, ht_lo);
-                upload_windows(fi, href, prm);
+                upload_windows(fi, href, increment_scales(fi, href, ht_hi, ht_lo, have_prev ? hp_hi : nullptr,
+                                                          have_prev ? hp_lo : nullptr), prm);
                 for (int first = 0; first < fi.nb; first += Bm) {
                     const int nb = std::min(Bm, fi.nb - first);
-                    build_features(fi, href, first, nb, ht_hi, ht_lo, g);
+                    build_features(fi, href, first, nb, ht_hi, ht_lo, g, have_prev ? hp_hi : nullptr,
+                                   have_prev ? hp_lo : nullptr);
                     forward(nb, false);
                     hipLaunchKernelGGL(predict_states_kernel,
                                        dim3(blocks_for(static_cast<long long>(nb) * N, 256, 16384)), dim3(256), 0,
-                                       stream, ht_hi, ht_lo, out, d_win, dscratch, first, nb, N,
-                                       static_cast<float>(sy), ha_hi, ha_lo, hb_hi, hb_lo);
+                                       stream, ht_hi, ht_lo, have_prev ? hp_hi : nullptr, have_prev ? hp_lo : nullptr,
+                                       fa(), fb(), out, d_win, dscratch, first, nb, N, ha_hi, ha_lo, hb_hi,
+                                       hb_lo);
                     SF_HIP(hipGetLastError());
                 }
             } else {
@@ -1147,7 +1284,7 @@ extern "C" int sf_fno_train_foam(sf_fno* m, sf_foam* foam, const sf_params* para
         Fno* f = FN(m);
         SF_REQUIRE(f && foam && params && windows >= 0 && lr > 0.0, SF_ERR_INVALID, "bad arguments");
         for (int k = 0; k < windows; ++k) {
-            const double l = f->train_window(foam, *params, lr);
+            const double l = f->train_window(foam, *params, lr, k > 0);
             if (losses) losses[k] = l;
         }
     });
@@ -1174,8 +1311,8 @@ extern "C" int sf_fno_features(sf_fno* m, sf_foam* foam, const sf_params* params
         const float *hi, *lo, *g;
         SF_REQUIRE(sf_foam_device_state(foam, &hi, &lo, &g) == SF_OK, SF_ERR_DEVICE, "%s", sf_last_error());
         const std::vector<double> href = f->bubble_means(fi, hi, lo);
-        f->upload_windows(fi, href, *params);
-        f->build_features(fi, href, first, count, hi, lo, g);
+        f->upload_windows(fi, href, f->increment_scales(fi, href, hi, lo, nullptr, nullptr), *params);
+        f->build_features(fi, href, first, count, hi, lo, g, nullptr, nullptr);
         SF_HIP(hipMemcpyAsync(in_host, f->in, static_cast<long long>(count) * f->N * kCin * 4, hipMemcpyDeviceToHost,
                               f->stream));
         SF_HIP(hipStreamSynchronize(f->stream));

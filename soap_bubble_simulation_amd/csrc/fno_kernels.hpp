@@ -22,7 +22,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Network input channels (h, surfactant and three geometry/parameter
 // features) and output channels (the two predicted increments).
-constexpr int kLiftIn = 5;
+constexpr int kLiftIn = 8;
 constexpr int kProjOut = 2;
 
 __device__ inline unsigned short to_bf16(float x) {

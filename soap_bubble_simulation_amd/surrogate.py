@@ -18,7 +18,7 @@ import numpy as np
 from . import _lib as L
 from .foam import FoamDrainage
 
-N_IN = 5
+N_IN = 8
 N_OUT = 2
 
 

@@ -212,6 +212,8 @@ def test_surrogate_features_match_state(trained):
         th = np.arange(64) * np.pi / 63
         assert np.allclose(x[k, :, 0, 2], np.sin(th), atol=1e-6)
         assert np.all(x[k, ..., 3] == x[k, 0, 0, 3]) and np.all(x[k, ..., 4] == x[k, 0, 0, 4])
+        assert np.all(x[k, ..., 5:7] == 0.0)    # no previous window: no increment
+        assert np.isfinite(x[k, ..., 7]).all() and np.all(x[k, [0, -1], :, 7] == 0.0)   # one plain step
 
 
 @pytest.mark.gpu
@@ -224,25 +226,47 @@ def test_surrogate_training_converges(trained):
 
 
 @pytest.mark.gpu
-def test_network_beats_identity_on_held_out_increments(trained):
-    """What the trained network gains over the method without it: its
-    K-step increment against the plain solver's, on bubbles it never saw."""
+def test_network_beats_extrapolation_on_held_out_bubbles(trained):
+    """What the trained network gains over the method without it, on bubbles
+    it never saw, from the same state after a burn-in of plain windows:
+
+    * open loop (every window accepted), its trajectory stays far closer to
+      the plain solver's than linear extrapolation of the previous increment
+      (the network's input, which it corrects) does;
+    * judged at the benchmark tolerance (1e-3), it is accepted several times
+      as often as extrapolation and keeps h within the 1 % bar.  Judged
+      extrapolation falls back to exact plain steps in most windows, and the
+      one-step judge does not see smooth errors, so its trajectory error is
+      the smaller one: that comparison is reported, not asserted."""
+    from soap_bubble_simulation_amd import _lib as L
     m, _ = trained
-    _, sy = m.scales()
-    fm = small_foam(4, 999)
-    K = SMALL["window"]
-    err_net, err_id = [], []
-    for w in range(8):
-        h0, _ = fm.fields()
-        out = m.forward(m.features(fm, 0, 4))
-        fm.step(K)
-        h1, _ = fm.fields()
-        pred = np.array([out[k, ..., 1] * fm.split(h0)[k].mean() * sy for k in range(4)])
-        true = np.array([fm.split(h1)[k] - fm.split(h0)[k] for k in range(4)])
-        err_net.append(np.linalg.norm(pred - true) / np.linalg.norm(true))
-        err_id.append(1.0)      # identity predicts no change
-    print("held-out increment rel-L2 per window, network:", np.round(err_net, 3))
-    assert np.mean(err_net) < 0.9 * np.mean(err_id)
+    K, windows, burn = SMALL["window"], 8, 4
+
+    def start():
+        fm = small_foam(4, 999)
+        fm.step(burn * K)
+        return fm
+
+    ref = start()
+    h0 = ref.fields()[0]
+    ref.step(windows * K)
+    h_ref = ref.fields()[0]
+    ref.close()
+    inc = np.linalg.norm(h_ref - h0)
+    res = {}
+    for name, pred in (("network", L.PREDICT_NETWORK), ("extrapolate", L.PREDICT_EXTRAPOLATE)):
+        for tol in (1e9, 1e-3):
+            fm = start()
+            st = m.advance_foam(fm, windows, tol=tol, predictor=pred)
+            h = fm.fields()[0]
+            res[name, tol] = dict(increment=float(np.linalg.norm(h - h_ref) / inc),
+                                  h=float(np.linalg.norm(h - h_ref) / np.linalg.norm(h_ref)),
+                                  accepted=int(st.accepted))
+            fm.close()
+    print("held-out, %d bubble-windows: %s" % (4 * windows, res))
+    assert res["network", 1e9]["increment"] < 0.1 * res["extrapolate", 1e9]["increment"]
+    assert res["network", 1e-3]["accepted"] >= 3 * max(1, res["extrapolate", 1e-3]["accepted"])
+    assert res["network", 1e-3]["h"] <= 1e-2
 
 
 @pytest.mark.gpu
