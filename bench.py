@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--train-windows", type=int, default=300,
                     help="c4/c5: online training windows before the timed surrogate run")
     ap.add_argument("--tol", type=float, default=1e-3, help="c4/c5: judge tolerance")
+    ap.add_argument("--fno-batch", type=int, default=64, help="c4/c5: bubbles per network chunk")
+    ap.add_argument("--no-open-loop", action="store_true", help="c4/c5: skip the open-loop runs")
     ap.add_argument("--bubbles", type=int, default=None,
                     help="bubbles of the c3 (default 4096) or c5 (default 1024) workload")
     ap.add_argument("--no-autotune", action="store_true", help="c3: keep the default tile shapes")
@@ -76,17 +78,33 @@ def parse():
                     help="c2: bubbles of the reduced surrogate run (256x512 each)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run the float4 copy kernel (counter calibration)")
-    ap.add_argument("--copy-bytes", type=int, default=256 << 20)
+                    help="only run the streaming-bandwidth probes (copy, read, write, 3-read/2-write)")
+    ap.add_argument("--copy-bytes", type=int, default=2 << 30)
+    ap.add_argument("--probe-modes", default="0,1,2,3,4",
+                    help="--copy-probe: streams to run (0 copy, 1 copy non-temporal, 2 read, 3 write, 4 3-read/2-write)")
+    ap.add_argument("--probe-unrolls", default="4,8,16", help="--copy-probe: accesses in flight per thread")
     ap.add_argument("--flags", type=int, default=None,
                     help="override the SF_FLAG_* bits of the workload")
     return ap.parse_args()
 
 
-def copy_probe(L, nbytes=256 << 20, iters=20):
-    ms = C.c_double()
-    L.check(L.lib().sf_copy_probe(nbytes, iters, C.byref(ms)))
-    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+PROBE_MODES = {0: "copy", 1: "copy, non-temporal", 2: "read", 3: "write, non-temporal", 4: "3-read/2-write mix"}
+
+
+def stream_probe(mode, unroll, nbytes, iters=20):
+    """One streaming-bandwidth probe of libsfprobe.so (probe/bandwidth.hip):
+    (average ms per launch, GB/s of the bytes it reads and writes)."""
+    from soap_bubble_simulation_amd import build as B
+    if not os.path.exists(B.PROBE_LIB):
+        raise RuntimeError("libsfprobe.so is not built (python -m soap_bubble_simulation_amd.build)")
+    lib = C.CDLL(B.PROBE_LIB)
+    lib.sfp_stream.argtypes = [C.c_int32, C.c_int32, C.c_size_t, C.c_int32, C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)]
+    ms, nb = C.c_double(), C.c_double()
+    rc = lib.sfp_stream(mode, unroll, nbytes, iters, C.byref(ms), C.byref(nb))
+    if rc != 0:
+        raise RuntimeError("sfp_stream(%d, %d) failed: %d" % (mode, unroll, rc))
+    return ms.value, nb.value / (ms.value * 1e-3) / 1e9
 
 
 def pmc_traffic(workload):
@@ -430,7 +448,7 @@ def fno_window_flops(cfg, cells):
     return cells * (cfg.layers * per_cell_layer + 2 * 5 * C + 2 * C * 2)
 
 
-def surrogate_runs(L, W, n_bubbles, steps, train_windows, tol, open_loop=True):
+def surrogate_runs(L, W, n_bubbles, steps, train_windows, tol, open_loop=True, max_batch=64):
     """Surrogate + judge on a foam of n_bubbles 256x512 bubbles against the
     plain solver on identical seeded inputs, with the no-network baselines
     (identity, linear extrapolation) judged the same way, and the network and
@@ -440,7 +458,7 @@ def surrogate_runs(L, W, n_bubbles, steps, train_windows, tol, open_loop=True):
     from soap_bubble_simulation_amd.surrogate import FnoSurrogate
 
     T, P = 256, 512
-    model = FnoSurrogate(n_theta=T, n_phi=P, max_batch=16)
+    model = FnoSurrogate(n_theta=T, n_phi=P, max_batch=max_batch)
     K = model.config.window
 
     def make(nb, seed, shapes=None):
@@ -524,7 +542,8 @@ def surrogate_runs(L, W, n_bubbles, steps, train_windows, tol, open_loop=True):
 def bench_surrogate(args, L, W, n_bubbles):
     """Surrogate + judge (c4: one bubble, c5: a batch of bubbles) against the
     plain solver on identical seeded inputs, with the no-network baselines."""
-    model, runs, plain, info = surrogate_runs(L, W, n_bubbles, args.steps, args.train_windows, args.tol)
+    model, runs, plain, info = surrogate_runs(L, W, n_bubbles, args.steps, args.train_windows, args.tol,
+                                              open_loop=not args.no_open_loop, max_batch=args.fno_batch)
     net = runs["network"]
     cells, windows, steps = info["cells"], info["windows"], info["steps"]
     flops = fno_window_flops(model.config, cells) * windows
@@ -567,8 +586,15 @@ def main():
         print("no GPU visible: bench.py needs an MI355X", file=sys.stderr)
         sys.exit(1)
     if args.copy_probe:
-        ms, gbs = copy_probe(L, args.copy_bytes)
-        print(json.dumps({"copy_probe_ms": ms, "copy_probe_GBps": gbs, "bytes": 2 * args.copy_bytes}))
+        # Every stream shape at arrays of --copy-bytes each (HBM-resident
+        # beyond the 256 MB last-level cache).
+        rows = []
+        for mode in [int(v) for v in args.probe_modes.split(",")]:
+            for unroll in [int(v) for v in args.probe_unrolls.split(",")]:
+                ms, gbs = stream_probe(mode, unroll, args.copy_bytes)
+                rows.append({"mode": PROBE_MODES[mode], "in_flight_per_thread": unroll, "ms": ms, "GBps": gbs})
+        print(json.dumps({"array_bytes": args.copy_bytes, "streams": rows,
+                          "best_GBps": max(r["GBps"] for r in rows)}))
         return
 
     if args.workload == "c3":

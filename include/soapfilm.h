@@ -98,10 +98,6 @@ const char* sf_last_error(void);
 const char* sf_version(void);
 /* 1 when a HIP device is usable, 0 otherwise (never fails). */
 int sf_device_available(void);
-/* Device-to-device float4 copy of n_bytes, timed over iters launches with
- * HIP events: the achievable-bandwidth reference of the roofline and the
- * calibration kernel of the FETCH_SIZE / WRITE_SIZE counters. */
-int sf_copy_probe(size_t n_bytes, int32_t iters, double* avg_ms);
 
 /* Defaults of DrainageSimulator::new(&SimulationConfig::default())
  * (drainage.rs:263-288, config.rs:305-316): 10 nm rupture, TFE on, h poles
@@ -562,11 +558,13 @@ int sf_fno_set_scales(sf_fno* model, const double* scales);
 /* Host-array forward pass: in[batch][N][8] -> out[batch][N][2]. */
 int sf_fno_forward(sf_fno* model, int32_t batch, const float* in, float* out);
 /* Kernel paths of this model's shape: the fused per-row layer kernel (training
- * and inference), the lift folded into the first phi DFT and the fused theta
- * stage (inference only); shapes outside them run the general batched GEMMs. */
+ * and inference), the lift folded into the first phi DFT, the fused theta
+ * stage and the one-wave-per-row layer kernel (inference only); shapes
+ * outside them run the general batched GEMMs. */
 #define SF_FNO_PATH_FUSED_ROWS   (1 << 0)
 #define SF_FNO_PATH_FUSED_THETA  (1 << 1)
 #define SF_FNO_PATH_LIFT_DFT     (1 << 2)
+#define SF_FNO_PATH_WAVE_ROWS    (1 << 3)   /* inference layers one wave per grid row */
 int sf_fno_kernel_paths(sf_fno* model, int32_t* paths);
 /* Loss mean((out - target)^2) and its gradient w.r.t. every parameter. */
 int sf_fno_gradient(sf_fno* model, int32_t batch, const float* in, const float* target, double* loss,

@@ -35,6 +35,7 @@ NOALLOC_MAX_THETA = 16384
 FNO_PATH_FUSED_ROWS = 1 << 0
 FNO_PATH_FUSED_THETA = 1 << 1
 FNO_PATH_LIFT_DFT = 1 << 2
+FNO_PATH_WAVE_ROWS = 1 << 3
 
 
 class SoapFilmError(RuntimeError):
@@ -170,7 +171,6 @@ SIGNATURES = {
     "sf_last_error": (C.c_char_p, []),
     "sf_version": (C.c_char_p, []),
     "sf_device_available": (C.c_int, []),
-    "sf_copy_probe": (C.c_int, [C.c_size_t, C.c_int32, _dp]),
     "sf_params_cpu_default": (C.c_int, [C.POINTER(Params)]),
     "sf_params_gpu_default": (C.c_int, [C.POINTER(Params)]),
     "sf_drainage_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_double,

@@ -50,6 +50,24 @@ def _compile(src: str) -> str:
     return obj
 
 
+PROBE_SRC = os.path.join(PKG_DIR, "probe", "bandwidth.hip")
+PROBE_LIB = os.path.join(PKG_DIR, "probe", "libsfprobe.so")
+
+
+def build_probe() -> str:
+    """Compile the streaming-bandwidth probes (probe/bandwidth.hip) into
+    libsfprobe.so: measurement tooling for the bench, not part of the
+    soapfilm library."""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC, *COMMON_FLAGS, "-shared", PROBE_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed on probe/bandwidth.hip")
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
+
+
 CPP_TEST_SRC = os.path.join(REPO_DIR, "tests", "cpp", "test_api.cpp")
 CPP_TEST_BIN = os.path.join(REPO_DIR, "tests", "cpp", "test_api")
 
@@ -81,6 +99,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB_PATH)
     if os.path.exists(CPP_TEST_SRC):
         build_cpp_test()
+    build_probe()
     if verbose:
         print("built", LIB_PATH)
     return LIB_PATH

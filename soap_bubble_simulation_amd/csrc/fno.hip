@@ -373,6 +373,7 @@ struct Fno {
     unsigned short* Iphi16 = nullptr;   // fused layer kernel's MFMA operands)
     unsigned short* Eth16 = nullptr;    // Eth and Ath rounded to BF16 (the fused
     unsigned short* Ath16 = nullptr;    // inference theta stage's MFMA operands)
+    float* wfold = nullptr;             // layer-0 weights with the lift folded in ([C][C] + [C])
 
     // activations (max batch Bm)
     float* in = nullptr;       // [Bm][N][kCin]
@@ -410,7 +411,7 @@ struct Fno {
 
     ~Fno() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, Eth16, Ath16, in, out, target, dout, ybuf, X1, G, Y, Z,
+        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, Eth16, Ath16, wfold, in, out, target, dout, ybuf, X1, G, Y, Z,
                       dxa, dxb, dz, dXs, part, loss_part, dscratch, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
                       sel_hi, sel_lo, hp_hi, hp_lo, d_win, d_take, rpart};
         for (void* p : ps) (void)hipFree(p);
@@ -573,6 +574,7 @@ struct Fno {
         adam_m = galloc<float>(n_params);
         adam_v = galloc<float>(n_params);
         init_transforms();
+        wfold = galloc<float>(static_cast<long long>(C) * C + C);
         const long long R = rows();
         in = galloc<float>(R * kCin);
         out = galloc<float>(R * kCout);
@@ -649,6 +651,26 @@ struct Fno {
 
     // Inference theta stage of layer l: the fused kernel where its shape
     // constraints and LDS tile allow, else the general sequence.
+    using RowWaveKernel = void (*)(RowArgs, int, long long);
+
+    // Inference layer kernel with one wave per grid row for layer l of L, or
+    // null (then the block-per-row kernel runs): width and 2 modes_phi both
+    // 32 or both 16.
+    RowWaveKernel row_wave_kernel(int l = 1, int nl = 3) const {
+        if (P % 16) return nullptr;
+        const bool first = l == 0, last = l == nl - 1;
+#define SF_WAVEK(c, j)                                                                   \
+    if (C == 16 * c && 2 * Mp == 16 * j) {                                                \
+        if (first && last) return &fno_row_wave_kernel<c, j, true, true>;               \
+        if (first) return &fno_row_wave_kernel<c, j, true, false>;                      \
+        if (last) return &fno_row_wave_kernel<c, j, false, true>;                       \
+        return &fno_row_wave_kernel<c, j, false, false>;                                \
+    }
+        SF_WAVEK(2, 2) SF_WAVEK(1, 1)
+#undef SF_WAVEK
+        return nullptr;
+    }
+
     size_t theta_fused_lds() const {
         return static_cast<size_t>(2 * C) * (T + 8) * 2 + static_cast<size_t>(2 * Mt) * (2 * C + 1) * 4 +
                static_cast<size_t>(C) * (2 * Mt + 8) * 2;
@@ -663,6 +685,7 @@ struct Fno {
             if (static_cast<size_t>(P * kCin + 2 * Mp * (kCin + 1)) * sizeof(float) <= 64 * 1024 && P <= kLiftDftMaxP)
                 p |= SF_FNO_PATH_LIFT_DFT;
             if (theta_fused_ok()) p |= SF_FNO_PATH_FUSED_THETA;
+            if (row_wave_kernel()) p |= SF_FNO_PATH_WAVE_ROWS;
         }
         return p;
     }
@@ -690,10 +713,17 @@ struct Fno {
             const int J = 2 * Mp;
             const size_t lds = static_cast<size_t>(P * kCin + J * (kCin + 1)) * sizeof(float);
             SF_REQUIRE(lds <= 64 * 1024 && P <= kLiftDftMaxP, SF_ERR_INTERNAL, "lift DFT tile too large");
-            hipLaunchKernelGGL(lift_dft_kernel<kCin>, dim3(static_cast<unsigned>(B * T)), dim3(256), lds, stream, in,
-                               Ephi, w + o_wl, w + o_bl, X1, P, J, C);
+            const long long nrows = static_cast<long long>(B) * T;
+            if (P % 16 == 0 && J == kLiftWaveJ && 64 % C == 0)
+                hipLaunchKernelGGL(lift_dft_wave_kernel<kCin>,
+                                   dim3(static_cast<unsigned>((nrows + 4 * kLiftRows - 1) / (4 * kLiftRows))), dim3(256),
+                                   0, stream, in, Ephi, w + o_wl, w + o_bl, X1, P, C, nrows);
+            else
+                hipLaunchKernelGGL(lift_dft_kernel<kCin>, dim3(static_cast<unsigned>(B * T)), dim3(256), lds, stream, in,
+                                   Ephi, w + o_wl, w + o_bl, X1, P, J, C);
             SF_HIP(hipGetLastError());
             for (int l = 0; l < L; ++l) {
+                const RowWaveKernel wk = row_wave_kernel(l, L);
                 theta_stage_inference(l, B);
                 const bool last = l == L - 1;
                 // Activations between layers in BF16 (the precision the next
@@ -701,7 +731,18 @@ struct Fno {
                 RowArgs ra{xs[l], Z, w + o_w[l], w + o_b[l], Iphi16, Ephi16, last ? nullptr : xs[l + 1], nullptr,
                            last ? nullptr : X1, last ? 0 : 1,
                            l == 0 ? in : nullptr, w + o_wl, w + o_bl, w + o_q, w + o_qb, last ? out : nullptr, 1};
-                hipLaunchKernelGGL(rk, dim3(static_cast<unsigned>(B * T)), dim3(row_threads), 0, stream, ra);
+                if (wk) {
+                    if (l == 0) {
+                        // layer 0 runs on the lift-folded weights
+                        hipLaunchKernelGGL(lift_fold_kernel, dim3(1), dim3(256), 0, stream, w + o_wl, w + o_bl,
+                                           w + o_w[0], w + o_b[0], wfold, wfold + static_cast<size_t>(C) * C, C);
+                        ra.W = wfold;
+                        ra.bias = wfold + static_cast<size_t>(C) * C;
+                    }
+                    hipLaunchKernelGGL(wk, dim3(static_cast<unsigned>((nrows + 3) / 4)), dim3(256), 0, stream, ra, P,
+                                       nrows);
+                } else
+                    hipLaunchKernelGGL(rk, dim3(static_cast<unsigned>(B * T)), dim3(row_threads), 0, stream, ra);
                 SF_HIP(hipGetLastError());
             }
             return;

@@ -343,6 +343,240 @@ __global__ __launch_bounds__(64 * WAVES) void fno_row_kernel(const RowArgs a) {
         X1[q] = sum;
     }
 }
+// ---- inference layer, one wave per (bubble, theta row) ----------------------
+//
+// The inference form of fno_row_kernel (BF16 activations, no saved
+// pre-activations).  A wave owns a whole grid row and walks its P cells in
+// 16-cell tiles:
+//   z  = [x | Iphi] [W ; Z[b][t]] + bias    on v_mfma_f32_16x16x32_bf16 (K = C + J
+//                                           in 32-wide steps; lane (m, g) supplies
+//                                           A[m][8 g .. 8 g + 7], B[8 g .. 8 g + 7][m])
+//   x' = act(z)
+//   X1'[b][t] += Ephi[:, tile] x'[tile]     on v_mfma_f32_16x16x16_bf16, x' taken
+//                                           straight from the first product's
+//                                           result layout D[4(l/16)+v][l%16],
+//                                           which is that product's B layout
+// so the next layer's phi DFT accumulates inside the wave: no cross-wave
+// reduction, no barrier after the set-up.  The B operand of the first
+// product (W stacked on the row's inverse-DFT input Z) is held in registers
+// for the whole row.  x' tiles go out through a per-wave LDS transpose as
+// 16-byte stores.  Layer 0 lifts x from the kLiftIn input features on the
+// fly (LIFT: with x = in W_l + b_l, x W = in (W_l W) + b_l W, so it runs on
+// the folded weights of lift_fold_kernel and its A operand reads the
+// features); the last layer forms the projection instead of the next phi
+// DFT (PROJ) and stores x' only when x_out is set.
+// Block: 4 waves = 4 rows.  Needs P % 16 == 0 and (C + J) % 32 == 0.
+// Layer-0 weights with the linear lift folded in, for the LIFT form of
+// fno_row_wave_kernel: Wf[k][n] = sum_c W_l[k][c] W[c][n] for k < kLiftIn
+// (rows kLiftIn .. C - 1 zero), bf[n] = b[n] + sum_c b_l[c] W[c][n]; FP32.
+__global__ __launch_bounds__(256) void lift_fold_kernel(const float* __restrict__ Wl, const float* __restrict__ bl,
+                                                        const float* __restrict__ W, const float* __restrict__ b,
+                                                        float* __restrict__ Wf, float* __restrict__ bf, int C) {
+    for (int t = threadIdx.x; t < (C + 1) * C; t += blockDim.x) {
+        const int k = t / C, n = t - k * C;
+        if (k < C) {
+            float v = 0.f;
+            if (k < kLiftIn)
+                for (int c = 0; c < C; ++c) v = fmaf(Wl[k * C + c], W[c * C + n], v);
+            Wf[t] = v;
+        } else {
+            float v = b[n];
+            for (int c = 0; c < C; ++c) v = fmaf(bl[c], W[c * C + n], v);
+            bf[n] = v;
+        }
+    }
+}
+
+__device__ inline float gelu_exp(float z) {
+    // 0.5 z (1 + tanh u) = z / (1 + exp(-2u)), u = k0 (z + k1 z^3); hardware
+    // exp2 and reciprocal (about 1e-6 relative, far below the BF16 rounding
+    // the result is stored with)
+    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+    const float u = k0 * fmaf(k1 * z, z * z, z);
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * u);
+    return z * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int CT, int JT, bool LIFT, bool PROJ>
+__global__ __launch_bounds__(256) void fno_row_wave_kernel(const RowArgs a, int P, long long rows) {
+    constexpr int C = 16 * CT, J = 16 * JT, K = C + J, KS = K / 32;
+    static_assert(K % 32 == 0, "K = C + J must be a multiple of 32");
+    constexpr int TRW = C + 8;                           // padded row of the transpose tile (bf16)
+    __shared__ __align__(16) unsigned short tr[4][16 * TRW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mr = lane & 15, g = lane >> 4;
+    constexpr bool lift = LIFT;
+    const long long row = blockIdx.x * 4LL + wave;
+    if (row >= rows) return;                  // no block barrier: waves are independent
+
+    // B fragments: lane (n = mr, g) holds B[32 s + 8 g + i][16 ct + mr], i < 8.
+    // In layer 0 (LIFT) W and bias are the lift-folded weights of
+    // lift_fold_kernel: rows k < kLiftIn of W multiply the features.
+    static_assert(!LIFT || kLiftIn == 8, "the lifted features fill one 8-wide lane group");
+    bf16x8 Bf[KS][CT];
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            u32x4 pk;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                unsigned short e2[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int k = 32 * st + 8 * g + 2 * h + u, n = 16 * ct + mr;
+                    const float v = k < C ? a.W[k * C + n] : a.Z[row * J * C + static_cast<long long>(k - C) * C + n];
+                    e2[u] = to_bf16(v);
+                }
+                pk[h] = static_cast<unsigned>(e2[0]) | (static_cast<unsigned>(e2[1]) << 16);
+            }
+            Bf[st][ct] = __builtin_bit_cast(bf16x8, pk);
+        }
+    float bias[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) bias[ct] = a.bias[16 * ct + mr];
+    float qv[CT][kProjOut];
+    if (PROJ)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int j = 0; j < kProjOut; ++j) qv[ct][j] = a.Q[(16 * ct + mr) * kProjOut + j];
+    f32x4 acc2[JT][CT];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc2[jt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const unsigned short* x16 = reinterpret_cast<const unsigned short*>(a.x) + row * P * C;
+    unsigned short* xo16 = a.x_out ? reinterpret_cast<unsigned short*>(a.x_out) + row * P * C : nullptr;
+    unsigned short* t = tr[wave];
+    for (int p0 = 0; p0 < P; p0 += 16) {
+        // ---- z = [x | Iphi] B + bias: A[m][k], m = cell p0 + mr
+        const int p = p0 + mr;
+        f32x4 acc[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < KS; ++st) {
+            const int k0 = 32 * st + 8 * g;                 // this lane's 8 k values
+            u32x4 av;
+            if (k0 < C) {
+                if (lift) {
+                    // the kLiftIn features of cell p in the first lane group
+                    av = u32x4{0u, 0u, 0u, 0u};
+                    if (k0 == 0) {
+                        const float4* ip = reinterpret_cast<const float4*>(a.in + (row * P + p) * kLiftIn);
+                        const float4 f0 = ip[0], f1 = ip[1];
+                        av[0] = static_cast<unsigned>(to_bf16(f0.x)) | (static_cast<unsigned>(to_bf16(f0.y)) << 16);
+                        av[1] = static_cast<unsigned>(to_bf16(f0.z)) | (static_cast<unsigned>(to_bf16(f0.w)) << 16);
+                        av[2] = static_cast<unsigned>(to_bf16(f1.x)) | (static_cast<unsigned>(to_bf16(f1.y)) << 16);
+                        av[3] = static_cast<unsigned>(to_bf16(f1.z)) | (static_cast<unsigned>(to_bf16(f1.w)) << 16);
+                    }
+                } else {
+                    av = *reinterpret_cast<const u32x4*>(x16 + static_cast<long long>(p) * C + k0);
+                }
+            } else {
+                av = *reinterpret_cast<const u32x4*>(a.Iphi + static_cast<long long>(p) * J + (k0 - C));
+            }
+            const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bf[st][ct], acc[ct], 0, 0, 0);
+        }
+        // ---- bias, activation; lane (mr, g) holds x'[p0 + 4 g + v][16 ct + mr]
+        bf16x4 Bx[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            unsigned short bv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float z = acc[ct][v] + bias[ct];
+                const float xv = a.act ? gelu_exp(z) : z;
+                acc[ct][v] = xv;
+                bv[v] = to_bf16(xv);
+                if (xo16) t[(4 * g + v) * TRW + 16 * ct + mr] = bv[v];
+            }
+            Bx[ct] = bf16x4{static_cast<short>(bv[0]), static_cast<short>(bv[1]), static_cast<short>(bv[2]),
+                            static_cast<short>(bv[3])};
+        }
+        if (xo16) {
+            // transpose through the wave's LDS tile: 16-byte stores of 8 channels
+            __builtin_amdgcn_wave_barrier();
+            constexpr int LPC = C / 8;                  // lanes per cell
+            constexpr int CPP = 64 / LPC;               // cells per pass (more than 16 when C = 16)
+#pragma unroll
+            for (int c0 = 0; c0 < 16; c0 += CPP) {
+                const int cell = c0 + lane / LPC, ch = 8 * (lane % LPC);
+                if (cell < 16) {
+                    const u32x4 v = *reinterpret_cast<const u32x4*>(&t[cell * TRW + ch]);
+                    *reinterpret_cast<u32x4*>(xo16 + static_cast<long long>(p0 + cell) * C + ch) = v;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (PROJ) {
+            // proj[p][j] = q[j] + sum_o x'[p][o] Q[o][j]: per-lane partials
+            // e = 2 v + j over ct, then a reduce-scatter over the 16 lanes of
+            // the group (halving the partials at each of the xor-8/4/2 steps,
+            // 7 shuffles in all): lane mr ends with output e = mr / 2, and the
+            // even lanes store outputs 8 g .. 8 g + 7 of the tile
+            static_assert(kProjOut == 2, "the reduce-scatter pairs 8 partials with 16 lanes");
+            float e8[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    float sum = 0.f;
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) sum = fmaf(acc[ct][v], qv[ct][j], sum);
+                    e8[2 * v + j] = sum;
+                }
+            const bool b8 = (mr & 8) != 0, b4 = (mr & 4) != 0, b2 = (mr & 2) != 0;
+            float e4[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float send = b8 ? e8[k] : e8[k + 4];
+                const float keep = b8 ? e8[k + 4] : e8[k];
+                e4[k] = keep + __shfl_xor(send, 8, 16);
+            }
+            float e2[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const float send = b4 ? e4[k] : e4[k + 2];
+                const float keep = b4 ? e4[k + 2] : e4[k];
+                e2[k] = keep + __shfl_xor(send, 4, 16);
+            }
+            const float send = b2 ? e2[0] : e2[1];
+            float e1 = (b2 ? e2[1] : e2[0]) + __shfl_xor(send, 2, 16);
+            e1 += __shfl_xor(e1, 1, 16);
+            const int e = mr >> 1;
+            if ((mr & 1) == 0) a.proj[(row * P + p0 + 4 * g) * kProjOut + e] = e1 + a.qb[e & 1];
+        }
+        if (!PROJ) {
+            // next layer's phi DFT: A[j][k] = Ephi[16 jt + mr][p0 + 4 g + v]
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                const bf16x4 E = *reinterpret_cast<const bf16x4*>(a.Ephi + static_cast<long long>(16 * jt + mr) * P + p0 +
+                                                                  4 * g);
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    acc2[jt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(E, Bx[ct], acc2[jt][ct], 0, 0, 0);
+            }
+        }
+    }
+    if (!PROJ) {
+        float* X1 = a.X1_out + row * J * C;
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) X1[(16 * jt + 4 * g + v) * C + 16 * ct + mr] = acc2[jt][ct][v];
+    }
+}
+
 __device__ inline float gelu_grad(float z) {
     const float k0 = 0.7978845608028654f, k1 = 0.044715f;
     const float u = k0 * (z + k1 * z * z * z);
@@ -465,6 +699,85 @@ __global__ __launch_bounds__(256) void lift_dft_kernel(const float* __restrict__
 #pragma unroll
         for (int i = 0; i < NIN; ++i) s = fmaf(ej[i], Wl[i * C + c], s);
         X1[row * J * C + t] = s;
+    }
+}
+
+// The same product on FP32 MFMA (v_mfma_f32_16x16x4_f32, lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]), one wave per kLiftRows rows of a
+// bubble's grid, so each 16-byte load of the Ephi rows (A) serves kLiftRows
+// products: S = Ephi [in | 1] (J x (NIN + 1), K = P split so that lane group
+// g sums cells g P/4 .. (g + 1) P/4 - 1), then X1 = S [W_l ; b_l] from a
+// per-wave LDS copy of S.  Needs P % 16 == 0, J == 32, NIN < 16 and
+// 64 % C == 0.
+constexpr int kLiftWaveJ = 32;
+constexpr int kLiftRows = 4;
+
+template <int NIN>
+__global__ __launch_bounds__(256) void lift_dft_wave_kernel(const float* __restrict__ in, const float* __restrict__ Ephi,
+                                                            const float* __restrict__ Wl, const float* __restrict__ bl,
+                                                            float* __restrict__ X1, int P, int C, long long rows) {
+    static_assert(NIN < 16, "the inputs and the constant column fill one 16-wide tile");
+    constexpr int SW = 17;                                // padded row of S in LDS
+    constexpr int J = kLiftWaveJ, JT = J / 16;
+    __shared__ float s_S[4][kLiftRows][J * SW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mr = lane & 15, g = lane >> 4;
+    const long long row0 = (blockIdx.x * 4LL + wave) * kLiftRows;
+    if (row0 >= rows) return;                             // no barrier below: waves are independent
+    const int nr = static_cast<int>(rows - row0 < kLiftRows ? rows - row0 : kLiftRows);
+    const int P4 = P >> 2;
+    f32x4 acc[kLiftRows][JT];
+#pragma unroll
+    for (int r = 0; r < kLiftRows; ++r)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[r][jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const float* e0 = Ephi + static_cast<long long>(mr) * P + g * P4;
+    for (int kk = 0; kk < P4; kk += 4) {
+        float4 ev[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) ev[jt] = *reinterpret_cast<const float4*>(e0 + static_cast<long long>(16 * jt) * P + kk);
+#pragma unroll
+        for (int r = 0; r < kLiftRows; ++r) {
+            if (r < nr) {
+                const float* inr = in + (row0 + r) * P * NIN + static_cast<long long>(g * P4 + kk) * NIN + mr;
+                float bv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) bv[u] = mr < NIN ? inr[u * NIN] : (mr == NIN ? 1.f : 0.f);
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) {
+                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].x, bv[0], acc[r][jt], 0, 0, 0);
+                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].y, bv[1], acc[r][jt], 0, 0, 0);
+                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].z, bv[2], acc[r][jt], 0, 0, 0);
+                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].w, bv[3], acc[r][jt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // lane (mr, g) holds S_r[16 jt + 4 g + v][mr]
+#pragma unroll
+    for (int r = 0; r < kLiftRows; ++r)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s_S[wave][r][(16 * jt + 4 * g + v) * SW + mr] = acc[r][jt][v];
+    __builtin_amdgcn_wave_barrier();
+    // X1[j][c] = S[j][NIN] b_l[c] + sum_f S[j][f] W_l[f][c]; c = lane % C is
+    // fixed per lane (64 % C == 0), its W_l column held in registers
+    const int c = lane % C;
+    float wl[NIN];
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) wl[f] = Wl[f * C + c];
+    const float b = bl[c];
+    for (int r = 0; r < nr; ++r) {
+        const float* S = s_S[wave][r];
+        float* xr = X1 + (row0 + r) * J * C;
+        for (int t = lane; t < J * C; t += 64) {
+            const float* sj = S + (t / C) * SW;
+            float sum = sj[NIN] * b;
+#pragma unroll
+            for (int f = 0; f < NIN; ++f) sum = fmaf(sj[f], wl[f], sum);
+            xr[t] = sum;
+        }
     }
 }
 

@@ -328,7 +328,8 @@ def test_gpu_benchmark_shape_matches_reference(batch):
     from soap_bubble_simulation_amd.surrogate import FnoSurrogate
     m = FnoSurrogate(**C4_CFG)
     paths = m.kernel_paths()
-    assert paths == L.FNO_PATH_FUSED_ROWS | L.FNO_PATH_FUSED_THETA | L.FNO_PATH_LIFT_DFT, paths
+    assert paths == (L.FNO_PATH_FUSED_ROWS | L.FNO_PATH_FUSED_THETA | L.FNO_PATH_LIFT_DFT
+                     | L.FNO_PATH_WAVE_ROWS), paths
     s = R.Shapes.of(m.config)
     f32 = scaled_params(s, seed=20 + batch)
     m.set_params(f32)
