@@ -271,8 +271,9 @@ int sf_drainage_synchronize(sf_drainage* ctx);
  * 256 x 64 RGBA8 LUT with clamp-to-edge addressing (pipeline.rs:350-359),
  * times `intensity` (pipeline.rs:75 default 4), clamped to [0, 1] -- written
  * as RGBA8 with alpha 255 into rgba_out[n_theta*n_phi*4], the frame layout
- * export_frame writes to PNG (image_export.rs:42).  lut_rgba8 NULL: the LUT
- * of sf_lut_rgba8(refractive_index, 1) as the renderer builds it
+ * export_frame writes to PNG (image_export.rs:42).  lut_rgba8: the renderer's
+ * angle-major 64 x 256 RGBA8 table, 65536 bytes are read; NULL: the LUT of
+ * sf_lut_rgba8(refractive_index, 1) as the renderer builds it
  * (pipeline.rs:311-314).  cam NULL: (0, 0, 4R). */
 int sf_render_lut_rgba8(sf_drainage* ctx, const uint8_t* lut_rgba8, double refractive_index, float intensity,
                         const sf_camera* cam, uint8_t* rgba_out);

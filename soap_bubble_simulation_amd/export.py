@@ -17,6 +17,10 @@ import numpy as np
 from . import _lib as L
 
 
+LUT_SHAPE = (64, 256, 4)          # angle-major RGBA8 runtime LUT (interference_lut.rs:9-13)
+LUT_BYTES = 64 * 256 * 4
+
+
 class ExportError(Exception):
     """Kinds: "BufferCreation", "SaveError", "InvalidDimensions" (image_export.rs:6-14)."""
 
@@ -30,10 +34,18 @@ def _chunk(tag: bytes, data: bytes) -> bytes:
 
 
 def export_frame(path: str, width: int, height: int, data) -> None:
-    """Write RGBA8 pixel data (row-major, width*height*4 bytes) as a PNG."""
+    """Write RGBA8 pixel data (row-major, width*height*4 bytes) as a PNG.
+    ``data`` is bytes or a uint8 array (the reference takes ``&[u8]``); other
+    element types are refused rather than cast."""
     if width == 0 or height == 0:
         raise ExportError("InvalidDimensions", "Invalid dimensions: %dx%d" % (width, height))
-    buf = bytes(np.ascontiguousarray(data, np.uint8).tobytes()) if not isinstance(data, (bytes, bytearray)) else bytes(data)
+    if isinstance(data, (bytes, bytearray)):
+        buf = bytes(data)
+    else:
+        arr = np.asarray(data)
+        if arr.dtype != np.uint8:
+            raise TypeError("export_frame takes RGBA8 bytes (uint8), got %s" % arr.dtype)
+        buf = np.ascontiguousarray(arr).tobytes()
     expected = width * height * 4
     if len(buf) != expected:
         raise ExportError("BufferCreation", "Data length %d doesn't match expected %d (%dx%dx4)"
@@ -80,7 +92,14 @@ def render_frame(ctx, camera: Optional[Sequence[float]] = None, refractive_index
         for k in range(3):
             cam.position[k] = camera[k]
     out = np.empty((ctx.n_theta, ctx.n_phi, 4), np.uint8)
-    lu = None if lut is None else np.ascontiguousarray(lut, np.uint8)
+    lu = None
+    if lut is not None:
+        # sf_render_lut_rgba8 reads the renderer's whole 64 x 256 RGBA8 table
+        lu = np.asarray(lut)
+        if lu.dtype != np.uint8 or lu.size != LUT_BYTES or (lu.ndim > 1 and lu.shape != LUT_SHAPE):
+            raise ValueError("lut must be the 64 x 256 RGBA8 table (uint8, shape %s or %d bytes), got %s %s"
+                             % (LUT_SHAPE, LUT_BYTES, lu.dtype, lu.shape))
+        lu = np.ascontiguousarray(lu)
     L.check(L.lib().sf_render_lut_rgba8(ctx.handle, L.u8ptr(lu), float(refractive_index), float(intensity),
                                         None if cam is None else C.byref(cam), L.u8ptr(out)))
     return out

@@ -73,3 +73,23 @@ def test_gpu_render_matches_oracle(tmp_path):
     path = str(tmp_path / "bubble.png")
     export_frame(path, npf, nt, frame)
     assert np.array_equal(read_png_rgba(path), frame)
+
+
+def test_export_refuses_non_byte_data(tmp_path):
+    """The reference takes &[u8]: wider or float data is refused, not cast."""
+    with pytest.raises(TypeError):
+        export_frame(str(tmp_path / "t.png"), 1, 1, np.array([1.0, 0.0, 0.0, 1.0]))
+    with pytest.raises(TypeError):
+        export_frame(str(tmp_path / "t.png"), 1, 1, np.array([255, 0, 0, 255], np.int32))
+
+
+def test_render_frame_refuses_wrong_lut():
+    """render_frame checks the user LUT before the C call reads 65536 bytes."""
+    from soap_bubble_simulation_amd.export import render_frame
+
+    class Ctx:                                 # never reaches the library
+        n_theta, n_phi, handle = 4, 8, None
+    for bad in (np.zeros((64, 256, 3), np.uint8), np.zeros((1024, 90, 3), np.uint8),
+                np.zeros((64, 256, 4), np.float32), np.zeros(100, np.uint8)):
+        with pytest.raises(ValueError):
+            render_frame(Ctx(), lut=bad)
