@@ -85,6 +85,11 @@ def parse():
     ap.add_argument("--probe-unrolls", default="4,8,16", help="--copy-probe: accesses in flight per thread")
     ap.add_argument("--flags", type=int, default=None,
                     help="override the SF_FLAG_* bits of the workload")
+    ap.add_argument("--grid", default=None,
+                    help="c2 / c2-drainage: n_theta x n_phi other than 2048x4096, e.g. 8192x16384")
+    ap.add_argument("--steps-per-launch", type=int, default=1, choices=[1, 2],
+                    help="c2-drainage: 2 = two steps per stencil launch (temporal blocking; "
+                         "needs flags without TFE, DIAG and Marangoni, e.g. --flags 2)")
     return ap.parse_args()
 
 
@@ -616,10 +621,10 @@ def main():
         flags = L.FLAG_TFE | L.FLAG_DIAG
         desc = "single bubble 128x256 FP32 drainage (reference default grid), TFE fronts, CFL dt"
     else:
-        n_theta, n_phi = 2048, 4096
+        n_theta, n_phi = (2048, 4096) if not args.grid else (int(v) for v in args.grid.lower().split("x"))
         h0 = W.smooth_bubble(n_theta, n_phi, seed=3)
         flags = L.FLAG_TFE | L.FLAG_SURFACTANT | L.FLAG_DIAG
-        desc = ("single bubble 2048x4096 FP32 drainage, CFL dt"
+        desc = ("single bubble %dx%d FP32 drainage, CFL dt" % (n_theta, n_phi)
                 + (" + 81-wavelength spectral RGB per step (FP16 MFMA, FP32 accumulate)" if spectral else ""))
     if args.flags is not None:
         flags = args.flags
@@ -628,6 +633,10 @@ def main():
     p.flags = flags
     ctx = DrainageContext(p, n_theta, n_phi, 500e-9)
     ctx.set_fields(h0, np.full_like(h0, 0.5))
+    ctx.set_steps_per_launch(args.steps_per_launch)
+    spl = ctx.steps_per_launch()
+    if spl != args.steps_per_launch:
+        print("note: these flags and this grid take one step per launch", file=sys.stderr)
     dt = ctx.stats().dt_cfl * p.cfl_safety
     cam = W.camera(p.bubble_radius)
     cmf = W.spectral_cmf(3).astype(np.float32)
@@ -680,13 +689,19 @@ def main():
     cells = n_theta * n_phi
     value = cells * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
+    # A launch reads and writes the state once whether it advances one step
+    # or two: 16 B/cell per launch either way.
     achieved = BYTES_PER_CELL_STEP * cells / (drain_ms * 1e-3) / 1e9
-    traffic, tsrc = pmc_traffic(args.workload) if args.workload in PMC_SUMMARY else (None, None)
+    kernel = ("drainage_band2_kernel" if spl == 2 else
+              # 16-row float4 bands at 2048 x 4096, the scalar kernel with one
+              # row per block at 128 x 256 (grids of up to 262144 cells)
+              "drainage_band_kernel" if n_theta >= 1024 else "drainage_step_kernel")
+    traffic, tsrc = (pmc_traffic(args.workload)
+                     if args.workload in PMC_SUMMARY and not args.grid and args.flags is None and spl == 1
+                     else (None, None))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source(tsrc),
-                # the kernel the context selects: 16-row float4 bands at 2048 x 4096, the scalar kernel
-                # with one row per block at 128 x 256 (grids of up to 262144 cells)
-                "kernel": "drainage_band_kernel" if n_theta >= 1024 else "drainage_step_kernel",
+                "kernel": kernel, "steps_per_launch": spl,
                 "avg_launch_ms": drain_ms,
                 "launches_timed": drain_n,
                 "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells}
@@ -707,12 +722,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "n_theta": n_theta, "n_phi": n_phi, "dt_s": dt,
-                   "cells": cells, "flags": int(flags)},
+                   "cells": cells, "flags": int(flags), "steps_per_launch": spl},
         "roofline": roofline,
         "kernels": {
-            "drainage_step_ms": drain_ms,
+            "drainage_step_ms": drain_ms / spl,
             "spectral_ms": ms_spec.value if spectral else None,
-            "drainage_only_cell_steps_per_s": cells / (drain_ms * 1e-3),
+            "drainage_only_cell_steps_per_s": spl * cells / (drain_ms * 1e-3),
             "spectral_cells_per_s": cells / (ms_spec.value * 1e-3) if spectral and ms_spec.value else None,
         },
         "state": {"min_h": stats.min_thickness, "max_h": stats.max_thickness,

@@ -196,6 +196,16 @@ int sf_drainage_stream(sf_drainage* ctx, void** stream);
  * events on the context's stream since the last call (0 when none). */
 int sf_drainage_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
                               int64_t* launches);
+/* Steps advanced per stencil launch: 1, or 2 (temporal blocking: each band
+ * is read and written once per two steps, bit-identical to two one-step
+ * launches).  Two steps per launch serve flags without SF_FLAG_TFE,
+ * SF_FLAG_DIAG, SF_FLAG_MARANGONI and averaged Γ poles, on grids with
+ * n_phi % 256 == 0 and rebase_interval != 1; other contexts keep one step
+ * per launch.  sf_drainage_get_steps_per_launch returns what sf_drainage_step
+ * will use.  No counterpart in the reference, whose step dispatches one
+ * compute pass per step (gpu_drainage.rs:354-369). */
+int sf_drainage_set_steps_per_launch(sf_drainage* ctx, int32_t steps);
+int sf_drainage_get_steps_per_launch(sf_drainage* ctx, int32_t* steps);
 
 /* ---- interference --------------------------------------------------------- */
 

@@ -209,6 +209,8 @@ SIGNATURES = {
     "sf_optical_path": (C.c_int, [C.c_double, C.c_int32, _fp, _fp, _fp, _fp]),
     "sf_phase_difference": (C.c_int, [C.c_int32, _fp, _fp, _fp]),
     "sf_spectral_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "sf_drainage_set_steps_per_launch": (C.c_int, [_vp, C.c_int32]),
+    "sf_drainage_get_steps_per_launch": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_spectral_set_operand": (C.c_int, [_vp, C.c_int32]),
     "sf_spectral_get_operand": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_drainage_step_spectral": (C.c_int, [_vp, C.c_int32, C.c_double, C.POINTER(Camera),

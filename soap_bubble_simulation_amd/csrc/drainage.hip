@@ -227,6 +227,18 @@ static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag, bool ve
     return band[rebase][gmode][marangoni][diag];
 }
 
+// Two-step launches: [rebase][gmode 0 or 2].
+template <bool RB, int GM>
+static void launch_band2(dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const StepArgs& a) {
+    hipExtLaunchKernelGGL((drainage_band2_kernel<RB, GM>), grid, block, 0, s, t0, t1, 0, a);
+}
+
+static StepFn step2_fn(bool rebase, int gmode) {
+    static const StepFn tab[2][2] = {{&launch_band2<false, 0>, &launch_band2<false, 2>},
+                                     {&launch_band2<true, 0>, &launch_band2<true, 2>}};
+    return tab[rebase][gmode == 2 ? 1 : 0];
+}
+
 }  // namespace sf
 
 using namespace sf;
@@ -262,6 +274,8 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     rows_per_block = vec4 ? (static_cast<size_t>(ntiles) * cdiv(nt - 2, 16) >= 512 ? 16 : 4) : (small ? 1 : 16);
     if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK")) rows_per_block = std::max(1, std::atoi(e));
     if (vec4) rows_per_block = rows_per_block <= 4 ? 4 : (rows_per_block <= 8 ? 8 : std::min(rows_per_block, kBandMax));
+    if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK2"))
+        rows_per_block2 = std::min(std::max(1, std::atoi(e)), kBandMax);
     for (int k = 0; k < 2; ++k) {
         hi[k] = dalloc<float>(n);
         lo[k] = dalloc<float>(n);
@@ -542,6 +556,13 @@ double Drainage::resolve_dt(double dt) {
     return p.cfl_safety * cfl;
 }
 
+bool Drainage::two_step_ok() const {
+    const uint32_t f = p.flags;
+    const bool g_avg = (f & SF_FLAG_SURFACTANT) && (f & SF_FLAG_POLES_AVG_GAMMA);
+    return vec4 && np % 256 == 0 && ntiles <= kMaxTiles2 && nt >= 4 && !(f & SF_FLAG_TFE) &&
+           !(f & SF_FLAG_DIAG) && !(f & SF_FLAG_MARANGONI) && !g_avg && p.rebase_interval != 1;
+}
+
 void Drainage::step(int n_steps, double dt, bool flush) {
     if (tables_dirty) upload_tables();
     dt = resolve_dt(dt);
@@ -558,7 +579,57 @@ void Drainage::step(int n_steps, double dt, bool flush) {
 
     StepArgs a{};
     fill_common_args(a, dt_f);
+    const bool two = steps_per_launch == 2 && two_step_ok();
+    const dim3 grid2(ntiles, cdiv(nt - 2, rows_per_block2));
     for (int s = 0; s < n_steps; ++s) {
+        if (two && s + 1 < n_steps && since_rebase + 1 < rebase_every) {
+            // Steps s and s + 1 in one launch; a rebase can only follow the
+            // second, and neither step has fronts or diagnostics.
+            schedule_fronts(dt, a);
+            schedule_fronts(dt, a);
+            last_boxes.clear();
+            finalize_diag();
+            const bool rebase = since_rebase + 2 >= rebase_every;
+            a.rows_per_block = rows_per_block2;
+            a.hi_in = hi[cur_hi];
+            a.lo_in = lo[cur];
+            a.g_in = g[cur_g];
+            a.hi_out = hi[1 - cur_hi];
+            a.lo_out = lo[1 - cur];
+            a.g_out = g[1 - cur_g];
+            a.h_pole_mean = (poles_virtual && h_avg) ? 1 : 0;
+            a.g_pole_mean = 0;
+            a.h_pole_sum = h_avg ? 1 : 0;
+            a.g_pole_sum = 0;
+            a.hpole_in = hpole[cur_pole];
+            a.hpole_out = hpole[1 - cur_pole];
+            const StepFn fn = step2_fn(rebase, gmode);
+            if (step_timer.enabled) {
+                auto& ev = step_timer.next(stream);
+                fn(grid2, blk, stream, ev.first, ev.second, a);
+            } else {
+                fn(grid2, blk, stream, nullptr, nullptr, a);
+            }
+            SF_HIP(hipGetLastError());
+            a.rows_per_block = rows_per_block;
+            cur = 1 - cur;
+            if (surfactant) cur_g = 1 - cur_g;
+            if (rebase) {
+                cur_hi = 1 - cur_hi;
+                since_rebase = 0;
+            } else {
+                since_rebase += 2;
+            }
+            if (h_avg) {
+                cur_pole = 1 - cur_pole;
+                poles_virtual = true;
+            }
+            current_time += dt;
+            current_time += dt;
+            steps_taken += 2;
+            ++s;
+            continue;
+        }
         schedule_fronts(dt, a);
         last_boxes.assign(a.fr, a.fr + a.nfronts);
         const bool rebase = (since_rebase + 1 >= rebase_every);
@@ -1153,6 +1224,22 @@ extern "C" int sf_spectral_kernel_timing(sf_drainage* ctx, int32_t enable, doubl
         SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
         Drainage* d = D(ctx);
         timer_query(d, d->spec_timer, enable, avg_ms, launches);
+    });
+}
+
+extern "C" int sf_drainage_set_steps_per_launch(sf_drainage* ctx, int32_t steps) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
+        SF_REQUIRE(steps == 1 || steps == 2, SF_ERR_INVALID, "steps per launch must be 1 or 2, got %d", steps);
+        D(ctx)->steps_per_launch = steps;
+    });
+}
+
+extern "C" int sf_drainage_get_steps_per_launch(sf_drainage* ctx, int32_t* steps) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && steps != nullptr, SF_ERR_INVALID, "bad arguments");
+        const Drainage* d = D(ctx);
+        *steps = (d->steps_per_launch == 2 && d->two_step_ok()) ? 2 : 1;
     });
 }
 

@@ -49,6 +49,10 @@ struct Drainage {
     // launch geometry
     bool vec4 = false;   // float4 kernel (n_phi % 4 == 0)
     int wx = 1, block = 64, ntiles = 1, rows_per_block = 16;
+    // Two steps per launch (drainage_band2_kernel) where two_step_ok() holds:
+    // steps_per_launch 2 asks for it, 1 keeps one step per launch.
+    int steps_per_launch = 1;
+    int rows_per_block2 = 32;
 
     // device state (see drainage_kernels.hpp for the layout)
     float* hi[2] = {nullptr, nullptr};
@@ -129,6 +133,8 @@ struct Drainage {
     void materialize();
     double resolve_dt(double dt);
     void step(int n_steps, double dt, bool flush = true);
+    // Whether drainage_band2_kernel serves these flags and this grid.
+    bool two_step_ok() const;
     StatsPart compute_stats();
     void ensure_gather(size_t count);
 };

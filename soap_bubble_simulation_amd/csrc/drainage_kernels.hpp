@@ -564,37 +564,9 @@ __global__ __launch_bounds__(kBlockMax) void drainage_vec4_kernel(const StepArgs
 
         float lo_new[4], g_new[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            // Rupture mask on the input state (drainage.rs:331-333): frozen
-            // cells keep h and Γ; the update is computed for every cell and
-            // selected, which keeps the wave free of divergent branches.
-            const bool rupt = ff_less(ch[v], cl[v], a.hc0, a.hc1, a.hc2);
-            const float dtp = (sh[v] - ch[v]) + (sl[v] - cl[v]);   // toward theta + 1
-            const float dtm = (ch[v] - nh[v]) + (cl[v] - nl[v]);
-            const float dpp = (eh[v] - ch[v]) + (el[v] - cl[v]);
-            const float dpm = (ch[v] - wh[v]) + (cl[v] - wl[v]);
-            const float lap = (dtp - dtm) * G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
-            const float H = ch[v] + cl[v];
-            float rate_dt = fmaf(dtD, lap, -cS * (H * H * H));
-            if (MARANGONI) {
-                const float gth = (sg[v] - ng[v]) * a.inv_2dth * a.inv_r;
-                const float gph = (eg[v] - wg[v]) * a.inv_2dph * a.inv_r * rc.inv_s;
-                const float mag = sqrtf(gth * gth + gph * gph);
-                rate_dt += a.dt * a.mgr * mag * (H * H);
-            }
-            const float ln = fmaxf(cl[v] + rate_dt, -ch[v]);          // h >= 0 (drainage.rs:378)
-            lo_new[v] = rupt ? cl[v] : ln;
-            g_new[v] = cg[v];
-            if (GWRITE) {
-                const float gtp = sg[v] - cg[v], gtm = cg[v] - ng[v];
-                const float gpp = eg[v] - cg[v], gpm = cg[v] - wg[v];
-                const float glap = (gtp - gtm) * G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
-                float grate = a.ds_r2 * glap;
-                if (MARANGONI) grate += 0.001f * lap * cg[v];
-                const float gn = fminf(fmaxf(fmaf(a.dt, grate, cg[v]), 0.f), 1.f);
-                g_new[v] = rupt ? cg[v] : gn;
-            }
-        }
+        for (int v = 0; v < 4; ++v)
+            cell_step<GMODE, MARANGONI>(a, rc, G0, G1, G2, dtD, cS, ch[v], cl[v], nh[v], nl[v], sh[v], sl[v], wh[v],
+                                        wl[v], eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo_new[v], g_new[v]);
 
         // Keep the ring rows' new values (before the fronts act, as
         // update_poles runs before update_marginal_regeneration) for the
@@ -701,6 +673,48 @@ __global__ __launch_bounds__(kBlockMax) void drainage_vec4_kernel(const StepArgs
 
     if (DIAG) diag_epilogue(a, mass, lam, lane, wave, nwaves, red_d);
 }
+// One cell's step from its centre (c), north (n, theta - 1), south (s), west
+// (w, phi - 1) and east (e) values, h as float-float pairs (h, l) and Γ (g):
+// the low word's update lo_new and Γ's (drainage.rs:331-378,
+// drainage.wgsl:150-201).  Rupture mask on the input state
+// (drainage.rs:331-333): frozen cells keep h and Γ; the update is computed
+// for every cell and selected, which keeps the wave free of divergent
+// branches.  Shared by the one- and two-step band kernels so both do the same
+// arithmetic.
+template <int GMODE, bool MARANGONI>
+__device__ __forceinline__ void cell_step(const StepArgs& a, const RowF& rc, float G0, float G1, float G2, float dtD,
+                                          float cS, float ch, float cl, float nh, float nl, float sh, float sl, float wh,
+                                          float wl, float eh, float el, float cg, float ng, float sg, float wg, float eg,
+                                          float& lo_new, float& g_new) {
+    constexpr bool GWRITE = GMODE == 2;
+    const bool rupt = ff_less(ch, cl, a.hc0, a.hc1, a.hc2);
+    const float dtp = (sh - ch) + (sl - cl);   // toward theta + 1
+    const float dtm = (ch - nh) + (cl - nl);
+    const float dpp = (eh - ch) + (el - cl);
+    const float dpm = (ch - wh) + (cl - wl);
+    const float lap = (dtp - dtm) * G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
+    const float H = ch + cl;
+    float rate_dt = fmaf(dtD, lap, -cS * (H * H * H));
+    if (MARANGONI) {
+        const float gth = (sg - ng) * a.inv_2dth * a.inv_r;
+        const float gph = (eg - wg) * a.inv_2dph * a.inv_r * rc.inv_s;
+        const float mag = sqrtf(gth * gth + gph * gph);
+        rate_dt += a.dt * a.mgr * mag * (H * H);
+    }
+    const float ln = fmaxf(cl + rate_dt, -ch);          // h >= 0 (drainage.rs:378)
+    lo_new = rupt ? cl : ln;
+    g_new = cg;
+    if (GWRITE) {
+        const float gtp = sg - cg, gtm = cg - ng;
+        const float gpp = eg - cg, gpm = cg - wg;
+        const float glap = (gtp - gtm) * G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
+        float grate = a.ds_r2 * glap;
+        if (MARANGONI) grate += 0.001f * lap * cg;
+        const float gn = fminf(fmaxf(fmaf(a.dt, grate, cg), 0.f), 1.f);
+        g_new = rupt ? cg : gn;
+    }
+}
+
 // Band-streaming variant of drainage_vec4_kernel: the same row update on a
 // band of a.rows_per_block rows (up to kBandMax) streamed through a window of
 // five row buffers, so tall bands cost no extra registers and re-read only
@@ -815,37 +829,9 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band_kernel(const StepArgs
 
         float lo_new[4], g_new[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            // Rupture mask on the input state (drainage.rs:331-333): frozen
-            // cells keep h and Γ; the update is computed for every cell and
-            // selected, which keeps the wave free of divergent branches.
-            const bool rupt = ff_less(ch[v], cl[v], a.hc0, a.hc1, a.hc2);
-            const float dtp = (sh[v] - ch[v]) + (sl[v] - cl[v]);   // toward theta + 1
-            const float dtm = (ch[v] - nh[v]) + (cl[v] - nl[v]);
-            const float dpp = (eh[v] - ch[v]) + (el[v] - cl[v]);
-            const float dpm = (ch[v] - wh[v]) + (cl[v] - wl[v]);
-            const float lap = (dtp - dtm) * G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
-            const float H = ch[v] + cl[v];
-            float rate_dt = fmaf(dtD, lap, -cS * (H * H * H));
-            if (MARANGONI) {
-                const float gth = (sg[v] - ng[v]) * a.inv_2dth * a.inv_r;
-                const float gph = (eg[v] - wg[v]) * a.inv_2dph * a.inv_r * rc.inv_s;
-                const float mag = sqrtf(gth * gth + gph * gph);
-                rate_dt += a.dt * a.mgr * mag * (H * H);
-            }
-            const float ln = fmaxf(cl[v] + rate_dt, -ch[v]);          // h >= 0 (drainage.rs:378)
-            lo_new[v] = rupt ? cl[v] : ln;
-            g_new[v] = cg[v];
-            if (GWRITE) {
-                const float gtp = sg[v] - cg[v], gtm = cg[v] - ng[v];
-                const float gpp = eg[v] - cg[v], gpm = cg[v] - wg[v];
-                const float glap = (gtp - gtm) * G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
-                float grate = a.ds_r2 * glap;
-                if (MARANGONI) grate += 0.001f * lap * cg[v];
-                const float gn = fminf(fmaxf(fmaf(a.dt, grate, cg[v]), 0.f), 1.f);
-                g_new[v] = rupt ? cg[v] : gn;
-            }
-        }
+        for (int v = 0; v < 4; ++v)
+            cell_step<GMODE, MARANGONI>(a, rc, G0, G1, G2, dtD, cS, ch[v], cl[v], nh[v], nl[v], sh[v], sl[v], wh[v],
+                                        wl[v], eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo_new[v], g_new[v]);
 
         // Keep the ring rows' new values (before the fronts act, as
         // update_poles runs before update_marginal_regeneration) for the
@@ -965,5 +951,380 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band_kernel(const StepArgs
     }
 
     if (DIAG) diag_epilogue(a, mass, lam, lane, wave, nwaves, red_d);
+}
+
+// ---- two steps per launch (temporal blocking) -------------------------------
+//
+// drainage_band2_kernel advances the state by two steps in one pass over the
+// band: each lane streams the step-s rows of its four columns through a
+// register window (with the west and east neighbour columns and one column
+// beyond, for the wave's edge cells), forms the step-(s+1) rows and keeps them
+// in a three-row ring in the wave's LDS, and forms the step-(s+2) rows from
+// that ring.  A band thus reads h, its low word and Γ once and writes them once
+// per two steps.  Every cell update is cell_step, the arithmetic of the
+// one-step band kernel, and the pole means between the two steps are formed
+// from the same partial sums in the same order — a block whose band touches a
+// pole recomputes that pole's ring row of step s+1 over every column tile —
+// so a launch equals two launches of drainage_band_kernel bit for bit.
+// Scope: no fronts, no Marangoni term, no fused diagnostics, Γ poles fixed,
+// n_phi a multiple of 256 (whole waves), a rebase only after the second step.
+constexpr int kRing2W = 264;     // LDS row of a wave: edge at 3, columns 4 .. 259, edge at 260
+constexpr int kMaxTiles2 = 64;   // column tiles (n_phi <= 65536 with four-wave blocks)
+
+struct Row5 {
+    float4 h, l, g;              // the lane's four columns cc .. cc + 3
+    float hw, lw, gw;            // column cc - 1
+    float he, le, ge;            // column cc + 4
+    float hf, lf, gf;            // lane 0: cc - 2; other lanes: cc + 5 (beyond the edge cell)
+};
+
+template <int GMODE>
+__device__ inline void load_row5(const StepArgs& a, int r, int cc, int jw, int je, int jf, Row5& R) {
+    const size_t base = static_cast<size_t>(max(0, min(r, a.nt - 1))) * a.np;
+    R.h = *reinterpret_cast<const float4*>(a.hi_in + base + cc);
+    R.l = *reinterpret_cast<const float4*>(a.lo_in + base + cc);
+    R.hw = a.hi_in[base + jw];
+    R.lw = a.lo_in[base + jw];
+    R.he = a.hi_in[base + je];
+    R.le = a.lo_in[base + je];
+    R.hf = a.hi_in[base + jf];
+    R.lf = a.lo_in[base + jf];
+    if (GMODE >= 1) {
+        R.g = *reinterpret_cast<const float4*>(a.g_in + base + cc);
+        R.gw = a.g_in[base + jw];
+        R.ge = a.g_in[base + je];
+        R.gf = a.g_in[base + jf];
+    } else {
+        R.g = make_float4(0.f, 0.f, 0.f, 0.f);
+        R.gw = R.ge = R.gf = 0.f;
+    }
+}
+
+// m ? a : b with m all ones or all zeros, as a bit blend of register values.
+__device__ __forceinline__ float blend(unsigned m, float a, float b) {
+    return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
+// Ring-1 (or ring nt-2) partial sums of step s+1 for every column tile, as
+// drainage_band_kernel forms them (same lanes, waves and order), and the pole
+// mean they give.  ring = 1 or nt - 2; Pin: the pole of step s (applied when
+// a.h_pole_mean).  All threads of the block call it; returns the mean.
+template <int GMODE>
+__device__ float2 pole_after_first_step(const StepArgs& a, int ring, float2 Pin, float2* part, float2* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int np = a.np;
+    const RowF rc = a.rows[ring];
+    const float G0 = a.inv_dth2, G1 = rc.cot * a.inv_2dth, G2 = rc.inv_s2 * a.inv_dph2;
+    const float dtD = a.dt * a.d_r2, cS = a.dt * a.coef * rc.sin_t;
+    const bool top = ring == 1;
+    for (int t = 0; t < a.ntiles; ++t) {
+        const int c0 = (t * nwaves + wave) * 256 + 4 * lane;
+        const bool valid = c0 < np;
+        const int cc = valid ? c0 : np - 4;
+        const int jw = cc == 0 ? np - 1 : cc - 1;
+        const int je = cc + 4 == np ? 0 : cc + 4;
+        Row4 N, C, S;
+        load_row4<GMODE>(a, ring - 1, cc, jw, je, N);
+        load_row4<GMODE>(a, ring, cc, jw, je, C);
+        load_row4<GMODE>(a, ring + 1, cc, jw, je, S);
+        if (top)
+            pole_row(N, Pin, 0.f, a.h_pole_mean, false);
+        else
+            pole_row(S, Pin, 0.f, a.h_pole_mean, false);
+        const float ch[4] = {C.h.x, C.h.y, C.h.z, C.h.w}, cl[4] = {C.l.x, C.l.y, C.l.z, C.l.w};
+        const float nh[4] = {N.h.x, N.h.y, N.h.z, N.h.w}, nl[4] = {N.l.x, N.l.y, N.l.z, N.l.w};
+        const float sh[4] = {S.h.x, S.h.y, S.h.z, S.h.w}, sl[4] = {S.l.x, S.l.y, S.l.z, S.l.w};
+        const float wh[4] = {C.hw, C.h.x, C.h.y, C.h.z}, wl[4] = {C.lw, C.l.x, C.l.y, C.l.z};
+        const float eh[4] = {C.h.y, C.h.z, C.h.w, C.he}, el[4] = {C.l.y, C.l.z, C.l.w, C.le};
+        float cg[4] = {0.f, 0.f, 0.f, 0.f}, ng[4] = {0.f, 0.f, 0.f, 0.f}, sg[4] = {0.f, 0.f, 0.f, 0.f};
+        float wg[4] = {0.f, 0.f, 0.f, 0.f}, eg[4] = {0.f, 0.f, 0.f, 0.f};
+        if (GMODE >= 1) {
+            cg[0] = C.g.x; cg[1] = C.g.y; cg[2] = C.g.z; cg[3] = C.g.w;
+            ng[0] = N.g.x; ng[1] = N.g.y; ng[2] = N.g.z; ng[3] = N.g.w;
+            sg[0] = S.g.x; sg[1] = S.g.y; sg[2] = S.g.z; sg[3] = S.g.w;
+            wg[0] = C.gw; wg[1] = C.g.x; wg[2] = C.g.y; wg[3] = C.g.z;
+            eg[0] = C.g.y; eg[1] = C.g.z; eg[2] = C.g.w; eg[3] = C.ge;
+        }
+        float2 sum = make_float2(0.f, 0.f);
+        if (valid) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float lo_new, g_new;
+                cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, ch[v], cl[v], nh[v], nl[v], sh[v], sl[v], wh[v],
+                                        wl[v], eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo_new, g_new);
+                float s_, e_;
+                two_sum(ch[v], lo_new, s_, e_);
+                sum = ff_add(sum, make_float2(s_, e_));
+            }
+        }
+        sum = wave_ff_sum(sum);
+        if (lane == 0) red[wave] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float2 tt = red[0];
+            for (int w = 1; w < nwaves; ++w) tt = ff_add(tt, red[w]);
+            part[t] = tt;
+        }
+        __syncthreads();
+    }
+    return pole_mean(part, a.ntiles, a.np_f);
+}
+
+template <bool REBASE, int GMODE>
+__global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArgs a) {
+    constexpr bool GREAD = GMODE >= 1;
+    constexpr bool GWRITE = GMODE == 2;
+    const int np = a.np, nt = a.nt;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    // The blocks of the first and last band (the pole rings, the longest
+    // blocks) go first; the rest keep the one-step kernel's XCD-contiguous
+    // order.
+    const unsigned nblk = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+    unsigned bx, by;
+    if (b < 2 * gridDim.x && gridDim.y > 1) {
+        bx = b % gridDim.x;
+        by = b < gridDim.x ? 0 : gridDim.y - 1;
+    } else {
+        const unsigned rest = gridDim.y > 1 ? b - 2 * gridDim.x : b;      // index among the inner bands
+        const unsigned ninner = gridDim.y > 1 ? nblk - 2 * gridDim.x : nblk;
+        const unsigned per_xcd = (ninner + kXcds - 1) / kXcds;
+        unsigned v = (rest % kXcds) * per_xcd + rest / kXcds;
+        if (ninner % kXcds != 0) v = rest;
+        bx = v % gridDim.x;
+        by = v / gridDim.x + (gridDim.y > 1 ? 1 : 0);
+    }
+    const int c0 = (static_cast<int>(bx) * nwaves + wave) * 256 + 4 * lane;
+    const bool valid = c0 < np;                   // whole waves: n_phi % 256 == 0
+    const int cc = valid ? c0 : np - 4;
+    const int jw = cc == 0 ? np - 1 : cc - 1;
+    const int je = cc + 4 == np ? 0 : cc + 4;
+    const int jf = lane == 0 ? (cc < 2 ? cc - 2 + np : cc - 2) : (cc + 5 >= np ? cc + 5 - np : cc + 5);
+    const int RB = a.rows_per_block;
+    const int r0 = 1 + static_cast<int>(by) * RB;
+    const int r1 = min(r0 + RB, nt - 1);          // rows [r0, r1) of step s+2 are written
+    const bool touches_top = r0 == 1;
+    const bool touches_bot = r1 == nt - 1;
+
+    __shared__ float2 red_ff[4][kBlockMax / 64];
+    __shared__ float2 s_part[kMaxTiles2];
+    __shared__ RowF s_rowf[kBandMax + 2];
+    __shared__ __align__(16) float ring[kBlockMax / 64][3][3][kRing2W];   // [wave][slot][h, l, g][column]
+
+    // Row coefficients of rows r0 - 1 .. r1 (step s+1 rows r0 - 1 .. r1 - 1 are
+    // formed with those of their own rows; step s+2 rows r0 .. r1 - 1).
+    for (int t = threadIdx.x; t < RB + 2; t += blockDim.x) {
+        const int ri = min(max(r0 - 1 + t, 0), nt - 1);
+        s_rowf[t] = a.rows[ri];
+    }
+    // Poles of step s (P0) and of step s+1 (P1).
+    float2 p0top = make_float2(0.f, 0.f), p0bot = make_float2(0.f, 0.f);
+    if (a.h_pole_mean) {
+        if (touches_top || r0 <= 2) p0top = pole_mean(a.hpole_in, a.ntiles, a.np_f);
+        if (touches_bot || r1 >= nt - 2) p0bot = pole_mean(a.hpole_in + a.ntiles, a.ntiles, a.np_f);
+    }
+    float2 p1top = make_float2(0.f, 0.f), p1bot = make_float2(0.f, 0.f);
+    const bool need_top = a.h_pole_sum && touches_top;           // step s+1 row 0 is used
+    const bool need_bot = a.h_pole_sum && touches_bot;           // step s+1 row nt - 1 is used
+    if (need_top) p1top = pole_after_first_step<GMODE>(a, 1, p0top, s_part, red_ff[0]);
+    if (need_bot) p1bot = pole_after_first_step<GMODE>(a, nt - 2, p0bot, s_part, red_ff[1]);
+    __syncthreads();
+
+    float* rw = &ring[wave][0][0][0];
+    auto slot = [&](int k, int arr) { return rw + (((k % 3) + 3) % 3) * 3 * kRing2W + arr * kRing2W; };
+    const int lc = 4 + 4 * lane;                  // the lane's first column in an LDS row
+
+    // Step s+1 row k from step-s rows (N, C, S) into the ring.
+    auto first_step = [&](int k, const Row5& N, const Row5& C, const Row5& S) {
+        // Pole neighbours are substituted as values (no copy of a row buffer).
+        const bool pn = k == 1 && a.h_pole_mean, ps = k == nt - 2 && a.h_pole_mean;
+        const float pnh = p0top.x, pnl = p0top.y, psh = p0bot.x, psl = p0bot.y;
+        const RowF rc = s_rowf[k - (r0 - 1)];
+        const float G0 = a.inv_dth2, G1 = rc.cot * a.inv_2dth, G2 = rc.inv_s2 * a.inv_dph2;
+        const float dtD = a.dt * a.d_r2, cS = a.dt * a.coef * rc.sin_t;
+        const float ch[4] = {C.h.x, C.h.y, C.h.z, C.h.w}, cl[4] = {C.l.x, C.l.y, C.l.z, C.l.w};
+        const float nh[4] = {pn ? pnh : N.h.x, pn ? pnh : N.h.y, pn ? pnh : N.h.z, pn ? pnh : N.h.w};
+        const float nl[4] = {pn ? pnl : N.l.x, pn ? pnl : N.l.y, pn ? pnl : N.l.z, pn ? pnl : N.l.w};
+        const float sh[4] = {ps ? psh : S.h.x, ps ? psh : S.h.y, ps ? psh : S.h.z, ps ? psh : S.h.w};
+        const float sl[4] = {ps ? psl : S.l.x, ps ? psl : S.l.y, ps ? psl : S.l.z, ps ? psl : S.l.w};
+        const float wh[4] = {C.hw, C.h.x, C.h.y, C.h.z}, wl[4] = {C.lw, C.l.x, C.l.y, C.l.z};
+        const float eh[4] = {C.h.y, C.h.z, C.h.w, C.he}, el[4] = {C.l.y, C.l.z, C.l.w, C.le};
+        const float cg[4] = {C.g.x, C.g.y, C.g.z, C.g.w}, ng[4] = {N.g.x, N.g.y, N.g.z, N.g.w};
+        const float sg[4] = {S.g.x, S.g.y, S.g.z, S.g.w};
+        const float wg[4] = {C.gw, C.g.x, C.g.y, C.g.z}, eg[4] = {C.g.y, C.g.z, C.g.w, C.ge};
+        float lo1[4], g1[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, ch[v], cl[v], nh[v], nl[v], sh[v], sl[v], wh[v], wl[v],
+                                    eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo1[v], g1[v]);
+        // the wave's edge cell: lane 0 its west column cc - 1, lane 63 its
+        // east column cc + 4 (the other lanes form the east one and drop it)
+        const unsigned wm = lane == 0 ? ~0u : 0u;
+        const float xch = blend(wm, C.hw, C.he), xcl = blend(wm, C.lw, C.le), xcg = blend(wm, C.gw, C.ge);
+        float xnh = blend(wm, N.hw, N.he), xnl = blend(wm, N.lw, N.le), xng = blend(wm, N.gw, N.ge);
+        float xsh = blend(wm, S.hw, S.he), xsl = blend(wm, S.lw, S.le), xsg = blend(wm, S.gw, S.ge);
+        xnh = pn ? pnh : xnh;
+        xnl = pn ? pnl : xnl;
+        xsh = ps ? psh : xsh;
+        xsl = ps ? psl : xsl;
+        const float xwh = blend(wm, C.hf, C.h.w), xwl = blend(wm, C.lf, C.l.w), xwg = blend(wm, C.gf, C.g.w);
+        const float xeh = blend(wm, C.h.x, C.hf), xel = blend(wm, C.l.x, C.lf), xeg = blend(wm, C.g.x, C.gf);
+        float xlo, xg;
+        cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, xch, xcl, xnh, xnl, xsh, xsl, xwh, xwl, xeh, xel, xcg, xng,
+                                xsg, xwg, xeg, xlo, xg);
+        // A pole row of step s+1 is not stepped: h and its low word are the
+        // pole mean P1 (or, with copied poles, the stored row) and Γ is
+        // copied through.  Its cells were formed above from clamped rows and
+        // are replaced here by value, so the row loop has no branches.
+        const bool pole = k == 0 || k == nt - 1;
+        const bool use_p = pole && a.h_pole_sum;
+        const float2 P = k == 0 ? p1top : p1bot;
+        float oh[4], ol[4], og[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            oh[v] = use_p ? P.x : ch[v];
+            ol[v] = use_p ? P.y : (pole ? cl[v] : lo1[v]);
+            og[v] = pole ? cg[v] : g1[v];
+        }
+        const float xoh = use_p ? P.x : xch;
+        const float xol = use_p ? P.y : (pole ? xcl : xlo);
+        const float xog = pole ? xcg : xg;
+        float* rh = slot(k, 0);
+        float* rl = slot(k, 1);
+        float* rg = slot(k, 2);
+        *reinterpret_cast<float4*>(rh + lc) = make_float4(oh[0], oh[1], oh[2], oh[3]);
+        *reinterpret_cast<float4*>(rl + lc) = make_float4(ol[0], ol[1], ol[2], ol[3]);
+        if (GREAD) *reinterpret_cast<float4*>(rg + lc) = make_float4(og[0], og[1], og[2], og[3]);
+        if (lane == 0) { rh[3] = xoh; rl[3] = xol; if (GREAD) rg[3] = xog; }
+        if (lane == 63) { rh[260] = xoh; rl[260] = xol; if (GREAD) rg[260] = xog; }
+    };
+
+    float top_h[4] = {0.f, 0.f, 0.f, 0.f}, top_l[4] = {0.f, 0.f, 0.f, 0.f};
+    float bot_h[4] = {0.f, 0.f, 0.f, 0.f}, bot_l[4] = {0.f, 0.f, 0.f, 0.f};
+
+    // Step s+2 row i from ring rows i - 1, i, i + 1.
+    auto second_step = [&](int i) {
+        const float* nh_ = slot(i - 1, 0); const float* nl_ = slot(i - 1, 1); const float* ng_ = slot(i - 1, 2);
+        const float* ch_ = slot(i, 0);     const float* cl_ = slot(i, 1);     const float* cg_ = slot(i, 2);
+        const float* sh_ = slot(i + 1, 0); const float* sl_ = slot(i + 1, 1); const float* sg_ = slot(i + 1, 2);
+        const float4 C4 = *reinterpret_cast<const float4*>(ch_ + lc), L4 = *reinterpret_cast<const float4*>(cl_ + lc);
+        const float4 N4 = *reinterpret_cast<const float4*>(nh_ + lc), NL4 = *reinterpret_cast<const float4*>(nl_ + lc);
+        const float4 S4 = *reinterpret_cast<const float4*>(sh_ + lc), SL4 = *reinterpret_cast<const float4*>(sl_ + lc);
+        const float ch[4] = {C4.x, C4.y, C4.z, C4.w}, cl[4] = {L4.x, L4.y, L4.z, L4.w};
+        const float nh[4] = {N4.x, N4.y, N4.z, N4.w}, nl[4] = {NL4.x, NL4.y, NL4.z, NL4.w};
+        const float sh[4] = {S4.x, S4.y, S4.z, S4.w}, sl[4] = {SL4.x, SL4.y, SL4.z, SL4.w};
+        const float wh[4] = {ch_[lc - 1], C4.x, C4.y, C4.z}, wl[4] = {cl_[lc - 1], L4.x, L4.y, L4.z};
+        const float eh[4] = {C4.y, C4.z, C4.w, ch_[lc + 4]}, el[4] = {L4.y, L4.z, L4.w, cl_[lc + 4]};
+        float cg[4] = {0.f, 0.f, 0.f, 0.f}, ng[4] = {0.f, 0.f, 0.f, 0.f}, sg[4] = {0.f, 0.f, 0.f, 0.f};
+        float wg[4] = {0.f, 0.f, 0.f, 0.f}, eg[4] = {0.f, 0.f, 0.f, 0.f};
+        if (GREAD) {
+            const float4 G4 = *reinterpret_cast<const float4*>(cg_ + lc);
+            const float4 NG4 = *reinterpret_cast<const float4*>(ng_ + lc);
+            const float4 SG4 = *reinterpret_cast<const float4*>(sg_ + lc);
+            cg[0] = G4.x; cg[1] = G4.y; cg[2] = G4.z; cg[3] = G4.w;
+            ng[0] = NG4.x; ng[1] = NG4.y; ng[2] = NG4.z; ng[3] = NG4.w;
+            sg[0] = SG4.x; sg[1] = SG4.y; sg[2] = SG4.z; sg[3] = SG4.w;
+            wg[0] = cg_[lc - 1]; wg[1] = G4.x; wg[2] = G4.y; wg[3] = G4.z;
+            eg[0] = G4.y; eg[1] = G4.z; eg[2] = G4.w; eg[3] = cg_[lc + 4];
+        }
+        const RowF rc = s_rowf[i - (r0 - 1)];
+        const float G0 = a.inv_dth2, G1 = rc.cot * a.inv_2dth, G2 = rc.inv_s2 * a.inv_dph2;
+        const float dtD = a.dt * a.d_r2, cS = a.dt * a.coef * rc.sin_t;
+        float lo2[4], g2[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, ch[v], cl[v], nh[v], nl[v], sh[v], sl[v], wh[v], wl[v],
+                                    eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo2[v], g2[v]);
+        if (i == 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { top_h[v] = ch[v]; top_l[v] = lo2[v]; }
+        }
+        if (i == nt - 2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { bot_h[v] = ch[v]; bot_l[v] = lo2[v]; }
+        }
+        if (valid) {
+            const size_t off = static_cast<size_t>(i) * np + cc;
+            if (REBASE) {
+                float4 hh, ll;
+                two_sum(ch[0], lo2[0], hh.x, ll.x);
+                two_sum(ch[1], lo2[1], hh.y, ll.y);
+                two_sum(ch[2], lo2[2], hh.z, ll.z);
+                two_sum(ch[3], lo2[3], hh.w, ll.w);
+                *reinterpret_cast<float4*>(a.hi_out + off) = hh;
+                *reinterpret_cast<float4*>(a.lo_out + off) = ll;
+            } else {
+                *reinterpret_cast<float4*>(a.lo_out + off) = make_float4(lo2[0], lo2[1], lo2[2], lo2[3]);
+            }
+            if (GWRITE) *reinterpret_cast<float4*>(a.g_out + off) = make_float4(g2[0], g2[1], g2[2], g2[3]);
+        }
+    };
+
+    // Step-s rows k - 1, k, k + 1 form step s+1 row k (k = r0 - 1 .. r1);
+    // after row k is in the ring, step s+2 row k - 1 follows.  Five buffers,
+    // two rows in flight, unrolled over the rotations (no buffer is copied
+    // while its load is outstanding).
+    Row5 B0, B1, B2, B3, B4;
+    const int ks = r0 - 1;
+    load_row5<GMODE>(a, ks - 1, cc, jw, je, jf, B0);
+    load_row5<GMODE>(a, ks, cc, jw, je, jf, B1);
+    load_row5<GMODE>(a, ks + 1, cc, jw, je, jf, B2);
+    load_row5<GMODE>(a, ks + 2, cc, jw, je, jf, B3);
+    load_row5<GMODE>(a, ks + 3, cc, jw, je, jf, B4);
+    auto row_k = [&](int k, const Row5& N, const Row5& C, const Row5& S) {
+        first_step(k, N, C, S);
+        __builtin_amdgcn_wave_barrier();
+        if (k - 1 >= r0) second_step(k - 1);
+        __builtin_amdgcn_wave_barrier();
+    };
+#define SF_BAND2_ROW(Nb, Cb, Sb)                                                 \
+    {                                                                            \
+        if (k > r1) break;                                                       \
+        row_k(k, Nb, Cb, Sb);                                                    \
+        if (k + 4 <= r1 + 1) load_row5<GMODE>(a, k + 4, cc, jw, je, jf, Nb);     \
+        ++k;                                                                     \
+    }
+    for (int k = ks;;) {
+        SF_BAND2_ROW(B0, B1, B2)
+        SF_BAND2_ROW(B1, B2, B3)
+        SF_BAND2_ROW(B2, B3, B4)
+        SF_BAND2_ROW(B3, B4, B0)
+        SF_BAND2_ROW(B4, B0, B1)
+    }
+#undef SF_BAND2_ROW
+
+    // Ring partial sums of step s+2 for the next launch's pole means.
+    if (touches_top || touches_bot) {
+        float2 sum_top = make_float2(0.f, 0.f), sum_bot = make_float2(0.f, 0.f);
+        if (valid) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float s_, e_;
+                two_sum(top_h[v], top_l[v], s_, e_);
+                sum_top = ff_add(sum_top, make_float2(s_, e_));
+                two_sum(bot_h[v], bot_l[v], s_, e_);
+                sum_bot = ff_add(sum_bot, make_float2(s_, e_));
+            }
+        }
+        sum_top = wave_ff_sum(sum_top);
+        sum_bot = wave_ff_sum(sum_bot);
+        __syncthreads();
+        if (lane == 0) {
+            red_ff[0][wave] = sum_top;
+            red_ff[1][wave] = sum_bot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && a.h_pole_sum) {
+            float2 t0 = red_ff[0][0], t1 = red_ff[1][0];
+            for (int w = 1; w < nwaves; ++w) {
+                t0 = ff_add(t0, red_ff[0][w]);
+                t1 = ff_add(t1, red_ff[1][w]);
+            }
+            if (touches_top) a.hpole_out[bx] = t0;
+            if (touches_bot) a.hpole_out[a.ntiles + bx] = t1;
+        }
+    }
 }
 }  // namespace sf

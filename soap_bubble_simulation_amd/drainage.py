@@ -164,6 +164,17 @@ class DrainageContext:
                                                      C.byref(n)))
         return ms.value, n.value
 
+    def set_steps_per_launch(self, steps: int) -> None:
+        """1, or 2 for two steps per stencil launch where the flags and grid
+        allow it (bit-identical results; see ``steps_per_launch``)."""
+        L.check(self._lib.sf_drainage_set_steps_per_launch(self._h, int(steps)))
+
+    def steps_per_launch(self) -> int:
+        """Steps per stencil launch that ``step`` uses for this context."""
+        k = C.c_int32()
+        L.check(self._lib.sf_drainage_get_steps_per_launch(self._h, C.byref(k)))
+        return k.value
+
 
 class ThicknessField:
     """Snapshot of the device field with the accessors of ThicknessField
