@@ -87,9 +87,9 @@ def parse():
                     help="override the SF_FLAG_* bits of the workload")
     ap.add_argument("--grid", default=None,
                     help="c2 / c2-drainage: n_theta x n_phi other than 2048x4096, e.g. 8192x16384")
-    ap.add_argument("--steps-per-launch", type=int, default=1, choices=[1, 2],
-                    help="c2-drainage: 2 = two steps per stencil launch (temporal blocking; "
-                         "needs flags without TFE, DIAG and Marangoni, e.g. --flags 2)")
+    ap.add_argument("--steps-per-launch", type=int, default=None, choices=[1, 2],
+                    help="c2-drainage: force one or two steps per stencil launch (default: the "
+                         "library's choice; two needs flags without TFE, DIAG and Marangoni, e.g. --flags 2)")
     return ap.parse_args()
 
 
@@ -633,9 +633,10 @@ def main():
     p.flags = flags
     ctx = DrainageContext(p, n_theta, n_phi, 500e-9)
     ctx.set_fields(h0, np.full_like(h0, 0.5))
-    ctx.set_steps_per_launch(args.steps_per_launch)
+    if args.steps_per_launch is not None:
+        ctx.set_steps_per_launch(args.steps_per_launch)
     spl = ctx.steps_per_launch()
-    if spl != args.steps_per_launch:
+    if args.steps_per_launch is not None and spl != args.steps_per_launch:
         print("note: these flags and this grid take one step per launch", file=sys.stderr)
     dt = ctx.stats().dt_cfl * p.cfl_safety
     cam = W.camera(p.bubble_radius)

@@ -201,9 +201,11 @@ int sf_drainage_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
  * launches).  Two steps per launch serve flags without SF_FLAG_TFE,
  * SF_FLAG_DIAG, SF_FLAG_MARANGONI and averaged Γ poles, on grids with
  * n_phi % 256 == 0 and rebase_interval != 1; other contexts keep one step
- * per launch.  sf_drainage_get_steps_per_launch returns what sf_drainage_step
- * will use.  No counterpart in the reference, whose step dispatches one
- * compute pass per step (gpu_drainage.rs:354-369). */
+ * per launch.  The default is 2 on grids of about 2 million cells and more
+ * (1024 x 2048 and up), 1 below.  sf_drainage_get_steps_per_launch returns
+ * what sf_drainage_step will use; sf_drainage_kernel_timing then averages
+ * over launches of two steps.  No counterpart in the reference, whose step
+ * dispatches one compute pass per step (gpu_drainage.rs:354-369). */
 int sf_drainage_set_steps_per_launch(sf_drainage* ctx, int32_t steps);
 int sf_drainage_get_steps_per_launch(sf_drainage* ctx, int32_t* steps);
 

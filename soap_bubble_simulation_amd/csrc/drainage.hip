@@ -274,6 +274,19 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     rows_per_block = vec4 ? (static_cast<size_t>(ntiles) * cdiv(nt - 2, 16) >= 512 ? 16 : 4) : (small ? 1 : 16);
     if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK")) rows_per_block = std::max(1, std::atoi(e));
     if (vec4) rows_per_block = rows_per_block <= 4 ? 4 : (rows_per_block <= 8 ? 8 : std::min(rows_per_block, kBandMax));
+    // Two-step bands: the tallest (fewest recomputed halo rows) that still
+    // gives 512 blocks, two per CU (measured at 1024 x 2048 .. 8192 x 16384:
+    // 4, 16, 64, 64 rows).  Two steps per launch are the default where such
+    // a band has at least four rows; on smaller grids the recomputed halo
+    // rows outweigh the saved traffic.
+    rows_per_block2 = 1;
+    for (int rb = kBandMax; rb >= 1; rb /= 2) {
+        if (static_cast<size_t>(ntiles) * cdiv(nt - 2, rb) >= 512) {
+            rows_per_block2 = rb;
+            break;
+        }
+    }
+    steps_per_launch = rows_per_block2 >= 4 ? 2 : 1;
     if (const char* e = std::getenv("SOAPFILM_ROWS_PER_BLOCK2"))
         rows_per_block2 = std::min(std::max(1, std::atoi(e)), kBandMax);
     for (int k = 0; k < 2; ++k) {

@@ -52,7 +52,7 @@ struct Drainage {
     // Two steps per launch (drainage_band2_kernel) where two_step_ok() holds:
     // steps_per_launch 2 asks for it, 1 keeps one step per launch.
     int steps_per_launch = 1;
-    int rows_per_block2 = 32;
+    int rows_per_block2 = 16;     // band height of the two-step kernel
 
     // device state (see drainage_kernels.hpp for the layout)
     float* hi[2] = {nullptr, nullptr};

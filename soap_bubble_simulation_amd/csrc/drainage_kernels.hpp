@@ -973,31 +973,66 @@ constexpr int kMaxTiles2 = 64;   // column tiles (n_phi <= 65536 with four-wave 
 
 struct Row5 {
     float4 h, l, g;              // the lane's four columns cc .. cc + 3
-    float hw, lw, gw;            // column cc - 1
-    float he, le, ge;            // column cc + 4
-    float hf, lf, gf;            // lane 0: cc - 2; other lanes: cc + 5 (beyond the edge cell)
+    float hx, lx, gx;            // lane 0: column cc - 1; other lanes: cc + 4
 };
 
+struct Far5 {                    // lane 0: column cc - 2; other lanes: cc + 5
+    float h, l, g;
+};
+
+// The far column of row r, which only the wave's edge cells use (as the
+// centre row of a first step): loaded one row ahead of its use, not with the
+// row window.
 template <int GMODE>
-__device__ inline void load_row5(const StepArgs& a, int r, int cc, int jw, int je, int jf, Row5& R) {
+__device__ inline void load_far5(const StepArgs& a, int r, int jf, Far5& F) {
     const size_t base = static_cast<size_t>(max(0, min(r, a.nt - 1))) * a.np;
-    R.h = *reinterpret_cast<const float4*>(a.hi_in + base + cc);
-    R.l = *reinterpret_cast<const float4*>(a.lo_in + base + cc);
-    R.hw = a.hi_in[base + jw];
-    R.lw = a.lo_in[base + jw];
-    R.he = a.hi_in[base + je];
-    R.le = a.lo_in[base + je];
-    R.hf = a.hi_in[base + jf];
-    R.lf = a.lo_in[base + jf];
+    F.h = a.hi_in[base + jf];
+    F.l = a.lo_in[base + jf];
+    F.g = GMODE >= 1 ? a.g_in[base + jf] : 0.f;
+}
+
+// Loads step-s row r (clamped into the grid).  With pm set, the pole rows
+// (0 for r <= 0, nt - 1 for r >= nt - 1) hold the pole means P0 of h and its
+// low word instead of the stored values, which is what every use of them
+// needs: they are the theta neighbours of rows 1 and nt - 2, and rows 0 and
+// nt - 1 themselves are replaced when stepped.
+template <int GMODE>
+__device__ inline void load_row5(const StepArgs& a, int r, int cc, int jx, bool pm, float2 ptop, float2 pbot,
+                                 Row5& R) {
+    const int rr = max(0, min(r, a.nt - 1));
+    const size_t base = static_cast<size_t>(rr) * a.np;
+    const bool pole = pm && (rr == 0 || rr == a.nt - 1);
+    if (pole) {
+        const float2 P = rr == 0 ? ptop : pbot;
+        R.h = make_float4(P.x, P.x, P.x, P.x);
+        R.l = make_float4(P.y, P.y, P.y, P.y);
+        R.hx = P.x;
+        R.lx = P.y;
+    } else {
+        R.h = *reinterpret_cast<const float4*>(a.hi_in + base + cc);
+        R.l = *reinterpret_cast<const float4*>(a.lo_in + base + cc);
+        R.hx = a.hi_in[base + jx];
+        R.lx = a.lo_in[base + jx];
+    }
     if (GMODE >= 1) {
         R.g = *reinterpret_cast<const float4*>(a.g_in + base + cc);
-        R.gw = a.g_in[base + jw];
-        R.ge = a.g_in[base + je];
-        R.gf = a.g_in[base + jf];
+        R.gx = a.g_in[base + jx];
     } else {
         R.g = make_float4(0.f, 0.f, 0.f, 0.f);
-        R.gw = R.ge = R.gf = 0.f;
+        R.gx = 0.f;
     }
+}
+
+// The west neighbour of the lane's first column: lane l - 1's last column,
+// or, in lane 0, x (the column before the wave's first).  DPP wave_shr:1.
+__device__ __forceinline__ float from_west(float last, float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(last), 0x138, 0xf, 0xf, false));
+}
+
+// The east neighbour of the lane's last column: lane l + 1's first column,
+// or, in lane 63, x (the column after the wave's last).  DPP wave_shl:1.
+__device__ __forceinline__ float from_east(float first, float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(first), 0x130, 0xf, 0xf, false));
 }
 
 // m ? a : b with m all ones or all zeros, as a bit blend of register values.
@@ -1070,8 +1105,11 @@ __device__ float2 pole_after_first_step(const StepArgs& a, int ring, float2 Pin,
     return pole_mean(part, a.ntiles, a.np_f);
 }
 
+// Three waves per SIMD (168 VGPRs with Γ transport): one step-s row in
+// flight per wave measured faster than three rows at two waves per SIMD.
 template <bool REBASE, int GMODE>
-__global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArgs a) {
+__global__ __launch_bounds__(kBlockMax) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void drainage_band2_kernel(const StepArgs a) {
     constexpr bool GREAD = GMODE >= 1;
     constexpr bool GWRITE = GMODE == 2;
     const int np = a.np, nt = a.nt;
@@ -1101,6 +1139,7 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
     const int cc = valid ? c0 : np - 4;
     const int jw = cc == 0 ? np - 1 : cc - 1;
     const int je = cc + 4 == np ? 0 : cc + 4;
+    const int jx = lane == 0 ? jw : je;
     const int jf = lane == 0 ? (cc < 2 ? cc - 2 + np : cc - 2) : (cc + 5 >= np ? cc + 5 - np : cc + 5);
     const int RB = a.rows_per_block;
     const int r0 = 1 + static_cast<int>(by) * RB;
@@ -1130,6 +1169,15 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
     const bool need_bot = a.h_pole_sum && touches_bot;           // step s+1 row nt - 1 is used
     if (need_top) p1top = pole_after_first_step<GMODE>(a, 1, p0top, s_part, red_ff[0]);
     if (need_bot) p1bot = pole_after_first_step<GMODE>(a, nt - 2, p0bot, s_part, red_ff[1]);
+    // Every lane holds the same pole means: keep them in scalar registers.
+    auto uniform = [](float2 v) {
+        return make_float2(__int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.x))),
+                           __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.y))));
+    };
+    p0top = uniform(p0top);
+    p0bot = uniform(p0bot);
+    p1top = uniform(p1top);
+    p1bot = uniform(p1bot);
     __syncthreads();
 
     float* rw = &ring[wave][0][0][0];
@@ -1137,23 +1185,21 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
     const int lc = 4 + 4 * lane;                  // the lane's first column in an LDS row
 
     // Step s+1 row k from step-s rows (N, C, S) into the ring.
-    auto first_step = [&](int k, const Row5& N, const Row5& C, const Row5& S) {
-        // Pole neighbours are substituted as values (no copy of a row buffer).
-        const bool pn = k == 1 && a.h_pole_mean, ps = k == nt - 2 && a.h_pole_mean;
-        const float pnh = p0top.x, pnl = p0top.y, psh = p0bot.x, psl = p0bot.y;
+    auto first_step = [&](int k, const Row5& N, const Row5& C, const Row5& S, const Far5& F) {
         const RowF rc = s_rowf[k - (r0 - 1)];
         const float G0 = a.inv_dth2, G1 = rc.cot * a.inv_2dth, G2 = rc.inv_s2 * a.inv_dph2;
         const float dtD = a.dt * a.d_r2, cS = a.dt * a.coef * rc.sin_t;
         const float ch[4] = {C.h.x, C.h.y, C.h.z, C.h.w}, cl[4] = {C.l.x, C.l.y, C.l.z, C.l.w};
-        const float nh[4] = {pn ? pnh : N.h.x, pn ? pnh : N.h.y, pn ? pnh : N.h.z, pn ? pnh : N.h.w};
-        const float nl[4] = {pn ? pnl : N.l.x, pn ? pnl : N.l.y, pn ? pnl : N.l.z, pn ? pnl : N.l.w};
-        const float sh[4] = {ps ? psh : S.h.x, ps ? psh : S.h.y, ps ? psh : S.h.z, ps ? psh : S.h.w};
-        const float sl[4] = {ps ? psl : S.l.x, ps ? psl : S.l.y, ps ? psl : S.l.z, ps ? psl : S.l.w};
-        const float wh[4] = {C.hw, C.h.x, C.h.y, C.h.z}, wl[4] = {C.lw, C.l.x, C.l.y, C.l.z};
-        const float eh[4] = {C.h.y, C.h.z, C.h.w, C.he}, el[4] = {C.l.y, C.l.z, C.l.w, C.le};
+        const float nh[4] = {N.h.x, N.h.y, N.h.z, N.h.w}, nl[4] = {N.l.x, N.l.y, N.l.z, N.l.w};
+        const float sh[4] = {S.h.x, S.h.y, S.h.z, S.h.w}, sl[4] = {S.l.x, S.l.y, S.l.z, S.l.w};
+        const float hw = from_west(C.h.w, C.hx), lw = from_west(C.l.w, C.lx);
+        const float he = from_east(C.h.x, C.hx), le = from_east(C.l.x, C.lx);
+        const float gw = GREAD ? from_west(C.g.w, C.gx) : 0.f, ge = GREAD ? from_east(C.g.x, C.gx) : 0.f;
+        const float wh[4] = {hw, C.h.x, C.h.y, C.h.z}, wl[4] = {lw, C.l.x, C.l.y, C.l.z};
+        const float eh[4] = {C.h.y, C.h.z, C.h.w, he}, el[4] = {C.l.y, C.l.z, C.l.w, le};
         const float cg[4] = {C.g.x, C.g.y, C.g.z, C.g.w}, ng[4] = {N.g.x, N.g.y, N.g.z, N.g.w};
         const float sg[4] = {S.g.x, S.g.y, S.g.z, S.g.w};
-        const float wg[4] = {C.gw, C.g.x, C.g.y, C.g.z}, eg[4] = {C.g.y, C.g.z, C.g.w, C.ge};
+        const float wg[4] = {gw, C.g.x, C.g.y, C.g.z}, eg[4] = {C.g.y, C.g.z, C.g.w, ge};
         float lo1[4], g1[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v)
@@ -1162,15 +1208,11 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
         // the wave's edge cell: lane 0 its west column cc - 1, lane 63 its
         // east column cc + 4 (the other lanes form the east one and drop it)
         const unsigned wm = lane == 0 ? ~0u : 0u;
-        const float xch = blend(wm, C.hw, C.he), xcl = blend(wm, C.lw, C.le), xcg = blend(wm, C.gw, C.ge);
-        float xnh = blend(wm, N.hw, N.he), xnl = blend(wm, N.lw, N.le), xng = blend(wm, N.gw, N.ge);
-        float xsh = blend(wm, S.hw, S.he), xsl = blend(wm, S.lw, S.le), xsg = blend(wm, S.gw, S.ge);
-        xnh = pn ? pnh : xnh;
-        xnl = pn ? pnl : xnl;
-        xsh = ps ? psh : xsh;
-        xsl = ps ? psl : xsl;
-        const float xwh = blend(wm, C.hf, C.h.w), xwl = blend(wm, C.lf, C.l.w), xwg = blend(wm, C.gf, C.g.w);
-        const float xeh = blend(wm, C.h.x, C.hf), xel = blend(wm, C.l.x, C.lf), xeg = blend(wm, C.g.x, C.gf);
+        const float xch = C.hx, xcl = C.lx, xcg = C.gx;
+        const float xnh = N.hx, xnl = N.lx, xng = N.gx;
+        const float xsh = S.hx, xsl = S.lx, xsg = S.gx;
+        const float xwh = blend(wm, F.h, C.h.w), xwl = blend(wm, F.l, C.l.w), xwg = blend(wm, F.g, C.g.w);
+        const float xeh = blend(wm, C.h.x, F.h), xel = blend(wm, C.l.x, F.l), xeg = blend(wm, C.g.x, F.g);
         float xlo, xg;
         cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, xch, xcl, xnh, xnl, xsh, xsl, xwh, xwl, xeh, xel, xcg, xng,
                                 xsg, xwg, xeg, xlo, xg);
@@ -1201,8 +1243,21 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
         if (lane == 63) { rh[260] = xoh; rl[260] = xol; if (GREAD) rg[260] = xog; }
     };
 
-    float top_h[4] = {0.f, 0.f, 0.f, 0.f}, top_l[4] = {0.f, 0.f, 0.f, 0.f};
-    float bot_h[4] = {0.f, 0.f, 0.f, 0.f}, bot_l[4] = {0.f, 0.f, 0.f, 0.f};
+    // The lane's float-float sums of its ring-row cells of step s+2 (rows 1
+    // and nt - 2), in the order of the one-step kernel.
+    float2 sum_top = make_float2(0.f, 0.f), sum_bot = make_float2(0.f, 0.f);
+    auto lane_ring_sum = [&](const float (&h)[4], const float (&l)[4]) {
+        float2 t = make_float2(0.f, 0.f);
+        if (valid) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float s_, e_;
+                two_sum(h[v], l[v], s_, e_);
+                t = ff_add(t, make_float2(s_, e_));
+            }
+        }
+        return t;
+    };
 
     // Step s+2 row i from ring rows i - 1, i, i + 1.
     auto second_step = [&](int i) {
@@ -1237,14 +1292,8 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
         for (int v = 0; v < 4; ++v)
             cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, ch[v], cl[v], nh[v], nl[v], sh[v], sl[v], wh[v], wl[v],
                                     eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo2[v], g2[v]);
-        if (i == 1) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) { top_h[v] = ch[v]; top_l[v] = lo2[v]; }
-        }
-        if (i == nt - 2) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) { bot_h[v] = ch[v]; bot_l[v] = lo2[v]; }
-        }
+        if (i == 1) sum_top = lane_ring_sum(ch, lo2);
+        if (i == nt - 2) sum_bot = lane_ring_sum(ch, lo2);
         if (valid) {
             const size_t off = static_cast<size_t>(i) * np + cc;
             if (REBASE) {
@@ -1263,51 +1312,47 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band2_kernel(const StepArg
     };
 
     // Step-s rows k - 1, k, k + 1 form step s+1 row k (k = r0 - 1 .. r1);
-    // after row k is in the ring, step s+2 row k - 1 follows.  Five buffers,
-    // two rows in flight, unrolled over the rotations (no buffer is copied
+    // after row k is in the ring, step s+2 row k - 1 follows.  Four buffers,
+    // one row in flight, unrolled over the rotations (no buffer is copied
     // while its load is outstanding).
-    Row5 B0, B1, B2, B3, B4;
+    // Step-s rows k - 1, k, k + 1 form step s+1 row k (k = r0 - 1 .. r1);
+    // after row k is in the ring, step s+2 row k - 1 follows.  Four row
+    // buffers (one row in flight) unrolled over their rotations, so no
+    // buffer is copied while its load is outstanding; the far columns of the
+    // edge cells come one row ahead in two more.
+    Row5 B0, B1, B2, B3;
+    Far5 F0, F1;
     const int ks = r0 - 1;
-    load_row5<GMODE>(a, ks - 1, cc, jw, je, jf, B0);
-    load_row5<GMODE>(a, ks, cc, jw, je, jf, B1);
-    load_row5<GMODE>(a, ks + 1, cc, jw, je, jf, B2);
-    load_row5<GMODE>(a, ks + 2, cc, jw, je, jf, B3);
-    load_row5<GMODE>(a, ks + 3, cc, jw, je, jf, B4);
-    auto row_k = [&](int k, const Row5& N, const Row5& C, const Row5& S) {
-        first_step(k, N, C, S);
+    const bool pm = a.h_pole_mean != 0;
+    load_row5<GMODE>(a, ks - 1, cc, jx, pm, p0top, p0bot, B0);
+    load_row5<GMODE>(a, ks, cc, jx, pm, p0top, p0bot, B1);
+    load_row5<GMODE>(a, ks + 1, cc, jx, pm, p0top, p0bot, B2);
+    load_row5<GMODE>(a, ks + 2, cc, jx, pm, p0top, p0bot, B3);
+    load_far5<GMODE>(a, ks, jf, F0);
+    auto row_k = [&](int k, const Row5& N, const Row5& C, const Row5& S, const Far5& F) {
+        first_step(k, N, C, S, F);
         __builtin_amdgcn_wave_barrier();
         if (k - 1 >= r0) second_step(k - 1);
         __builtin_amdgcn_wave_barrier();
     };
-#define SF_BAND2_ROW(Nb, Cb, Sb)                                                 \
+#define SF_BAND2_ROW(Nb, Cb, Sb, Fc, Fn)                                         \
     {                                                                            \
         if (k > r1) break;                                                       \
-        row_k(k, Nb, Cb, Sb);                                                    \
-        if (k + 4 <= r1 + 1) load_row5<GMODE>(a, k + 4, cc, jw, je, jf, Nb);     \
+        if (k + 1 <= r1) load_far5<GMODE>(a, k + 1, jf, Fn);                     \
+        row_k(k, Nb, Cb, Sb, Fc);                                                \
+        if (k + 3 <= r1 + 1) load_row5<GMODE>(a, k + 3, cc, jx, pm, p0top, p0bot, Nb); \
         ++k;                                                                     \
     }
     for (int k = ks;;) {
-        SF_BAND2_ROW(B0, B1, B2)
-        SF_BAND2_ROW(B1, B2, B3)
-        SF_BAND2_ROW(B2, B3, B4)
-        SF_BAND2_ROW(B3, B4, B0)
-        SF_BAND2_ROW(B4, B0, B1)
+        SF_BAND2_ROW(B0, B1, B2, F0, F1)
+        SF_BAND2_ROW(B1, B2, B3, F1, F0)
+        SF_BAND2_ROW(B2, B3, B0, F0, F1)
+        SF_BAND2_ROW(B3, B0, B1, F1, F0)
     }
 #undef SF_BAND2_ROW
 
     // Ring partial sums of step s+2 for the next launch's pole means.
     if (touches_top || touches_bot) {
-        float2 sum_top = make_float2(0.f, 0.f), sum_bot = make_float2(0.f, 0.f);
-        if (valid) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                float s_, e_;
-                two_sum(top_h[v], top_l[v], s_, e_);
-                sum_top = ff_add(sum_top, make_float2(s_, e_));
-                two_sum(bot_h[v], bot_l[v], s_, e_);
-                sum_bot = ff_add(sum_bot, make_float2(s_, e_));
-            }
-        }
         sum_top = wave_ff_sum(sum_top);
         sum_bot = wave_ff_sum(sum_bot);
         __syncthreads();

@@ -78,7 +78,9 @@ def test_two_step_bit_identical(monkeypatch, flags, nt, npf):
 
 @pytest.mark.parametrize("rows2", [1, 3, 16, 64])
 def test_two_step_band_heights(monkeypatch, rows2):
+    """Band heights from one row to the largest."""
     run_pair(monkeypatch, 130, 1024, L.FLAG_SURFACTANT, steps=(7, 10), rows2=rows2)
+    run_pair(monkeypatch, 67, 512, 0, steps=(3, 6), rows2=rows2)
 
 
 def test_two_step_headline_grid(monkeypatch):
