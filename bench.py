@@ -50,6 +50,8 @@ BYTES_PER_CELL_STEP = 16.0       # h and surfactant, FP32, read + write
 PMC_SUMMARY = {
     "c2": os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json"),
     "c2-drainage": os.path.join(ROOT, "profiles", "drainage_step_2048x4096_pmc.json"),
+    "c2-two-step": os.path.join(ROOT, "profiles", "drainage_step2_2048x4096_pmc.json"),
+    "c2-drainage-two-step": os.path.join(ROOT, "profiles", "drainage_step2_2048x4096_pmc.json"),
     "c3": os.path.join(ROOT, "profiles", "foam_step_4096_bubbles_pmc.json"),
     "frame": os.path.join(ROOT, "profiles", "caustics_render_2048x4096_pmc.json"),
 }
@@ -653,8 +655,9 @@ def main():
         else:
             L.check(lib.sf_drainage_step(ctx.handle, n, dt))
 
-    # warmup (also uploads the colour curves)
-    run(max(1, args.warmup))
+    # warmup (also uploads the colour curves); an even count keeps the step
+    # counter on launch boundaries when launches take two steps
+    run(max(spl, args.warmup + args.warmup % spl))
     L.check(lib.sf_drainage_synchronize(ctx.handle))
 
     def timed_pass(steps):
@@ -679,7 +682,7 @@ def main():
     n_spec = C.c_int64()
     L.check(lib.sf_spectral_kernel_timing(ctx.handle, 1 if spectral else 0, C.byref(ms_spec),
                                           C.byref(n_spec)))
-    timed_pass(max(args.steps, 500))
+    timed_pass(max(args.steps + args.steps % spl, 500))
     drain_ms, drain_n = ctx.kernel_timing(False)
     L.check(lib.sf_spectral_kernel_timing(ctx.handle, 0, C.byref(ms_spec), C.byref(n_spec)))
 
@@ -687,25 +690,47 @@ def main():
     # both sides.
     elapsed = timed_pass(args.steps)
 
+    # With two steps per launch the one-step kernel is timed as well, on the
+    # same state (the results of the two are bit-identical): temporal
+    # blocking moves the state once per two steps, so its fraction of the
+    # HBM peak on the per-step basis is reported beside the one-step one.
+    one_step = None
+    if spl == 2:
+        ctx.set_steps_per_launch(1)
+        ctx.kernel_timing(True)
+        L.check(lib.sf_drainage_step(ctx.handle, 500, dt))
+        L.check(lib.sf_drainage_synchronize(ctx.handle))
+        one_ms, one_n = ctx.kernel_timing(False)
+        ctx.set_steps_per_launch(2)
+        one_ach = BYTES_PER_CELL_STEP * n_theta * n_phi / (one_ms * 1e-3) / 1e9
+        one_step = {"kernel": "drainage_band_kernel" if n_theta >= 1024 else "drainage_step_kernel",
+                    "avg_launch_ms": one_ms, "launches_timed": one_n, "achieved": one_ach,
+                    "frac": one_ach / HBM_PEAK_GBS,
+                    "traffic": pmc_traffic(args.workload)[0] if args.workload in PMC_SUMMARY and not args.grid
+                    and args.flags is None else None}
+
     cells = n_theta * n_phi
     value = cells * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
-    # A launch reads and writes the state once whether it advances one step
-    # or two: 16 B/cell per launch either way.
-    achieved = BYTES_PER_CELL_STEP * cells / (drain_ms * 1e-3) / 1e9
+    # Algorithmic bytes per launch: 16 B per cell-step times the cell-steps
+    # a launch advances (two with temporal blocking, whose HBM traffic is
+    # about 24 B per cell per launch: see `traffic` and `one_step`).
+    achieved = spl * BYTES_PER_CELL_STEP * cells / (drain_ms * 1e-3) / 1e9
     kernel = ("drainage_band2_kernel" if spl == 2 else
               # 16-row float4 bands at 2048 x 4096, the scalar kernel with one
               # row per block at 128 x 256 (grids of up to 262144 cells)
               "drainage_band_kernel" if n_theta >= 1024 else "drainage_step_kernel")
-    traffic, tsrc = (pmc_traffic(args.workload)
-                     if args.workload in PMC_SUMMARY and not args.grid and args.flags is None and spl == 1
-                     else (None, None))
+    pmc_key = args.workload + ("-two-step" if spl == 2 else "")
+    traffic, tsrc = (pmc_traffic(pmc_key)
+                     if pmc_key in PMC_SUMMARY and not args.grid and args.flags is None else (None, None))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source(tsrc),
                 "kernel": kernel, "steps_per_launch": spl,
                 "avg_launch_ms": drain_ms,
                 "launches_timed": drain_n,
-                "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP * cells}
+                "algorithmic_bytes_per_launch": spl * BYTES_PER_CELL_STEP * cells}
+    if one_step is not None:
+        roofline["one_step"] = one_step
     stats = ctx.stats()
     h, g = ctx.fields()
     finite = bool(np.isfinite(h).all() and np.isfinite(g).all())

@@ -198,14 +198,20 @@ int sf_drainage_kernel_timing(sf_drainage* ctx, int32_t enable, double* avg_ms,
                               int64_t* launches);
 /* Steps advanced per stencil launch: 1, or 2 (temporal blocking: each band
  * is read and written once per two steps, bit-identical to two one-step
- * launches).  Two steps per launch serve flags without SF_FLAG_TFE,
- * SF_FLAG_DIAG, SF_FLAG_MARANGONI and averaged Γ poles, on grids with
- * n_phi % 256 == 0 and rebase_interval != 1; other contexts keep one step
- * per launch.  The default is 2 on grids of about 2 million cells and more
- * (1024 x 2048 and up), 1 below.  sf_drainage_get_steps_per_launch returns
- * what sf_drainage_step will use; sf_drainage_kernel_timing then averages
- * over launches of two steps.  No counterpart in the reference, whose step
- * dispatches one compute pass per step (gpu_drainage.rs:354-369). */
+ * launches, marginal-regeneration fronts and the fused FP64 diagnostics of
+ * both steps included).  Two steps per launch serve flags without
+ * SF_FLAG_MARANGONI and averaged Γ poles, on grids with n_phi % 256 == 0,
+ * fewer than 2^30 cells and rebase_interval != 1; other contexts keep one
+ * step per launch.  A launch of two steps never straddles a rebase (every
+ * rebase_interval steps), so an odd step count or a rebase boundary gives
+ * one one-step launch.  The default is 2 on grids of about 2 million cells
+ * and more (1024 x 2048 and up), 1 below.  sf_drainage_get_steps_per_launch
+ * returns what sf_drainage_step will use; sf_drainage_kernel_timing then
+ * averages over all its launches, of one or two steps.
+ * sf_drainage_step_spectral renders both states of a two-step launch: the
+ * launch also writes the state between its steps.  No counterpart in the
+ * reference, whose step dispatches one compute pass per step
+ * (gpu_drainage.rs:354-369). */
 int sf_drainage_set_steps_per_launch(sf_drainage* ctx, int32_t steps);
 int sf_drainage_get_steps_per_launch(sf_drainage* ctx, int32_t* steps);
 
@@ -274,6 +280,13 @@ int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_in
  * (sf_spectral_copy_rgb, sf_spectral_device_rgb). */
 int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, double dt, const sf_camera* cam,
                               double refractive_index, const float* cmf);
+/* sf_drainage_step_spectral keeping every step's frame: frame s (s = 0 ..
+ * n_steps - 1) is written to frames_dev + 3 * n_theta * n_phi * s (device
+ * memory the caller owns, planar R, G, B as sf_spectral_rgb), for consumers
+ * of every frame (video export, temporal analysis).  No counterpart in the
+ * reference, which renders once per displayed frame (pipeline.rs). */
+int sf_drainage_step_spectral_frames(sf_drainage* ctx, int32_t n_steps, double dt, const sf_camera* cam,
+                                     double refractive_index, const float* cmf, float* frames_dev);
 /* Blocks until all work enqueued on ctx has finished. */
 int sf_drainage_synchronize(sf_drainage* ctx);
 /* The thickness field rendered through the runtime LUT, one pixel per grid

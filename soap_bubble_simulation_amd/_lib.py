@@ -215,6 +215,8 @@ SIGNATURES = {
     "sf_spectral_get_operand": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_drainage_step_spectral": (C.c_int, [_vp, C.c_int32, C.c_double, C.POINTER(Camera),
                                             C.c_double, _fp]),
+    "sf_drainage_step_spectral_frames": (C.c_int, [_vp, C.c_int32, C.c_double, C.POINTER(Camera),
+                                                   C.c_double, _fp, _vp]),
     "sf_drainage_synchronize": (C.c_int, [_vp]),
     "sf_spectral_rgb": (C.c_int, [_vp, C.POINTER(Camera), C.c_double, _fp, _fp]),
     "sf_spectral_device_rgb": (C.c_int, [_vp, C.POINTER(_vp)]),

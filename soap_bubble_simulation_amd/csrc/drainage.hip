@@ -227,16 +227,19 @@ static StepFn step_fn(bool rebase, int gmode, bool marangoni, bool diag, bool ve
     return band[rebase][gmode][marangoni][diag];
 }
 
-// Two-step launches: [rebase][gmode 0 or 2].
-template <bool RB, int GM>
+// Two-step launches: [rebase][gmode 0 or 2][OPT].
+template <bool RB, int GM, int OPT>
 static void launch_band2(dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const StepArgs& a) {
-    hipExtLaunchKernelGGL((drainage_band2_kernel<RB, GM>), grid, block, 0, s, t0, t1, 0, a);
+    hipExtLaunchKernelGGL((drainage_band2_kernel<RB, GM, OPT>), grid, block, 0, s, t0, t1, 0, a);
 }
 
-static StepFn step2_fn(bool rebase, int gmode) {
-    static const StepFn tab[2][2] = {{&launch_band2<false, 0>, &launch_band2<false, 2>},
-                                     {&launch_band2<true, 0>, &launch_band2<true, 2>}};
-    return tab[rebase][gmode == 2 ? 1 : 0];
+static StepFn step2_fn(bool rebase, int gmode, int opt) {
+#define SF_O(RB, GM) {&launch_band2<RB, GM, 0>, &launch_band2<RB, GM, 1>, &launch_band2<RB, GM, 2>, \
+                      &launch_band2<RB, GM, 3>, &launch_band2<RB, GM, 4>, &launch_band2<RB, GM, 5>, \
+                      &launch_band2<RB, GM, 6>, &launch_band2<RB, GM, 7>}
+    static const StepFn tab[2][2][8] = {{SF_O(false, 0), SF_O(false, 2)}, {SF_O(true, 0), SF_O(true, 2)}};
+#undef SF_O
+    return tab[rebase][gmode == 2 ? 1 : 0][opt & 7];
 }
 
 }  // namespace sf
@@ -299,8 +302,9 @@ Drainage::Drainage(const sf_params& prm, int n_theta, int n_phi) : p(prm), nt(n_
     rows = dalloc<RowF>(nt);
     rowd = dalloc<RowD>(nt);
     nblocks_max = static_cast<size_t>(ntiles) * cdiv(nt, 1);
-    diag_part[0] = dalloc<double2>(nblocks_max);
-    diag_part[1] = dalloc<double2>(nblocks_max);
+    for (int k = 0; k < kDiagBufs; ++k) diag_part[k] = dalloc<double2>(nblocks_max);
+    hpole_mid[0] = dalloc<float2>(2 * static_cast<size_t>(ntiles));
+    hpole_mid[1] = dalloc<float2>(2 * static_cast<size_t>(ntiles));
     diag_hist = dalloc<double2>(kHistCap);
     stats_part = dalloc<StatsPart>(kStatsBlocks);
     stats_out = dalloc<StatsPart>(1);
@@ -312,7 +316,9 @@ Drainage::~Drainage() {
     for (int k = 0; k < 2; ++k) {
         dfree(hi[k]); dfree(lo[k]); dfree(g[k]); dfree(hpole[k]); dfree(gpole[k]);
     }
-    dfree(rows); dfree(rowd); dfree(diag_part[0]); dfree(diag_part[1]); dfree(diag_hist);
+    dfree(rows); dfree(rowd); dfree(diag_hist);
+    for (int k = 0; k < kDiagBufs; ++k) dfree(diag_part[k]);
+    dfree(lo_mid); dfree(hpole_mid[0]); dfree(hpole_mid[1]);
     dfree(stats_part); dfree(stats_out); dfree(gather_idx); dfree(gather_out);
     dfree(rgb); dfree(cmf_dev); dfree(spec_rows); dfree(spec_cols); dfree(spec_view); dfree(caustic);
     step_timer.destroy();
@@ -547,12 +553,46 @@ void Drainage::materialize_poles() {
 // Reduces the per-block FP64 partials of the last diagnosed step, which no
 // later launch has consumed yet.
 void Drainage::finalize_diag() {
-    if (pending_slot < 0) return;
-    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(256), 0, stream, diag_part[pending_part],
-                       pending_nblocks, hpole[cur_pole], ntiles, pending_pole_sum ? 1 : 0,
-                       pole_area_top, pole_area_bot, pole_mass_fixed, diag_hist, pending_slot);
-    SF_HIP(hipGetLastError());
-    pending_slot = -1;
+    for (int q = 0; q < npending; ++q) {
+        const PendingDiag& e = pending[q];
+        hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(256), 0, stream, diag_part[e.buf], e.nblocks,
+                           e.hpole, ntiles, e.pole_sum ? 1 : 0, pole_area_top, pole_area_bot, pole_mass_fixed,
+                           diag_hist, e.slot);
+        SF_HIP(hipGetLastError());
+    }
+    npending = 0;
+}
+
+// Partial buffers for a diagnosed launch of nsteps steps (ones no pending
+// step holds) and the pending steps its block 0 reduces.
+void Drainage::diag_args(StepArgs& a, int nsteps) {
+    int k = 0;
+    for (int b = 0; b < kDiagBufs && k < nsteps; ++b) {
+        bool held = false;
+        for (int q = 0; q < npending; ++q) held |= pending[q].buf == b;
+        if (!held) new_buf[k++] = b;
+    }
+    a.diag_part = diag_part[new_buf[0]];
+    a.diag_part2 = nsteps == 2 ? diag_part[new_buf[1]] : nullptr;
+    a.nprev = npending;
+    for (int q = 0; q < 2; ++q) {
+        a.prev[q] = DiagPending{nullptr, nullptr, 0, 0, -1, 0};
+        if (q < npending) {
+            const PendingDiag& e = pending[q];
+            a.prev[q] = DiagPending{diag_part[e.buf], e.hpole, e.nblocks, e.pole_sum ? 1 : 0, e.slot, 0};
+        }
+    }
+}
+
+// The launch's steps become the pending ones, in step order.
+void Drainage::diag_commit(int nsteps, int nblocks, bool pole_sum, const float2* hpole_first,
+                           const float2* hpole_last) {
+    npending = nsteps;
+    for (int q = 0; q < nsteps; ++q) {
+        pending[q] = PendingDiag{new_buf[q], static_cast<int>(hist_count % kHistCap), nblocks, pole_sum,
+                                 q == nsteps - 1 ? hpole_last : hpole_first};
+        ++hist_count;
+    }
 }
 
 void Drainage::materialize() {
@@ -572,13 +612,17 @@ double Drainage::resolve_dt(double dt) {
 bool Drainage::two_step_ok() const {
     const uint32_t f = p.flags;
     const bool g_avg = (f & SF_FLAG_SURFACTANT) && (f & SF_FLAG_POLES_AVG_GAMMA);
-    return vec4 && np % 256 == 0 && ntiles <= kMaxTiles2 && nt >= 4 && !(f & SF_FLAG_TFE) &&
-           !(f & SF_FLAG_DIAG) && !(f & SF_FLAG_MARANGONI) && !g_avg && p.rebase_interval != 1;
+    // byte offsets of the two-step kernel are 32-bit: below 2^30 cells
+    return vec4 && np % 256 == 0 && ntiles <= kMaxTiles2 && nt >= 4 && n < (size_t{1} << 30) &&
+           !(f & SF_FLAG_MARANGONI) && !g_avg && p.rebase_interval != 1;
 }
 
-void Drainage::step(int n_steps, double dt, bool flush) {
-    if (tables_dirty) upload_tables();
-    dt = resolve_dt(dt);
+bool Drainage::two_step_next(int remaining) const {
+    const int rebase_every = p.rebase_interval > 0 ? static_cast<int>(p.rebase_interval) : 1;
+    return remaining >= 2 && steps_per_launch == 2 && two_step_ok() && since_rebase + 1 < rebase_every;
+}
+
+void Drainage::launch_one(double dt) {
     const bool marangoni = (p.flags & SF_FLAG_MARANGONI) != 0;
     const bool surfactant = (p.flags & SF_FLAG_SURFACTANT) != 0;
     const int gmode = surfactant ? 2 : (marangoni ? 1 : 0);
@@ -587,118 +631,140 @@ void Drainage::step(int n_steps, double dt, bool flush) {
     const bool g_avg = surfactant && (p.flags & SF_FLAG_POLES_AVG_GAMMA);
     const int rebase_every = p.rebase_interval > 0 ? static_cast<int>(p.rebase_interval) : 1;
     const dim3 grid(ntiles, cdiv(nt - 2, rows_per_block));
-    const dim3 blk(block);
-    const float dt_f = static_cast<float>(dt);
-
     StepArgs a{};
-    fill_common_args(a, dt_f);
-    const bool two = steps_per_launch == 2 && two_step_ok();
+    fill_common_args(a, static_cast<float>(dt));
+    schedule_fronts(dt, a);
+    last_boxes.assign(a.fr, a.fr + a.nfronts);
+    const bool rebase = (since_rebase + 1 >= rebase_every);
+    a.hi_in = hi[cur_hi];
+    a.lo_in = lo[cur];
+    a.g_in = g[cur_g];
+    a.hi_out = hi[1 - cur_hi];
+    a.lo_out = lo[1 - cur];
+    a.g_out = g[1 - cur_g];
+    a.h_pole_mean = (poles_virtual && h_avg) ? 1 : 0;
+    a.g_pole_mean = (poles_virtual && g_avg) ? 1 : 0;
+    a.h_pole_sum = h_avg ? 1 : 0;
+    a.g_pole_sum = g_avg ? 1 : 0;
+    a.hpole_in = hpole[cur_pole];
+    a.gpole_in = gpole[cur_pole];
+    a.hpole_out = hpole[1 - cur_pole];
+    a.gpole_out = gpole[1 - cur_pole];
+    if (diag)
+        diag_args(a, 1);
+    else
+        finalize_diag();
+    const StepFn fn = step_fn(rebase, gmode, marangoni, diag, vec4, rows_per_block);
+    if (step_timer.enabled) {
+        auto& ev = step_timer.next(stream);
+        fn(grid, dim3(block), stream, ev.first, ev.second, a);
+    } else {
+        fn(grid, dim3(block), stream, nullptr, nullptr, a);
+    }
+    SF_HIP(hipGetLastError());
+    cur = 1 - cur;
+    if (surfactant) cur_g = 1 - cur_g;
+    if (rebase) {
+        cur_hi = 1 - cur_hi;
+        since_rebase = 0;
+    } else {
+        ++since_rebase;
+    }
+    if (h_avg || g_avg) {
+        cur_pole = 1 - cur_pole;
+        poles_virtual = true;
+    }
+    if (diag) diag_commit(1, static_cast<int>(grid.x * grid.y), h_avg, a.hpole_out, a.hpole_out);
+    current_time += dt;
+    ++steps_taken;
+}
+
+// Two steps in one launch of drainage_band2_kernel (two_step_next holds).
+void Drainage::launch_two(double dt, bool with_mid) {
+    const bool surfactant = (p.flags & SF_FLAG_SURFACTANT) != 0;
+    const int gmode = surfactant ? 2 : 0;
+    const bool diag = (p.flags & SF_FLAG_DIAG) != 0;
+    const bool h_avg = !(p.flags & SF_FLAG_POLES_COPY_H);
+    const int rebase_every = p.rebase_interval > 0 ? static_cast<int>(p.rebase_interval) : 1;
     const dim3 grid2(ntiles, cdiv(nt - 2, rows_per_block2));
-    for (int s = 0; s < n_steps; ++s) {
-        if (two && s + 1 < n_steps && since_rebase + 1 < rebase_every) {
-            // Steps s and s + 1 in one launch; a rebase can only follow the
-            // second, and neither step has fronts or diagnostics.
-            schedule_fronts(dt, a);
-            schedule_fronts(dt, a);
-            last_boxes.clear();
-            finalize_diag();
-            const bool rebase = since_rebase + 2 >= rebase_every;
-            a.rows_per_block = rows_per_block2;
-            a.hi_in = hi[cur_hi];
-            a.lo_in = lo[cur];
-            a.g_in = g[cur_g];
-            a.hi_out = hi[1 - cur_hi];
-            a.lo_out = lo[1 - cur];
-            a.g_out = g[1 - cur_g];
-            a.h_pole_mean = (poles_virtual && h_avg) ? 1 : 0;
-            a.g_pole_mean = 0;
-            a.h_pole_sum = h_avg ? 1 : 0;
-            a.g_pole_sum = 0;
-            a.hpole_in = hpole[cur_pole];
-            a.hpole_out = hpole[1 - cur_pole];
-            const StepFn fn = step2_fn(rebase, gmode);
-            if (step_timer.enabled) {
-                auto& ev = step_timer.next(stream);
-                fn(grid2, blk, stream, ev.first, ev.second, a);
-            } else {
-                fn(grid2, blk, stream, nullptr, nullptr, a);
-            }
-            SF_HIP(hipGetLastError());
-            a.rows_per_block = rows_per_block;
-            cur = 1 - cur;
-            if (surfactant) cur_g = 1 - cur_g;
-            if (rebase) {
-                cur_hi = 1 - cur_hi;
-                since_rebase = 0;
-            } else {
-                since_rebase += 2;
-            }
-            if (h_avg) {
-                cur_pole = 1 - cur_pole;
-                poles_virtual = true;
-            }
-            current_time += dt;
-            current_time += dt;
-            steps_taken += 2;
-            ++s;
-            continue;
-        }
-        schedule_fronts(dt, a);
-        last_boxes.assign(a.fr, a.fr + a.nfronts);
-        const bool rebase = (since_rebase + 1 >= rebase_every);
-        a.hi_in = hi[cur_hi];
-        a.lo_in = lo[cur];
-        a.g_in = g[cur_g];
-        a.hi_out = hi[1 - cur_hi];
-        a.lo_out = lo[1 - cur];
-        a.g_out = g[1 - cur_g];
-        a.h_pole_mean = (poles_virtual && h_avg) ? 1 : 0;
-        a.g_pole_mean = (poles_virtual && g_avg) ? 1 : 0;
-        a.h_pole_sum = h_avg ? 1 : 0;
-        a.g_pole_sum = g_avg ? 1 : 0;
-        a.hpole_in = hpole[cur_pole];
-        a.gpole_in = gpole[cur_pole];
-        a.hpole_out = hpole[1 - cur_pole];
-        a.gpole_out = gpole[1 - cur_pole];
-        if (diag) {
-            a.diag_part = diag_part[1 - pending_part];
-            a.diag_prev = diag_part[pending_part];
-            a.hpole_prev = a.hpole_in;
-            a.prev_slot = pending_slot;
-            a.prev_nblocks = pending_nblocks;
-            a.prev_pole_sum = pending_pole_sum ? 1 : 0;
+    StepArgs a{};
+    fill_common_args(a, static_cast<float>(dt));
+    a.rows_per_block = rows_per_block2;
+    // Front schedules of both steps, each at its own start time, as two
+    // one-step launches take them.
+    schedule_fronts(dt, a);
+    current_time += dt;
+    StepArgs b = a;
+    schedule_fronts(dt, b);
+    current_time += dt;
+    a.nfronts2 = b.nfronts;
+    for (int k = 0; k < b.nfronts; ++k) a.fr2[k] = b.fr[k];
+    if (b.nfronts > 0) {
+        a.tfe_g_hi = b.tfe_g_hi;
+        a.tfe_g_lo = b.tfe_g_lo;
+    }
+    last_boxes.assign(b.fr, b.fr + b.nfronts);
+    const bool rebase = since_rebase + 2 >= rebase_every;
+    a.hi_in = hi[cur_hi];
+    a.lo_in = lo[cur];
+    a.g_in = g[cur_g];
+    a.hi_out = hi[1 - cur_hi];
+    a.lo_out = lo[1 - cur];
+    a.g_out = g[1 - cur_g];
+    a.h_pole_mean = (poles_virtual && h_avg) ? 1 : 0;
+    a.g_pole_mean = 0;
+    a.h_pole_sum = h_avg ? 1 : 0;
+    a.g_pole_sum = 0;
+    a.hpole_in = hpole[cur_pole];
+    a.hpole_out = hpole[1 - cur_pole];
+    a.hpole_mid = hpole_mid[cur_mid];
+    if (with_mid) {
+        if (!lo_mid) lo_mid = dalloc<float>(n);
+        a.lo_mid = lo_mid;
+        mid_hi = hi[cur_hi];
+        mid_hpole = hpole_mid[cur_mid];
+    }
+    if (diag)
+        diag_args(a, 2);
+    else
+        finalize_diag();
+    const int opt = (diag ? kOptDiag : 0) | (a.nfronts + a.nfronts2 > 0 ? kOptFronts : 0) | (with_mid ? kOptMid : 0);
+    const StepFn fn = step2_fn(rebase, gmode, opt);
+    if (step_timer.enabled) {
+        auto& ev = step_timer.next(stream);
+        fn(grid2, dim3(block), stream, ev.first, ev.second, a);
+    } else {
+        fn(grid2, dim3(block), stream, nullptr, nullptr, a);
+    }
+    SF_HIP(hipGetLastError());
+    cur = 1 - cur;
+    if (surfactant) cur_g = 1 - cur_g;
+    if (rebase) {
+        cur_hi = 1 - cur_hi;
+        since_rebase = 0;
+    } else {
+        since_rebase += 2;
+    }
+    if (h_avg) {
+        cur_pole = 1 - cur_pole;
+        poles_virtual = true;
+    }
+    if (diag) diag_commit(2, static_cast<int>(grid2.x * grid2.y), h_avg, a.hpole_mid, a.hpole_out);
+    cur_mid = 1 - cur_mid;
+    steps_taken += 2;
+}
+
+void Drainage::step(int n_steps, double dt, bool flush) {
+    if (tables_dirty) upload_tables();
+    dt = resolve_dt(dt);
+    for (int s = 0; s < n_steps;) {
+        if (two_step_next(n_steps - s)) {
+            launch_two(dt, false);
+            s += 2;
         } else {
-            finalize_diag();
+            launch_one(dt);
+            s += 1;
         }
-        const StepFn fn = step_fn(rebase, gmode, marangoni, diag, vec4, rows_per_block);
-        if (step_timer.enabled) {
-            auto& ev = step_timer.next(stream);
-            fn(grid, blk, stream, ev.first, ev.second, a);
-        } else {
-            fn(grid, blk, stream, nullptr, nullptr, a);
-        }
-        SF_HIP(hipGetLastError());
-        cur = 1 - cur;
-        if (surfactant) cur_g = 1 - cur_g;
-        if (rebase) {
-            cur_hi = 1 - cur_hi;
-            since_rebase = 0;
-        } else {
-            ++since_rebase;
-        }
-        if (h_avg || g_avg) {
-            cur_pole = 1 - cur_pole;
-            poles_virtual = true;
-        }
-        if (diag) {
-            pending_part = 1 - pending_part;
-            pending_slot = static_cast<int>(hist_count % kHistCap);
-            pending_nblocks = static_cast<int>(grid.x * grid.y);
-            pending_pole_sum = h_avg;
-            ++hist_count;
-        }
-        current_time += dt;
-        ++steps_taken;
     }
     if (flush) materialize();
 }

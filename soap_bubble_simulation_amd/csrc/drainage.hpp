@@ -33,7 +33,20 @@ struct KernelTimer {
 };
 
 struct Drainage;
-void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st);
+
+// A film state on the device: h = hi + lo, with the pole rows held as ring
+// partial sums when pole_virtual is set.
+struct FieldView {
+    const float* hi;
+    const float* lo;
+    const float2* hpole;
+    bool pole_virtual;
+};
+
+// Spectral RGB of the context's current state, or of `view` when given,
+// into the context's RGB planes, or into `rgb_out` (device) when given.
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st,
+                     const FieldView* view = nullptr, float* rgb_out = nullptr);
 
 struct Drainage {
     static constexpr int64_t kHistCap = 1 << 16;
@@ -63,21 +76,37 @@ struct Drainage {
     float2* hpole[2] = {nullptr, nullptr};
     float2* gpole[2] = {nullptr, nullptr};
     int cur_pole = 0;
+    // The state between the two steps of a two-step launch, for consumers
+    // of every step: its low word (allocated on first use) and ring partial
+    // sums (ping-pong, so a pending diagnosis of one launch's first step
+    // survives the next launch).
+    float* lo_mid = nullptr;
+    float2* hpole_mid[2] = {nullptr, nullptr};
+    int cur_mid = 0;
     bool poles_virtual = false;   // pole rows are held as ring partials
     RowF* rows = nullptr;
     RowD* rowd = nullptr;
     bool tables_dirty = true;
     double pole_area_top = 0.0, pole_area_bot = 0.0, pole_mass_fixed = 0.0;
 
-    // FP64 diagnostics
+    // FP64 diagnostics: per-block partials of each diagnosed step, reduced
+    // into the history by the next launch (or finalize_diag); the steps of
+    // the last launch (one or two) are pending until then.
+    static constexpr int kDiagBufs = 4;
+    struct PendingDiag {
+        int buf;                  // diag_part index
+        int slot;                 // history slot
+        int nblocks;
+        bool pole_sum;
+        const float2* hpole;      // the step's ring partial sums
+    };
     size_t nblocks_max = 0;
-    double2* diag_part[2] = {nullptr, nullptr};   // per-block partials, ping-pong by step
+    double2* diag_part[kDiagBufs] = {nullptr, nullptr, nullptr, nullptr};
     double2* diag_hist = nullptr;
     int64_t hist_count = 0;
-    int pending_part = 0;         // diag_part half written by the last diagnosed step
-    int pending_slot = -1;        // its history slot while not yet reduced
-    int pending_nblocks = 0;
-    bool pending_pole_sum = false;
+    PendingDiag pending[2] = {{0, -1, 0, false, nullptr}, {0, -1, 0, false, nullptr}};
+    int npending = 0;
+    int new_buf[2] = {0, 1};      // diag_part buffers of the launch being set up
     StatsPart* stats_part = nullptr;
     StatsPart* stats_out = nullptr;
 
@@ -135,6 +164,20 @@ struct Drainage {
     void step(int n_steps, double dt, bool flush = true);
     // Whether drainage_band2_kernel serves these flags and this grid.
     bool two_step_ok() const;
+    // Whether the next two of `remaining` steps (remaining >= 2) go into one
+    // launch: two steps per launch are on and no rebase falls between them.
+    bool two_step_next(int remaining) const;
+    // One launch of one step, or of two steps (with_mid: also writing the
+    // state between them).  dt is resolved; tables are uploaded.
+    void launch_one(double dt);
+    void launch_two(double dt, bool with_mid);
+    // The state between the steps of the last two-step launch made with
+    // with_mid: high word, low word, ring partials.
+    const float* mid_hi = nullptr;
+    const float2* mid_hpole = nullptr;
+    // Pending-diagnosis bookkeeping of a launch's steps.
+    void diag_args(StepArgs& a, int nsteps);
+    void diag_commit(int nsteps, int nblocks, bool pole_sum, const float2* hpole_first, const float2* hpole_last);
     StatsPart compute_stats();
     void ensure_gather(size_t count);
 };

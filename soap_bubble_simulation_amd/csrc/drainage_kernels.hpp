@@ -44,6 +44,18 @@ struct RowD {          // per theta row, FP64 (diagnostics only)
     double pad;
 };
 
+// A diagnosed step whose per-block partials the next launch reduces: the
+// partials, the step's ring partial sums (its pole-row film volume), and its
+// history slot.
+struct DiagPending {
+    const double2* part;
+    const float2* hpole;
+    int nblocks;
+    int pole_sum;
+    int slot;
+    int pad;
+};
+
 struct StepArgs {
     const float* hi_in;
     const float* lo_in;
@@ -58,12 +70,10 @@ struct StepArgs {
     const float2* gpole_in;
     float2* gpole_out;
     double2* diag_part;       // this step's per-block {mass, max lambda}
-    const double2* diag_prev; // the previous step's per-block partials
-    const float2* hpole_prev; // the previous step's ring partials (its pole mass)
+    double2* diag_part2;      // two-step launches: the second step's partials
+    DiagPending prev[2];      // steps of the previous launch not yet reduced
+    int nprev;
     double2* diag_hist;       // ring buffer {mass, dt_cfl} per step
-    int prev_slot;            // history slot of the previous step, -1 if none
-    int prev_nblocks;
-    int prev_pole_sum;
     int nt, np, ntiles, rows_per_block;
     int h_pole_mean;          // ring-1 theta neighbour = mean from hpole_in
     int g_pole_mean;
@@ -81,6 +91,12 @@ struct StepArgs {
     float np_f;
     int nfronts;
     FrontBox fr[kMaxFronts];
+    // two-step launches: the second step's fronts, and the state between the
+    // two steps (its low word and ring partial sums; null: not written)
+    int nfronts2;
+    FrontBox fr2[kMaxFronts];
+    float* lo_mid;
+    float2* hpole_mid;
     double coef3;             // 3 rho g / (3 eta), FP64
     double pole_area_top, pole_area_bot;
     double pole_mass_fixed;   // film volume of the pole rows when they are copied through
@@ -118,17 +134,31 @@ __device__ inline double wave_dmax(double v) {
     return v;
 }
 
-__device__ inline bool in_fronts(const StepArgs& a, int i, int j) {
+__device__ inline bool in_front_boxes(const FrontBox* fr, int nfronts, int np, int i, int j) {
     bool hit = false;
-    for (int k = 0; k < a.nfronts; ++k) {
-        const FrontBox f = a.fr[k];
+    for (int k = 0; k < nfronts; ++k) {
+        const FrontBox f = fr[k];
         if (i >= f.ilo && i <= f.ihi) {
             int d = j - f.j0;
-            if (d < 0) d += a.np;
+            if (d < 0) d += np;
             hit |= d < f.len;
         }
     }
     return hit;
+}
+
+__device__ inline bool in_fronts(const StepArgs& a, int i, int j) {
+    return in_front_boxes(a.fr, a.nfronts, a.np, i, j);
+}
+
+// Thinning inside a front (drainage.rs:441-459): h <- h * factor with
+// factor - 1 = tfe_g_hi + tfe_g_lo, added to the low word as a float-float
+// product of (ch, lo) and the factor's offset.
+__device__ __forceinline__ float front_thin(const StepArgs& a, float ch, float lo) {
+    const float p = ch * a.tfe_g_hi;
+    const float pe = fmaf(ch, a.tfe_g_hi, -p);
+    const float corr = pe + (ch * a.tfe_g_lo + lo * a.tfe_g_hi);
+    return lo + (p + corr);
 }
 
 __device__ inline double atomic_ld(const double* p) {
@@ -180,13 +210,29 @@ __device__ inline void diag_reduce(const double2* part, int nblocks, const float
     __syncthreads();
 }
 
+// Block 0 of a diagnosed launch reduces the previous launch's steps (one or
+// two) into their history slots.
+__device__ inline void diag_reduce_pending(const StepArgs& a, double (&red_d)[2][kBlockMax / 64]) {
+    __syncthreads();
+    diag_reduce(a.prev[0].part, a.prev[0].nblocks, a.prev[0].hpole, a.ntiles, a.prev[0].pole_sum,
+                a.pole_area_top, a.pole_area_bot, a.pole_mass_fixed, a.diag_hist, a.prev[0].slot, red_d);
+    if (a.nprev > 1)
+        diag_reduce(a.prev[1].part, a.prev[1].nblocks, a.prev[1].hpole, a.ntiles, a.prev[1].pole_sum,
+                    a.pole_area_top, a.pole_area_bot, a.pole_mass_fixed, a.diag_hist, a.prev[1].slot, red_d);
+}
+
 // Per-block FP64 partials of film volume and CFL lambda for this step (no
 // fence and no atomics: they are consumed by the next launch); block 0 also
 // reduces the previous step's partials into the history.
-__device__ inline void diag_epilogue(const StepArgs& a, double mass, double lam, int lane, int wave,
-                                     int nwaves, double (&red_d)[2][kBlockMax / 64]) {
-    mass = wave_dsum(mass);
-    lam = wave_dmax(lam);
+__device__ inline void diag_block_partial(double mass, double lam, double2* part, int lane, int wave,
+                                          int nwaves, double (&red_d)[2][kBlockMax / 64]) {
+    // the sum and the maximum in one pass, so the shuffles of both overlap
+    for (int off = 32; off > 0; off >>= 1) {
+        const double m = __shfl_down(mass, off);
+        const double l = __shfl_down(lam, off);
+        mass += m;
+        lam = fmax(lam, l);
+    }
     __syncthreads();
     if (lane == 0) {
         red_d[0][wave] = mass;
@@ -199,13 +245,28 @@ __device__ inline void diag_epilogue(const StepArgs& a, double mass, double lam,
             m += red_d[0][w];
             l = fmax(l, red_d[1][w]);
         }
-        a.diag_part[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(m, l);
+        part[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(m, l);
     }
-    if (a.prev_slot >= 0 && blockIdx.x == 0 && blockIdx.y == 0) {
-        __syncthreads();
-        diag_reduce(a.diag_prev, a.prev_nblocks, a.hpole_prev, a.ntiles, a.prev_pole_sum,
-                    a.pole_area_top, a.pole_area_bot, a.pole_mass_fixed, a.diag_hist, a.prev_slot, red_d);
-    }
+}
+
+__device__ inline void diag_epilogue(const StepArgs& a, double mass, double lam, int lane, int wave,
+                                     int nwaves, double (&red_d)[2][kBlockMax / 64]) {
+    diag_block_partial(mass, lam, a.diag_part, lane, wave, nwaves, red_d);
+    if (a.nprev > 0 && blockIdx.x == 0 && blockIdx.y == 0) diag_reduce_pending(a, red_d);
+}
+
+// A lane's four cells of one row into its FP64 film volume and CFL lambda:
+// the cells share the row's area and CFL terms, so their FP64 sum is
+// multiplied once per row.
+__device__ __forceinline__ void diag_row(const RowD& rd, double coef3, const float (&h)[4], const float (&l)[4],
+                                         double& mass, double& lam) {
+    double hsum = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) hsum += static_cast<double>(h[v]) + static_cast<double>(l[v]);
+    const float hmax = fmaxf(fmaxf(h[0] + l[0], h[1] + l[1]), fmaxf(h[2] + l[2], h[3] + l[3]));
+    mass = fma(hsum, rd.area, mass);
+    const double hm = static_cast<double>(hmax);
+    lam = fmax(lam, rd.lam_grid + coef3 * rd.sin_abs * hm * hm);
 }
 
 // GMODE: 0 = Γ unused, 1 = Γ read (Marangoni) but frozen, 2 = Γ evolves.
@@ -346,12 +407,7 @@ __global__ __launch_bounds__(kBlockMax) void drainage_step_kernel(const StepArgs
 
         // Marginal regeneration: h *= 1 - rate dt on front cells, rupture
         // ignored (drainage.rs:440-460).
-        if (a.nfronts > 0 && in_fronts(a, i, jc)) {
-            const float p = ch * a.tfe_g_hi;
-            const float pe = fmaf(ch, a.tfe_g_hi, -p);
-            const float corr = pe + (ch * a.tfe_g_lo + lo_new * a.tfe_g_hi);
-            lo_new = lo_new + (p + corr);
-        }
+        if (a.nfronts > 0 && in_fronts(a, i, jc)) lo_new = front_thin(a, ch, lo_new);
 
         if (valid) {
             const size_t c = at(i, j);
@@ -583,12 +639,7 @@ __global__ __launch_bounds__(kBlockMax) void drainage_vec4_kernel(const StepArgs
         if (a.nfronts > 0) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                if (in_fronts(a, i, cc + v)) {
-                    const float p = ch[v] * a.tfe_g_hi;
-                    const float pe = fmaf(ch[v], a.tfe_g_hi, -p);
-                    const float corr = pe + (ch[v] * a.tfe_g_lo + lo_new[v] * a.tfe_g_hi);
-                    lo_new[v] = lo_new[v] + (p + corr);
-                }
+                if (in_fronts(a, i, cc + v)) lo_new[v] = front_thin(a, ch[v], lo_new[v]);
             }
         }
 
@@ -608,20 +659,7 @@ __global__ __launch_bounds__(kBlockMax) void drainage_vec4_kernel(const StepArgs
             }
             if (GWRITE)
                 *reinterpret_cast<float4*>(a.g_out + off) = make_float4(g_new[0], g_new[1], g_new[2], g_new[3]);
-            if (DIAG) {
-                // The lane's four cells share the row's area and CFL terms:
-                // their FP64 sum is multiplied once per row.
-                double hsum = 0.0;
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    hsum += static_cast<double>(ch[v]) + static_cast<double>(lo_new[v]);
-                const float hmax = fmaxf(fmaxf(ch[0] + lo_new[0], ch[1] + lo_new[1]),
-                                         fmaxf(ch[2] + lo_new[2], ch[3] + lo_new[3]));
-                const RowD rd = s_rowd[DIAG ? t : 0];
-                mass = fma(hsum, rd.area, mass);
-                const double hm = static_cast<double>(hmax);
-                lam = fmax(lam, rd.lam_grid + a.coef3 * rd.sin_abs * hm * hm);
-            }
+            if (DIAG) diag_row(s_rowd[DIAG ? t : 0], a.coef3, ch, lo_new, mass, lam);
         }
         }
     }
@@ -848,12 +886,7 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band_kernel(const StepArgs
         if (a.nfronts > 0) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                if (in_fronts(a, i, cc + v)) {
-                    const float p = ch[v] * a.tfe_g_hi;
-                    const float pe = fmaf(ch[v], a.tfe_g_hi, -p);
-                    const float corr = pe + (ch[v] * a.tfe_g_lo + lo_new[v] * a.tfe_g_hi);
-                    lo_new[v] = lo_new[v] + (p + corr);
-                }
+                if (in_fronts(a, i, cc + v)) lo_new[v] = front_thin(a, ch[v], lo_new[v]);
             }
         }
 
@@ -873,20 +906,7 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band_kernel(const StepArgs
             }
             if (GWRITE)
                 *reinterpret_cast<float4*>(a.g_out + off) = make_float4(g_new[0], g_new[1], g_new[2], g_new[3]);
-            if (DIAG) {
-                // The lane's four cells share the row's area and CFL terms:
-                // their FP64 sum is multiplied once per row.
-                double hsum = 0.0;
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    hsum += static_cast<double>(ch[v]) + static_cast<double>(lo_new[v]);
-                const float hmax = fmaxf(fmaxf(ch[0] + lo_new[0], ch[1] + lo_new[1]),
-                                         fmaxf(ch[2] + lo_new[2], ch[3] + lo_new[3]));
-                const RowD rd = s_rowd[DIAG ? t : 0];
-                mass = fma(hsum, rd.area, mass);
-                const double hm = static_cast<double>(hmax);
-                lam = fmax(lam, rd.lam_grid + a.coef3 * rd.sin_abs * hm * hm);
-            }
+            if (DIAG) diag_row(s_rowd[DIAG ? t : 0], a.coef3, ch, lo_new, mass, lam);
         }
     };
 #define SF_BAND_ROW(Nb, Cb, Sb)                                                  \
@@ -966,10 +986,30 @@ __global__ __launch_bounds__(kBlockMax) void drainage_band_kernel(const StepArgs
 // from the same partial sums in the same order — a block whose band touches a
 // pole recomputes that pole's ring row of step s+1 over every column tile —
 // so a launch equals two launches of drainage_band_kernel bit for bit.
-// Scope: no fronts, no Marangoni term, no fused diagnostics, Γ poles fixed,
-// n_phi a multiple of 256 (whole waves), a rebase only after the second step.
+// Marginal-regeneration fronts act on the step-(s+1) rows in the ring (halo
+// rows and the edge columns included) and on the step-(s+2) rows, each step
+// with its own front boxes.  With DIAG the launch leaves per-block FP64
+// partials for both steps.  With a.lo_mid set it also writes the state
+// between the two steps (its low word; h's high word is unchanged, as a
+// rebase only follows the second step) and that state's ring partial sums,
+// for consumers of every step such as the spectral pass.
+// Scope: no Marangoni term, Γ poles fixed, n_phi a multiple of 256 (whole
+// waves), a rebase only after the second step.
 constexpr int kRing2W = 264;     // LDS row of a wave: edge at 3, columns 4 .. 259, edge at 260
 constexpr int kMaxTiles2 = 64;   // column tiles (n_phi <= 65536 with four-wave blocks)
+
+// Element at a 32-bit byte offset from a uniform base: the global access
+// then takes the base in scalar registers and one 32-bit offset register.
+// The two-step kernel runs on grids below 2^30 cells, whose byte offsets
+// fit 32 bits.
+template <class T>
+__device__ __forceinline__ const T* at_bytes(const float* base, uint32_t off) {
+    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ T* at_bytes(float* base, uint32_t off) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
 
 struct Row5 {
     float4 h, l, g;              // the lane's four columns cc .. cc + 3
@@ -985,10 +1025,10 @@ struct Far5 {                    // lane 0: column cc - 2; other lanes: cc + 5
 // row window.
 template <int GMODE>
 __device__ inline void load_far5(const StepArgs& a, int r, int jf, Far5& F) {
-    const size_t base = static_cast<size_t>(max(0, min(r, a.nt - 1))) * a.np;
-    F.h = a.hi_in[base + jf];
-    F.l = a.lo_in[base + jf];
-    F.g = GMODE >= 1 ? a.g_in[base + jf] : 0.f;
+    const uint32_t off = static_cast<uint32_t>(max(0, min(r, a.nt - 1)) * a.np + jf) * 4u;
+    F.h = *at_bytes<float>(a.hi_in, off);
+    F.l = *at_bytes<float>(a.lo_in, off);
+    F.g = GMODE >= 1 ? *at_bytes<float>(a.g_in, off) : 0.f;
 }
 
 // Loads step-s row r (clamped into the grid).  With pm set, the pole rows
@@ -1000,7 +1040,8 @@ template <int GMODE>
 __device__ inline void load_row5(const StepArgs& a, int r, int cc, int jx, bool pm, float2 ptop, float2 pbot,
                                  Row5& R) {
     const int rr = max(0, min(r, a.nt - 1));
-    const size_t base = static_cast<size_t>(rr) * a.np;
+    const uint32_t oc = static_cast<uint32_t>(rr * a.np + cc) * 4u;
+    const uint32_t ox = static_cast<uint32_t>(rr * a.np + jx) * 4u;
     const bool pole = pm && (rr == 0 || rr == a.nt - 1);
     if (pole) {
         const float2 P = rr == 0 ? ptop : pbot;
@@ -1009,14 +1050,14 @@ __device__ inline void load_row5(const StepArgs& a, int r, int cc, int jx, bool 
         R.hx = P.x;
         R.lx = P.y;
     } else {
-        R.h = *reinterpret_cast<const float4*>(a.hi_in + base + cc);
-        R.l = *reinterpret_cast<const float4*>(a.lo_in + base + cc);
-        R.hx = a.hi_in[base + jx];
-        R.lx = a.lo_in[base + jx];
+        R.h = *at_bytes<float4>(a.hi_in, oc);
+        R.l = *at_bytes<float4>(a.lo_in, oc);
+        R.hx = *at_bytes<float>(a.hi_in, ox);
+        R.lx = *at_bytes<float>(a.lo_in, ox);
     }
     if (GMODE >= 1) {
-        R.g = *reinterpret_cast<const float4*>(a.g_in + base + cc);
-        R.gx = a.g_in[base + jx];
+        R.g = *at_bytes<float4>(a.g_in, oc);
+        R.gx = *at_bytes<float>(a.g_in, ox);
     } else {
         R.g = make_float4(0.f, 0.f, 0.f, 0.f);
         R.gx = 0.f;
@@ -1107,9 +1148,17 @@ __device__ float2 pole_after_first_step(const StepArgs& a, int ring, float2 Pin,
 
 // Three waves per SIMD (168 VGPRs with Γ transport): one step-s row in
 // flight per wave measured faster than three rows at two waves per SIMD.
-template <bool REBASE, int GMODE>
+// OPT: kOptDiag (fused FP64 diagnostics), kOptFronts (marginal-regeneration
+// fronts in either step), kOptMid (write the state between the steps); each
+// its own instantiation, so that launches without them keep their registers.
+constexpr int kOptDiag = 1, kOptFronts = 2, kOptMid = 4;
+
+template <bool REBASE, int GMODE, int OPT>
 __global__ __launch_bounds__(kBlockMax) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void drainage_band2_kernel(const StepArgs a) {
+    constexpr bool DIAG = (OPT & kOptDiag) != 0;
+    constexpr bool FRONTS = (OPT & kOptFronts) != 0;
+    constexpr bool MID = (OPT & kOptMid) != 0;
     constexpr bool GREAD = GMODE >= 1;
     constexpr bool GWRITE = GMODE == 2;
     const int np = a.np, nt = a.nt;
@@ -1150,13 +1199,23 @@ void drainage_band2_kernel(const StepArgs a) {
     __shared__ float2 red_ff[4][kBlockMax / 64];
     __shared__ float2 s_part[kMaxTiles2];
     __shared__ RowF s_rowf[kBandMax + 2];
+    __shared__ RowD s_rowd[DIAG ? kBandMax : 1];
+    __shared__ double red_d[2][kBlockMax / 64];
+    // Each lane's FP64 film volume and CFL lambda of steps s+1 and s+2,
+    // accumulated in LDS (the registers are taken by the row window).
+    __shared__ double s_diag[DIAG ? 4 : 1][DIAG ? kBlockMax : 1];
+    if constexpr (DIAG) {
+        for (int q = 0; q < 4; ++q) s_diag[q][threadIdx.x] = 0.0;
+    }
     __shared__ __align__(16) float ring[kBlockMax / 64][3][3][kRing2W];   // [wave][slot][h, l, g][column]
 
     // Row coefficients of rows r0 - 1 .. r1 (step s+1 rows r0 - 1 .. r1 - 1 are
-    // formed with those of their own rows; step s+2 rows r0 .. r1 - 1).
+    // formed with those of their own rows; step s+2 rows r0 .. r1 - 1), and
+    // the FP64 terms of the band's own rows r0 .. r1 - 1.
     for (int t = threadIdx.x; t < RB + 2; t += blockDim.x) {
         const int ri = min(max(r0 - 1 + t, 0), nt - 1);
         s_rowf[t] = a.rows[ri];
+        if (DIAG && t < RB) s_rowd[t] = a.rowd[min(r0 + t, nt - 1)];
     }
     // Poles of step s (P0) and of step s+1 (P1).
     float2 p0top = make_float2(0.f, 0.f), p0bot = make_float2(0.f, 0.f);
@@ -1167,8 +1226,17 @@ void drainage_band2_kernel(const StepArgs a) {
     float2 p1top = make_float2(0.f, 0.f), p1bot = make_float2(0.f, 0.f);
     const bool need_top = a.h_pole_sum && touches_top;           // step s+1 row 0 is used
     const bool need_bot = a.h_pole_sum && touches_bot;           // step s+1 row nt - 1 is used
-    if (need_top) p1top = pole_after_first_step<GMODE>(a, 1, p0top, s_part, red_ff[0]);
-    if (need_bot) p1bot = pole_after_first_step<GMODE>(a, nt - 2, p0bot, s_part, red_ff[1]);
+    // The partials of step s+1 go out with the state between the steps, and
+    // for that step's pole-row film volume (written by one block per pole).
+    const bool mid_poles = (MID || DIAG) && bx == 0;
+    if (need_top) {
+        p1top = pole_after_first_step<GMODE>(a, 1, p0top, s_part, red_ff[0]);
+        if (mid_poles && threadIdx.x < a.ntiles) a.hpole_mid[threadIdx.x] = s_part[threadIdx.x];
+    }
+    if (need_bot) {
+        p1bot = pole_after_first_step<GMODE>(a, nt - 2, p0bot, s_part, red_ff[1]);
+        if (mid_poles && threadIdx.x < a.ntiles) a.hpole_mid[a.ntiles + threadIdx.x] = s_part[threadIdx.x];
+    }
     // Every lane holds the same pole means: keep them in scalar registers.
     auto uniform = [](float2 v) {
         return make_float2(__int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.x))),
@@ -1216,6 +1284,14 @@ void drainage_band2_kernel(const StepArgs a) {
         float xlo, xg;
         cell_step<GMODE, false>(a, rc, G0, G1, G2, dtD, cS, xch, xcl, xnh, xnl, xsh, xsl, xwh, xwl, xeh, xel, xcg, xng,
                                 xsg, xwg, xeg, xlo, xg);
+        // Fronts of step s+1 (never on a pole row), the edge column with its
+        // own index: lane 0 column cc - 1, lane 63 column cc + 4.
+        if (FRONTS && a.nfronts > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (in_front_boxes(a.fr, a.nfronts, np, k, cc + v)) lo1[v] = front_thin(a, ch[v], lo1[v]);
+            if (in_front_boxes(a.fr, a.nfronts, np, k, jx)) xlo = front_thin(a, xch, xlo);
+        }
         // A pole row of step s+1 is not stepped: h and its low word are the
         // pole mean P1 (or, with copied poles, the stored row) and Γ is
         // copied through.  Its cells were formed above from clamped rows and
@@ -1241,6 +1317,13 @@ void drainage_band2_kernel(const StepArgs a) {
         if (GREAD) *reinterpret_cast<float4*>(rg + lc) = make_float4(og[0], og[1], og[2], og[3]);
         if (lane == 0) { rh[3] = xoh; rl[3] = xol; if (GREAD) rg[3] = xog; }
         if (lane == 63) { rh[260] = xoh; rl[260] = xol; if (GREAD) rg[260] = xog; }
+        const bool own = k >= r0 && k < r1;
+        if (MID && valid && (own || pole))
+            *at_bytes<float4>(a.lo_mid, static_cast<uint32_t>(k * np + cc) * 4u) =
+                make_float4(ol[0], ol[1], ol[2], ol[3]);
+        if constexpr (DIAG) {
+            if (valid && own) diag_row(s_rowd[k - r0], a.coef3, oh, ol, s_diag[0][threadIdx.x], s_diag[1][threadIdx.x]);
+        }
     };
 
     // The lane's float-float sums of its ring-row cells of step s+2 (rows 1
@@ -1294,20 +1377,28 @@ void drainage_band2_kernel(const StepArgs a) {
                                     eh[v], el[v], cg[v], ng[v], sg[v], wg[v], eg[v], lo2[v], g2[v]);
         if (i == 1) sum_top = lane_ring_sum(ch, lo2);
         if (i == nt - 2) sum_bot = lane_ring_sum(ch, lo2);
+        if (FRONTS && a.nfronts2 > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (in_front_boxes(a.fr2, a.nfronts2, np, i, cc + v)) lo2[v] = front_thin(a, ch[v], lo2[v]);
+        }
+        if constexpr (DIAG) {
+            if (valid) diag_row(s_rowd[i - r0], a.coef3, ch, lo2, s_diag[2][threadIdx.x], s_diag[3][threadIdx.x]);
+        }
         if (valid) {
-            const size_t off = static_cast<size_t>(i) * np + cc;
+            const uint32_t off = static_cast<uint32_t>(i * np + cc) * 4u;
             if (REBASE) {
                 float4 hh, ll;
                 two_sum(ch[0], lo2[0], hh.x, ll.x);
                 two_sum(ch[1], lo2[1], hh.y, ll.y);
                 two_sum(ch[2], lo2[2], hh.z, ll.z);
                 two_sum(ch[3], lo2[3], hh.w, ll.w);
-                *reinterpret_cast<float4*>(a.hi_out + off) = hh;
-                *reinterpret_cast<float4*>(a.lo_out + off) = ll;
+                *at_bytes<float4>(a.hi_out, off) = hh;
+                *at_bytes<float4>(a.lo_out, off) = ll;
             } else {
-                *reinterpret_cast<float4*>(a.lo_out + off) = make_float4(lo2[0], lo2[1], lo2[2], lo2[3]);
+                *at_bytes<float4>(a.lo_out, off) = make_float4(lo2[0], lo2[1], lo2[2], lo2[3]);
             }
-            if (GWRITE) *reinterpret_cast<float4*>(a.g_out + off) = make_float4(g2[0], g2[1], g2[2], g2[3]);
+            if (GWRITE) *at_bytes<float4>(a.g_out, off) = make_float4(g2[0], g2[1], g2[2], g2[3]);
         }
     };
 
@@ -1369,6 +1460,40 @@ void drainage_band2_kernel(const StepArgs a) {
             }
             if (touches_top) a.hpole_out[bx] = t0;
             if (touches_bot) a.hpole_out[a.ntiles + bx] = t1;
+        }
+    }
+    if constexpr (DIAG) {
+        // Block partials of the four accumulators (volume and lambda of both
+        // steps), one wave each: a lane folds the entries lane, lane + 64, ...
+        // in order, then the wave reduces (the reductions of the four waves
+        // run side by side).
+        __syncthreads();
+        const unsigned slot = b;      // partials are indexed by hardware block
+        for (int q = wave; q < 4; q += nwaves) {
+            const bool is_max = (q & 1) != 0;
+            double v = 0.0;
+            for (int t = lane; t < static_cast<int>(blockDim.x); t += 64)
+                v = is_max ? fmax(v, s_diag[q][t]) : v + s_diag[q][t];
+            v = is_max ? wave_dmax(v) : wave_dsum(v);
+            if (lane == 0) {
+                double2* part = q < 2 ? a.diag_part : a.diag_part2;
+                if (is_max)
+                    part[slot].y = v;
+                else
+                    part[slot].x = v;
+            }
+        }
+        // The previous launch's steps are reduced by the first inner-band
+        // blocks, one step each: the pole-band blocks, dispatched first, are
+        // the longest, and a reduction appended to one of them would end
+        // the launch late.
+        const unsigned first_inner = gridDim.y > 2 ? 2 * gridDim.x : 0;
+        for (int q = 0; q < a.nprev; ++q) {
+            if (b != (first_inner + static_cast<unsigned>(q)) % nblk) continue;
+            const DiagPending e = q == 0 ? a.prev[0] : a.prev[1];
+            __syncthreads();
+            diag_reduce(e.part, e.nblocks, e.hpole, a.ntiles, e.pole_sum, a.pole_area_top, a.pole_area_bot,
+                        a.pole_mass_fixed, a.diag_hist, e.slot, red_d);
         }
     }
 }

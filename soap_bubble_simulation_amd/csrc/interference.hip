@@ -857,12 +857,13 @@ static int spectral_format(const Drainage* d) {
 
 // Launches spectral_kernel on stream st over the context's current state
 // (lazy pole rows are taken from the ring partials).
-void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st) {
+void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const float* cmf, hipStream_t st,
+                     const FieldView* view, float* rgb_out) {
     spectral_prepare(d, cmf);
     SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
     SpecArgs a;
-    a.hi = d->hi[d->cur_hi];
-    a.lo = d->lo[d->cur];
+    a.hi = view ? view->hi : d->hi[d->cur_hi];
+    a.lo = view ? view->lo : d->lo[d->cur];
     a.rows = d->spec_rows;
     a.cols = d->spec_cols;
     const int f = spectral_format(d);
@@ -872,11 +873,12 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.s1 = d->cmf_sum[f][1];
     a.s2 = d->cmf_sum[f][2];
     a.b_lo = d->cmf_lo[f];
-    a.hpole = d->hpole[d->cur_pole];
-    a.pole_virtual = (d->poles_virtual && !(d->p.flags & SF_FLAG_POLES_COPY_H)) ? 1 : 0;
+    a.hpole = view ? view->hpole : d->hpole[d->cur_pole];
+    a.pole_virtual = view ? (view->pole_virtual ? 1 : 0)
+                          : ((d->poles_virtual && !(d->p.flags & SF_FLAG_POLES_COPY_H)) ? 1 : 0);
     a.nt = d->nt;
     a.ntiles = d->ntiles;
-    a.rgb = d->rgb;
+    a.rgb = rgb_out ? rgb_out : d->rgb;
     a.np = d->np;
     a.n = d->n;
     const sf_camera c = cam ? *cam : sf_camera{{0.0, 0.0, 4.0 * d->p.bubble_radius}};
@@ -929,28 +931,57 @@ extern "C" int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double re
     });
 }
 
+// n_steps of {drainage step, spectral RGB of the new state}; frame s goes to
+// frames + 3 n s when frames (device) is given, else to the context's planes.
+static void step_spectral(Drainage* d, int32_t n_steps, double dt, const sf_camera* cam, double refractive_index,
+                          const float* cmf, float* frames) {
+    SF_REQUIRE(n_steps >= 0, SF_ERR_INVALID, "bad arguments");
+    SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
+    SF_REQUIRE(std::isfinite(dt), SF_ERR_INVALID, "dt must be finite");
+    if (n_steps > 0) dt = d->resolve_dt(dt);
+    if (n_steps > 0) {
+        spectral_prepare(d, cmf);
+        // Curves and operand format are checked before any step is queued.
+        SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
+        (void)spectral_format(d);
+        if (d->tables_dirty) d->upload_tables();
+    }
+    auto frame = [&](int s) { return frames ? frames + 3 * d->n * static_cast<size_t>(s) : nullptr; };
+    // The pole rows stay lazy (ring partials): the spectral kernel takes
+    // their ring mean itself, and the next step reads the partials.  Where
+    // two steps go into one stencil launch, that launch also writes the
+    // state between them, and both states are rendered.
+    for (int s = 0; s < n_steps;) {
+        if (d->two_step_next(n_steps - s)) {
+            d->launch_two(dt, true);
+            const FieldView mid{d->mid_hi, d->lo_mid, d->mid_hpole, !(d->p.flags & SF_FLAG_POLES_COPY_H)};
+            spectral_launch(d, cam, refractive_index, nullptr, d->stream, &mid, frame(s));
+            spectral_launch(d, cam, refractive_index, nullptr, d->stream, nullptr, frame(s + 1));
+            s += 2;
+        } else {
+            d->launch_one(dt);
+            spectral_launch(d, cam, refractive_index, nullptr, d->stream, nullptr, frame(s));
+            s += 1;
+        }
+    }
+    d->finalize_diag();
+}
+
 extern "C" int sf_drainage_step_spectral(sf_drainage* ctx, int32_t n_steps, double dt,
                                          const sf_camera* cam, double refractive_index,
                                          const float* cmf) {
     return guard([&] {
-        SF_REQUIRE(ctx != nullptr && n_steps >= 0, SF_ERR_INVALID, "bad arguments");
-        SF_REQUIRE(refractive_index > 0.0, SF_ERR_INVALID, "refractive index must be positive");
-        SF_REQUIRE(std::isfinite(dt), SF_ERR_INVALID, "dt must be finite");
-        Drainage* d = reinterpret_cast<Drainage*>(ctx);
-        if (n_steps > 0) dt = d->resolve_dt(dt);
-        if (n_steps > 0) {
-            spectral_prepare(d, cmf);
-            // Curves and operand format are checked before any step is queued.
-            SF_REQUIRE(d->cmf_dev != nullptr, SF_ERR_INVALID, "no colour-matching curves given yet");
-            (void)spectral_format(d);
-        }
-        // The pole rows stay lazy (ring partials): the spectral kernel takes
-        // their ring mean itself, and the next step reads the partials.
-        for (int s = 0; s < n_steps; ++s) {
-            d->step(1, dt, false);
-            spectral_launch(d, cam, refractive_index, nullptr, d->stream);
-        }
-        d->finalize_diag();
+        SF_REQUIRE(ctx != nullptr, SF_ERR_INVALID, "ctx is NULL");
+        step_spectral(reinterpret_cast<Drainage*>(ctx), n_steps, dt, cam, refractive_index, cmf, nullptr);
+    });
+}
+
+extern "C" int sf_drainage_step_spectral_frames(sf_drainage* ctx, int32_t n_steps, double dt,
+                                                const sf_camera* cam, double refractive_index,
+                                                const float* cmf, float* frames_dev) {
+    return guard([&] {
+        SF_REQUIRE(ctx != nullptr && frames_dev != nullptr, SF_ERR_INVALID, "NULL argument");
+        step_spectral(reinterpret_cast<Drainage*>(ctx), n_steps, dt, cam, refractive_index, cmf, frames_dev);
     });
 }
 
