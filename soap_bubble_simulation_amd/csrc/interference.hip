@@ -240,7 +240,9 @@ __global__ void calc_kernel(CalcArgs a) {
 // a cell of a 16-cell group; lane (m, q) evaluates r for wavelengths
 // k = q + 4 s, s = 0..20 (84 slots, 81 used, B zero on the rest), slot s in
 // MFMA t = s / 8 at v = s % 8; slots 21..23 are zero.  A wave takes 64 cells
-// at a time as four 16-cell groups.
+// at a time as four 16-cell groups.  The phases of slots 0..19 come from
+// small FP16 MFMAs (v_mfma_f32_4x4x4_16b_f16, see phase_parts) rather than
+// one VALU multiply per slot.
 //
 // Operand formats.  r lies in [0, 1], where FP16 (11-bit significand) rounds
 // to nearest within 2^-12, so one FP16 operand changes channel c by at most
@@ -258,10 +260,13 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSpecLambdas = 81;
 constexpr int kSpecSlots = 21;      // wavelength slots per lane and cell
 constexpr int kSpecMfma = 3;        // 16x16x32 MFMAs per group and operand part
+constexpr int kSpecPhase = 5;       // 4x4x4 (16-block) phase MFMAs per group (slots 0..19)
 constexpr float kSpecFCos2 = 16.f;
 // Largest RGB change the FP16 rounding of r may cause for the FP16 operand
 // format to be chosen automatically (a quarter of the 1e-3 RGB tolerance).
@@ -277,6 +282,7 @@ struct SpecArgs {
     const float2* cols;     // per phi column: sin, cos
     const u32x4* bfrag;     // [2 (hi, lo)][3 MFMAs][64 lanes] B fragments of the operand format
     const float* ulam;      // [21 slots][4 q] 1 / (2 lambda) of wavelength q + 4 s (0 past 80)
+    const u32x2* pfrag;     // [5 MFMAs][64 lanes] A fragments of the phase products (FP16 parts of 512 / lambda)
     float* rgb;             // 3 planes of n cells
     int np;
     size_t n;
@@ -352,15 +358,13 @@ __device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, size_t last
     opc = 2.f * a.n_film * d_nm * cos_t;
 }
 
-// r of one slot.  FAST: P = 2 opd, A = 1 + F/2, B = F/2 (cos 2x form);
-// otherwise P = opd, A = 1, B = F (square form, reduced argument).
+// r of one slot from its phase x in revolutions.  FAST: x = opd / lambda,
+// A = 1 + F/2, B = F/2 (cos 2x form); otherwise x = opd / (2 lambda), A = 1,
+// B = F (square form, reduced argument).
 template <bool FAST>
-__device__ inline float spec_r(float P, float A, float B, float u) {
-    if (FAST) {
-        const float C = __builtin_amdgcn_cosf(P * u);
-        return __builtin_amdgcn_rcpf(fmaf(B, C, A));
-    }
-    const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(P * u));
+__device__ inline float spec_r_x(float x, float A, float B) {
+    if (FAST) return __builtin_amdgcn_rcpf(fmaf(B, __builtin_amdgcn_cosf(x), A));
+    const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(x));
     return __builtin_amdgcn_rcpf(fmaf(B, c * c, A));
 }
 
@@ -369,20 +373,74 @@ __device__ inline uint32_t pack_hi16(float x0, float x1) {
     return __builtin_amdgcn_perm(__float_as_uint(x1), __float_as_uint(x0), 0x07060302u);
 }
 
+// Two FP16 values (round to nearest even) in one register: x0 low, x1 high.
+__device__ inline uint32_t pack_f16(float x0, float x1) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{x0, x1}, f16x2));
+}
+
+// The phase products.  x = P / (2 lambda) of a cell and a wavelength slot is
+// P' U with P' = P 2^-10 and U = 512 / lambda (nm), each split into two FP16
+// parts, P' ~ p0 + 2^-11 p1 and U ~ u0 + 2^-11 u1 (p1, u1 the scaled
+// remainders, so the parts are normal FP16 numbers away from tiny values):
+//   x = p0 u0 + 2^-11 p0 u1 + 2^-11 p1 u0 + 2^-22 p1 u1,
+// the exact product of the two 22-bit approximations (relative error of x
+// at most about 2^-22, a phase error below 1e-5 rad for films up to 1 um).
+// The four products are the K terms of v_mfma_f32_4x4x4_16b_f16 (16
+// independent 4 x 4 x 4 blocks, two passes): lane l holds row l % 4 of
+// block l / 4 in A, column l % 4 of block l / 4 in B, and receives column
+// l % 4 of block l / 4 of D, rows 0..3.  B columns are cells (lane (m, q)
+// supplies cell m of the group: [p0, 2^-11 p0, p1, 2^-11 p1]), A rows are
+// the wavelengths q + 4 (4 e + i) of the lane's group q (host-packed:
+// [u0, u1, 2^-11 u0, 2^-11 u1]), so lane (m, q) receives x of its own cell
+// at slots 4 e .. 4 e + 3, in the registers the contraction reads.
+struct PhaseParts {
+    uint32_t x0, x1;       // [p0, 2^-11 p0], [p1, 2^-11 p1] as FP16 pairs
+};
+
+__device__ inline PhaseParts phase_parts(float P) {
+    // P' = P 2^-10 <= 65504 (optical paths below 3.3e7 nm; longer ones are
+    // clamped and lose the phase, which FP32 had already lost there).
+    const float Pp = fminf(P * 0x1p-10f, 65504.f);
+    const float p0 = static_cast<float>(static_cast<_Float16>(Pp));
+    const float p1 = (Pp - p0) * 0x1p11f;
+    PhaseParts o;
+    o.x0 = pack_f16(p0, p0 * 0x1p-11f);
+    o.x1 = pack_f16(p1, p1 * 0x1p-11f);
+    return o;
+}
+
+__device__ inline uint32_t lane_read_u(uint32_t v, int src_lane) {
+    return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src_lane << 2, static_cast<int>(v)));
+}
+
+// One 16-cell group: the phases of slots 0..19 on the phase MFMAs, slot 20
+// (wavelengths 80..83, 81..83 padding) as one multiply; r on the VALU; r
+// contracted with the curves on 16x16x32 MFMAs into acc.
 template <bool FAST, bool F16>
-__device__ inline void spec_group(const float (&u)[kSpecSlots], const u32x4 (&bh)[kSpecMfma],
+__device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, float u20, const u32x4 (&bh)[kSpecMfma],
                                   const u32x4 (&bl)[kSpecMfma], bool b_lo, float P, float A, float B,
                                   v4f& acc) {
+    const f16x4 bp = __builtin_bit_cast(f16x4, bop);
+    const v4f z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < kSpecMfma; ++t) {
         float r[8];
+        const v4f x0 = __builtin_amdgcn_mfma_f32_4x4x4f16(__builtin_bit_cast(f16x4, pa[2 * t]), bp, z, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 8; ++v) r[v] = (8 * t + v < kSpecSlots) ? spec_r<FAST>(P, A, B, u[8 * t + v]) : 0.f;
+        for (int v = 0; v < 4; ++v) r[v] = spec_r_x<FAST>(x0[v], A, B);
+        if (2 * t + 1 < kSpecPhase) {
+            const v4f x1 =
+                __builtin_amdgcn_mfma_f32_4x4x4f16(__builtin_bit_cast(f16x4, pa[2 * t + 1]), bp, z, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) r[4 + v] = spec_r_x<FAST>(x1[v], A, B);
+        } else {
+            r[4] = spec_r_x<FAST>(P * u20, A, B);
+            r[5] = r[6] = r[7] = 0.f;
+        }
         if (F16) {
             u32x4 h;
 #pragma unroll
-            for (int p = 0; p < 4; ++p)
-                h[p] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r[2 * p], r[2 * p + 1]}, f16x2));
+            for (int p = 0; p < 4; ++p) h[p] = pack_f16(r[2 * p], r[2 * p + 1]);
             const f16x8 ah = __builtin_bit_cast(f16x8, h);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(f16x8, bh[t]), acc, 0, 0, 0);
             if (b_lo) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(f16x8, bl[t]), acc, 0, 0, 0);
@@ -415,10 +473,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     const size_t wave_global = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const size_t nwaves = (static_cast<size_t>(gridDim.x) * blockDim.x) >> 6;
 
-    // 1 / (2 lambda) of this lane's slots (0 on the padding) and its B fragments.
-    float u[kSpecSlots];
+    // This lane's phase-MFMA A fragments, 1 / (2 lambda) of its slot 20 (0 on
+    // the padding) and its B fragments.
+    u32x2 pa[kSpecPhase];
 #pragma unroll
-    for (int s = 0; s < kSpecSlots; ++s) u[s] = a.ulam[4 * s + q];
+    for (int e = 0; e < kSpecPhase; ++e) pa[e] = a.pfrag[e * 64 + lane];
+    const float u20 = a.ulam[4 * (kSpecSlots - 1) + q];
     u32x4 bh[kSpecMfma], bl[kSpecMfma];
 #pragma unroll
     for (int t = 0; t < kSpecMfma; ++t) {
@@ -438,22 +498,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         float Fc, opc;
         spec_cell_terms(a, cell < a.n ? cell : a.n - 1, last_row, Fc, opc);
         const bool fast = !__any(!(Fc <= kSpecFCos2 && opc <= kSpecOpFast));
-        // This lane's cell's (P, A, B) of the chosen form, read by the lanes
-        // of its 16-cell group.
+        // This lane's cell's (P, A, B) of the chosen form and the FP16 parts
+        // of P, read by the lanes of its 16-cell group.
         const float Pc = fast ? 2.f * opc : opc;
         const float Ac = fast ? fmaf(0.5f, Fc, 1.f) : 1.f;
         const float Bc = fast ? 0.5f * Fc : Fc;
+        const PhaseParts pp = phase_parts(Pc);
         v4f acc[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const float P = lane_read(Pc, 16 * g + m);
-            const float A = lane_read(Ac, 16 * g + m);
-            const float B = lane_read(Bc, 16 * g + m);
+            const int src = 16 * g + m;
+            const float P = lane_read(Pc, src);
+            const float A = lane_read(Ac, src);
+            const float B = lane_read(Bc, src);
+            u32x2 bop;
+            bop[0] = lane_read_u(pp.x0, src);
+            bop[1] = lane_read_u(pp.x1, src);
             acc[g] = v4f{S, S, S, S};
             if (fast)
-                spec_group<true, F16>(u, bh, bl, b_lo, P, A, B, acc[g]);
+                spec_group<true, F16>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
             else
-                spec_group<false, F16>(u, bh, bl, b_lo, P, A, B, acc[g]);
+                spec_group<false, F16>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
         }
         // acc[g][v] = D[4 q + v][m]: cell 64 sg + 16 g + 4 q + v, colour m.
         if (m < 3) {
@@ -695,7 +760,8 @@ extern "C" int sf_phase_difference(int32_t n, const float* opd_nm, const float* 
 // then the slot table of 1 / (2 lambda) (FP64 quotients, rounded).
 constexpr size_t kCmfFragsPerFormat = 2 * kSpecMfma * 64;
 constexpr size_t kCmfFrags = 2 * kCmfFragsPerFormat;
-constexpr size_t kCmfFloats = kCmfFrags * 4 + 4 * kSpecSlots;
+constexpr size_t kCmfPhaseOffset = kCmfFrags * 4 + 4 * kSpecSlots;   // in floats; 8-byte aligned
+constexpr size_t kCmfFloats = kCmfPhaseOffset + 2 * kSpecPhase * 64;
 
 static uint16_t bf16_rne(float x) {
     uint32_t u;
@@ -728,6 +794,7 @@ static float f16_value(uint16_t u) {
 struct PackedCmf {
     std::vector<u32x4> frag;
     float ulam[4 * kSpecSlots];
+    u32x2 pfrag[kSpecPhase * 64];
     float sum[2][3];          // [format][colour]
     int has_lo[2];
     bool f16_bound;           // 2^-12 sum_k |cmf[k][c]| / 81 <= kSpecF16Budget for every c
@@ -795,6 +862,20 @@ static PackedCmf pack_cmf(const float* cmf) {
             const int k = q + 4 * s;
             out.ulam[4 * s + q] = k < kSpecLambdas ? static_cast<float>(0.5 / (380.0 + 5.0 * k)) : 0.f;
         }
+    // Phase-MFMA A fragments: lane l of MFMA e holds row l % 4 of block l / 4,
+    // the wavelength k = l / 16 + 4 (4 e + l % 4), as [u0, u1, 2^-11 u0,
+    // 2^-11 u1] with U = 512 / lambda ~ u0 + 2^-11 u1 in FP16 parts.
+    for (int e = 0; e < kSpecPhase; ++e)
+        for (int lane = 0; lane < 64; ++lane) {
+            const int k = (lane >> 4) + 4 * (4 * e + (lane & 3));
+            const double U = 512.0 / (380.0 + 5.0 * k);
+            const double u0 = f16_value(f16_rne(static_cast<float>(U)));
+            const double u1 = f16_value(f16_rne(static_cast<float>((U - u0) * 2048.0)));
+            const uint16_t hv[4] = {f16_rne(static_cast<float>(u0)), f16_rne(static_cast<float>(u1)),
+                                    f16_rne(static_cast<float>(u0 / 2048.0)), f16_rne(static_cast<float>(u1 / 2048.0))};
+            out.pfrag[e * 64 + lane] = u32x2{hv[0] | (static_cast<uint32_t>(hv[1]) << 16),
+                                             hv[2] | (static_cast<uint32_t>(hv[3]) << 16)};
+        }
     return out;
 }
 
@@ -835,6 +916,8 @@ static void spectral_prepare(Drainage* d, const float* cmf) {
                               d->stream));
         SF_HIP(hipMemcpyAsync(d->cmf_dev + kCmfFrags * 4, packed.ulam, sizeof(packed.ulam), hipMemcpyHostToDevice,
                               d->stream));
+        SF_HIP(hipMemcpyAsync(d->cmf_dev + kCmfPhaseOffset, packed.pfrag, sizeof(packed.pfrag),
+                              hipMemcpyHostToDevice, d->stream));
         SF_HIP(hipStreamSynchronize(d->stream));
         for (int f = 0; f < 2; ++f) {
             for (int c = 0; c < 3; ++c) d->cmf_sum[f][c] = packed.sum[f][c];
@@ -869,6 +952,7 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     const int f = spectral_format(d);
     a.bfrag = reinterpret_cast<const u32x4*>(d->cmf_dev) + f * kCmfFragsPerFormat;
     a.ulam = d->cmf_dev + kCmfFrags * 4;
+    a.pfrag = reinterpret_cast<const u32x2*>(d->cmf_dev + kCmfPhaseOffset);
     a.s0 = d->cmf_sum[f][0];
     a.s1 = d->cmf_sum[f][1];
     a.s2 = d->cmf_sum[f][2];
