@@ -392,6 +392,17 @@ int sf_foam_step(sf_foam* foam, int32_t n_steps);
 int sf_foam_synchronize(sf_foam* foam);
 /* Replays 16-step HIP graphs for long step counts (default on). */
 int sf_foam_use_graph(sf_foam* foam, int32_t enable);
+/* Steps per stencil launch, 1 or 2 (temporal blocking: a launch reads and
+ * writes the foam's state once for two steps, bit-identical to two one-step
+ * launches).  Two need every bubble at least 4 x 8 cells; they are the
+ * default for foams above 2^23 cells.  A step count that is odd, or that
+ * starts at an odd step of the 8-step renormalisation cycle, takes one step
+ * per launch where needed.  In foams where two steps are possible, tile
+ * shapes whose last column chunk would be one full lane wide run at half the
+ * columns per lane (in both modes, which give identical results).  No counterpart
+ * in the reference (foam.rs keeps no per-bubble grids). */
+int sf_foam_set_steps_per_launch(sf_foam* foam, int32_t steps);
+int sf_foam_get_steps_per_launch(sf_foam* foam, int32_t* steps);
 /* Tile shapes: a wave streams a band of rows of one column chunk.
  * index:            0   1   2   3   4   5   6   7   8   9
  * columns per lane: 1   1   2   2   2   4   4   4   1   1
@@ -401,7 +412,7 @@ int sf_foam_use_graph(sf_foam* foam, int32_t enable);
  * foams above 2^23 cells) on each bubble-size bucket (n_phi <= 96, <= 192,
  * <= 384, > 384) for trial_steps steps on a copy of the state, keeps the
  * fastest per bucket and restores the state.  chosen[4] receives the shape
- * per bucket, trial_ms[4][10] the average step-kernel time of each trial (-1
+ * per bucket, trial_ms[4][10] the average kernel time per step of each trial (-1
  * for buckets without bubbles or shapes not tried); either may be NULL. */
 int sf_foam_autotune(sf_foam* foam, int32_t trial_steps, int32_t* chosen, double* trial_ms);
 /* Sets the tile shape index (0-9, as above) of each of the 4 buckets. */

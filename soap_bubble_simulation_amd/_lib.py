@@ -235,6 +235,8 @@ SIGNATURES = {
     "sf_foam_step": (C.c_int, [_vp, C.c_int32]),
     "sf_foam_synchronize": (C.c_int, [_vp]),
     "sf_foam_use_graph": (C.c_int, [_vp, C.c_int32]),
+    "sf_foam_set_steps_per_launch": (C.c_int, [_vp, C.c_int32]),
+    "sf_foam_get_steps_per_launch": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_foam_autotune": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32), _dp]),
     "sf_foam_set_tile_shapes": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sf_foam_bubble_dt": (C.c_int, [_vp, _dp]),

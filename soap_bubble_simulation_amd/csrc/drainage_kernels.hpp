@@ -102,16 +102,21 @@ struct StepArgs {
     double pole_mass_fixed;   // film volume of the pole rows when they are copied through
 };
 
-// Sum of the ring partials of one pole, divided by n_phi, as a float pair.
-__device__ inline float2 pole_mean(const float2* part, int ntiles, float np_f) {
-    float2 s = make_float2(0.f, 0.f);
-    for (int t = 0; t < ntiles; ++t) s = ff_add(s, part[t]);
+// A pole's summed ring partials divided by n_phi, as a float pair.
+__device__ inline float2 pole_mean_of_sum(float2 s, float np_f) {
     float q = s.x / np_f;
     float r = fmaf(-q, np_f, s.x);
     float ql = (r + s.y) / np_f;
     float h, l;
     fast_two_sum(q, ql, h, l);
     return make_float2(h, l);
+}
+
+// Sum of the ring partials of one pole, divided by n_phi, as a float pair.
+__device__ inline float2 pole_mean(const float2* part, int ntiles, float np_f) {
+    float2 s = make_float2(0.f, 0.f);
+    for (int t = 0; t < ntiles; ++t) s = ff_add(s, part[t]);
+    return pole_mean_of_sum(s, np_f);
 }
 
 __device__ inline float2 wave_ff_sum(float2 v) {

@@ -89,6 +89,9 @@ constexpr int kShapeBand[kShapes] = {16, 32, 16, 32, 64, 16, 32, 64, 4, 2};
 // the short-band shapes there, whose extra halo reads a bucket timed on its
 // own (latency-bound) does not show.
 constexpr int64_t kShortBandMaxCells = int64_t{1} << 23;
+// The same band with half the columns per lane (CPL 4 -> 2 -> 1; the 64-row
+// band of 2 columns per lane falls back to 32 rows at one column).
+constexpr int kShapeHalfCpl[kShapes] = {0, 1, 0, 1, 1, 2, 3, 4, 8, 9};
 constexpr int kPaths = 4;     // 1, 2, 4 (float4) and 4 (two float2) columns per lane
 
 // One row of a lane's columns plus its west and east neighbours.
@@ -192,6 +195,35 @@ __device__ inline float lane_value(float v, int t) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
 }
 
+// One cell of a drainage step: the new low word of h and the new Γ from the
+// cell (c) and its north, south, west and east neighbours (drainage.rs:298-392
+// with the Γ transport of drainage.wgsl:178-201).  G1, G2 and cS are the
+// row's coefficients (cot / 2 dtheta, 1 / (sin^2 dphi^2), dt rho g sin / 3 eta).
+__device__ __forceinline__ void foam_cell(const FoamBubble& B, float G1, float G2, float cS, float ch, float cl,
+                                          float cg, float nh, float nl, float ng, float sh, float sl, float sg,
+                                          float wh, float wl, float wg, float eh, float el, float eg, float& lo_new,
+                                          float& g_new) {
+    // Every fused multiply-add is written out and no other is formed, so the
+    // one- and two-step kernels, which inline this in different contexts,
+    // round the same operations.
+#pragma clang fp contract(off)
+    // Rupture mask on the input state (drainage.rs:331-333).
+    const bool rupt = ff_less(ch, cl, B.hc0, B.hc1, B.hc2);
+    const float dtp = (sh - ch) + (sl - cl);
+    const float dtm = (ch - nh) + (cl - nl);
+    const float dpp = (eh - ch) + (el - cl);
+    const float dpm = (ch - wh) + (cl - wl);
+    const float lap = fmaf(dpp - dpm, G2, fmaf(dtp + dtm, G1, (dtp - dtm) * B.G0));
+    const float H = ch + cl;
+    const float ln = fmaxf(cl + fmaf(B.dtD, lap, -cS * (H * H * H)), -ch);   // h >= 0
+    lo_new = rupt ? cl : ln;
+    const float gtp = sg - cg, gtm = cg - ng;
+    const float gpp = eg - cg, gpm = cg - wg;
+    const float glap = fmaf(gpp - gpm, G2, fmaf(gtp + gtm, G1, (gtp - gtm) * B.G0));
+    const float gn = fminf(fmaxf(fmaf(B.dtDs, glap, cg), 0.f), 1.f);  // drainage.wgsl:198-201
+    g_new = rupt ? cg : gn;
+}
+
 // Computes and stores row i from its north, centre and south rows.  The
 // pole rows are replaced in place: row 0 is reloaded after row 1, and row
 // nt - 2 is the last row any band computes.
@@ -215,24 +247,9 @@ __device__ inline void band_row(const FoamArgs& a, const FoamItem& it, const Foa
     }
     float lo_new[CPL], g_new[CPL];
 #pragma unroll
-    for (int v = 0; v < CPL; ++v) {
-        const float ch = C.h[v], cl = C.l[v], cg = C.g[v];
-        // Rupture mask on the input state (drainage.rs:331-333).
-        const bool rupt = ff_less(ch, cl, B.hc0, B.hc1, B.hc2);
-        const float dtp = (Sx.h[v] - ch) + (Sx.l[v] - cl);
-        const float dtm = (ch - N.h[v]) + (cl - N.l[v]);
-        const float dpp = (eh[v] - ch) + (el[v] - cl);
-        const float dpm = (ch - wh[v]) + (cl - wl[v]);
-        const float lap = (dtp - dtm) * B.G0 + (dtp + dtm) * G1 + (dpp - dpm) * G2;
-        const float H = ch + cl;
-        const float ln = fmaxf(cl + fmaf(B.dtD, lap, -cS * (H * H * H)), -ch);   // h >= 0
-        lo_new[v] = rupt ? cl : ln;
-        const float gtp = Sx.g[v] - cg, gtm = cg - N.g[v];
-        const float gpp = eg[v] - cg, gpm = cg - wg[v];
-        const float glap = (gtp - gtm) * B.G0 + (gtp + gtm) * G1 + (gpp - gpm) * G2;
-        const float gn = fminf(fmaxf(fmaf(B.dtDs, glap, cg), 0.f), 1.f);  // drainage.wgsl:198-201
-        g_new[v] = rupt ? cg : gn;
-    }
+    for (int v = 0; v < CPL; ++v)
+        foam_cell(B, G1, G2, cS, C.h[v], C.l[v], C.g[v], N.h[v], N.l[v], N.g[v], Sx.h[v], Sx.l[v], Sx.g[v], wh[v],
+                  wl[v], wg[v], eh[v], el[v], eg[v], lo_new[v], g_new[v]);
     // Ring partials of this chunk for the pole means of the next step
     // (drainage.rs:480-503), from the new ring values.
     if (i == 1) ring_partial<CPL>(C.h, lo_new, L.nvalid, lane, a.hpole_out + B.pole_off + it.chunk);
@@ -325,6 +342,324 @@ __global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a, const 
     else foam_band<4, false, REBASE>(a, it, B, lane);
 }
 
+// ---- two steps per launch (temporal blocking) -------------------------------
+//
+// foam_step2_kernel advances every bubble by two steps in one pass: a wave
+// streams the step-s rows r0 - 2 .. r1 + 1 of its item's columns, forms the
+// step-(s+1) rows r0 - 1 .. r1 into a three-row ring in its LDS (the chunk's
+// columns plus one halo column on each side), and forms the step-(s+2) rows
+// r0 .. r1 - 1 from that ring, so the foam's state is read and written once
+// per two steps.  Every cell update is foam_cell, the arithmetic of the
+// one-step kernel, and the ring partial sums and pole means are formed in
+// the same order, so a launch equals two launches of foam_step_kernel bit
+// for bit:
+//  * halo columns: lane 0 re-forms the chunk's west halo column (c0 - 1) and
+//    the chunk's last lane with a full set of columns its east halo column
+//    from the step-s columns one further out (the far columns); with a
+//    two-column tail lane (4 columns per lane, n_phi = 2 mod 4) the east halo
+//    is that lane's third column.  Halo columns wrap in phi, so a bubble that
+//    is a single chunk re-forms its own edge columns;
+//  * poles: a band touching a pole first re-forms that pole's ring row of step
+//    s+1 over every chunk of its bubble and takes the step-(s+1) pole mean
+//    from those partial sums, in chunk order.
+// Scope: the pole rows are the ring means of the previous step (virtual
+// poles), a rebase only follows the second step, and no chunk may have lane
+// 0 as its only full lane (the host picks tile shapes accordingly).
+template <int CPL>
+constexpr int foam_ring_width() { return 64 * CPL + 8; }   // [3 pad][west halo][columns][east halo][pad]
+
+template <int CPL>
+struct FRow2 {
+    float h[CPL], l[CPL], g[CPL];
+    float hx, lx, gx;     // column jx: lane 0 the west halo column, the others the column after their own
+};
+
+struct FFar {             // column jf of the centre row: one beyond the halo column
+    float h, l, g;
+};
+
+// Loads step-s row r (clamped into the grid) of the lane's columns and its
+// column jx.  Pole rows hold the ring mean of step s (virtual poles: ptop,
+// pbot); Γ poles are stored.
+template <int CPL, bool V4>
+__device__ inline void foam_load2(const FoamArgs& a, const FoamBubble& B, int r, int cc, int cc2, int jx,
+                                  float2 ptop, float2 pbot, FRow2<CPL>& R) {
+    const int rr = max(0, min(r, B.nt - 1));
+    const int64_t base = B.off + static_cast<int64_t>(rr) * B.np;
+    if (rr == 0 || rr == B.nt - 1) {
+        const float px = rr == 0 ? ptop.x : pbot.x;
+        const float py = rr == 0 ? ptop.y : pbot.y;
+#pragma unroll
+        for (int v = 0; v < CPL; ++v) {
+            R.h[v] = px;
+            R.l[v] = py;
+        }
+        R.hx = px;
+        R.lx = py;
+    } else {
+        load_vec<CPL, V4>(a.hi_in + base, cc, cc2, R.h);
+        load_vec<CPL, V4>(a.lo_in + base, cc, cc2, R.l);
+        R.hx = a.hi_in[base + jx];
+        R.lx = a.lo_in[base + jx];
+    }
+    load_vec<CPL, V4>(a.g_in + base, cc, cc2, R.g);
+    R.gx = a.g_in[base + jx];
+}
+
+__device__ inline void foam_far2(const FoamArgs& a, const FoamBubble& B, int r, int jf, FFar& F) {
+    const int rr = max(0, min(r, B.nt - 1));
+    const int64_t o = B.off + static_cast<int64_t>(rr) * B.np + jf;
+    F.h = a.hi_in[o];
+    F.l = a.lo_in[o];
+    F.g = a.g_in[o];
+}
+
+__device__ __forceinline__ float lane_value_f(float v, int t) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
+}
+
+// The step-(s+1) mean of pole ring row `ring` (1 or nt - 2) of bubble B: the
+// ring row is formed over every chunk as foam_step_kernel forms it, each
+// chunk's float-float sum reduced over the wave as ring_partial does, and
+// the chunk sums added in chunk order as pole_mean adds the partials.
+template <int CPL, bool V4>
+__device__ __forceinline__ float2 foam_pole_after_step(const FoamArgs& a, const FoamBubble& B, int ring, float2 P0, int lane) {
+    const int np = B.np, nt = B.nt;
+    const FoamRow rw = a.rows[B.row_off + ring];
+    const float cS = B.dtc * rw.sin_t;
+    float2 total = make_float2(0.f, 0.f);
+    for (int ch = 0; ch < B.nchunks; ++ch) {
+        const int c = ch * 64 * CPL + CPL * lane;
+        const bool valid = c < np;
+        const int cc = valid ? c : 0;
+        const int jw = cc == 0 ? np - 1 : cc - 1;
+        const int je = (cc + CPL) % np;
+        const bool full = cc + CPL <= np;
+        const int cc2 = full ? cc + 2 : cc + 2 - np;
+        const int nvalid = valid ? (full ? CPL : 2) : 0;
+        FRow<CPL> N, C, S;
+        foam_load<CPL, V4>(a, B.off, np, nt, ring - 1, cc, cc2, jw, je, N);
+        foam_load<CPL, V4>(a, B.off, np, nt, ring, cc, cc2, jw, je, C);
+        foam_load<CPL, V4>(a, B.off, np, nt, ring + 1, cc, cc2, jw, je, S);
+        if (ring == 1) foam_pole_row<CPL>(N, P0);
+        if (ring == nt - 2) foam_pole_row<CPL>(S, P0);
+        float wh[CPL], wl[CPL], wg[CPL], eh[CPL], el[CPL], eg[CPL];
+        wh[0] = C.hw; wl[0] = C.lw; wg[0] = C.gw;
+        eh[CPL - 1] = C.he; el[CPL - 1] = C.le; eg[CPL - 1] = C.ge;
+#pragma unroll
+        for (int v = 1; v < CPL; ++v) {
+            wh[v] = C.h[v - 1]; wl[v] = C.l[v - 1]; wg[v] = C.g[v - 1];
+            eh[v - 1] = C.h[v]; el[v - 1] = C.l[v]; eg[v - 1] = C.g[v];
+        }
+        float lo1[CPL], g1[CPL];
+#pragma unroll
+        for (int v = 0; v < CPL; ++v)
+            foam_cell(B, rw.G1, rw.G2, cS, C.h[v], C.l[v], C.g[v], N.h[v], N.l[v], N.g[v], S.h[v], S.l[v], S.g[v],
+                      wh[v], wl[v], wg[v], eh[v], el[v], eg[v], lo1[v], g1[v]);
+        float2 sum = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int v = 0; v < CPL; ++v) {
+            if (v < nvalid) {
+                float a_, e_;
+                two_sum(C.h[v], lo1[v], a_, e_);
+                sum = ff_add(sum, make_float2(a_, e_));
+            }
+        }
+        sum = wave_ff_sum(sum);
+        total = ff_add(total, make_float2(lane_value_f(sum.x, 0), lane_value_f(sum.y, 0)));
+    }
+    return pole_mean_of_sum(total, B.np_f);
+}
+
+template <int CPL, bool V4, bool REBASE>
+__device__ __forceinline__ void foam_band2(const FoamArgs& a, const FoamItem& it, const FoamBubble& B, int lane, float* ring) {
+    constexpr int W = foam_ring_width<CPL>();
+    const int np = B.np, nt = B.nt;
+    const int r0 = it.r0, r1 = it.r1;
+    const int c = it.c0 + CPL * lane;
+    const bool valid = c < np;
+    const int cc = valid ? c : 0;                      // idle lanes shadow column 0
+    const bool full = cc + CPL <= np;                  // false only for a 2-column tail lane
+    const int cc2 = full ? cc + 2 : cc + 2 - np;
+    const int nvalid = valid ? (full ? CPL : 2) : 0;
+    const int ncols = min(64 * CPL, np - it.c0);       // columns of the chunk inside the bubble
+    const int last = (ncols - 1) / CPL;                // the chunk's last lane
+    const bool edge = lane == last && ncols % CPL == 0;   // forms the east halo column from jx
+    const int jw = it.c0 == 0 ? np - 1 : it.c0 - 1;
+    const int jx = lane == 0 ? jw : (cc + CPL) % np;   // halo column (lane 0 west, the edge lane east)
+    const int jf = lane == 0 ? (jw == 0 ? np - 1 : jw - 1) : (cc + CPL + 1) % np;   // far column
+
+    // Pole means of step s (p0) and of step s+1 (p1).
+    float2 p0top = make_float2(0.f, 0.f), p0bot = make_float2(0.f, 0.f);
+    float2 p1top = make_float2(0.f, 0.f), p1bot = make_float2(0.f, 0.f);
+    if (r0 <= 2) p0top = pole_mean(a.hpole_in + B.pole_off, B.nchunks, B.np_f);
+    if (r1 >= nt - 2) p0bot = pole_mean(a.hpole_in + B.pole_off + B.nchunks, B.nchunks, B.np_f);
+    if (r0 == 1) p1top = foam_pole_after_step<CPL, V4>(a, B, 1, p0top, lane);
+    if (r1 == nt - 1) p1bot = foam_pole_after_step<CPL, V4>(a, B, nt - 2, p0bot, lane);
+
+    // Row coefficients of rows r0 - 1 .. r1 (at most 66), read with readlane.
+    const int rb = r0 - 1;
+    const FoamRow rl0 = a.rows[B.row_off + min(rb + lane, nt - 1)];
+    const FoamRow rl1 = a.rows[B.row_off + min(rb + 64 + lane, nt - 1)];
+    auto row_coef = [&](int k, float& G1, float& G2, float& sn) __attribute__((always_inline)) {
+        const int t = k - rb;                          // wave-uniform
+        if (t < 64) {
+            G1 = lane_value_f(rl0.G1, t); G2 = lane_value_f(rl0.G2, t); sn = lane_value_f(rl0.sin_t, t);
+        } else {
+            G1 = lane_value_f(rl1.G1, t - 64); G2 = lane_value_f(rl1.G2, t - 64); sn = lane_value_f(rl1.sin_t, t - 64);
+        }
+    };
+    auto slot = [&](int k, int arr) __attribute__((always_inline)) { return ring + ((k % 3) * 3 + arr) * W; };
+    const int lc = 4 + CPL * lane;                     // the lane's first column in a ring row
+
+    // Step s+1 row k from step-s rows (N, C, S) into the ring.
+    auto first_step = [&](int k, const FRow2<CPL>& N, const FRow2<CPL>& C, const FRow2<CPL>& S,
+                          const FFar& F) __attribute__((always_inline)) {
+        float G1, G2, sn;
+        row_coef(k, G1, G2, sn);
+        const float cS = B.dtc * sn;
+        float wh[CPL], wl[CPL], wg[CPL], eh[CPL], el[CPL], eg[CPL];
+        wh[0] = from_west(C.h[CPL - 1], C.hx);
+        wl[0] = from_west(C.l[CPL - 1], C.lx);
+        wg[0] = from_west(C.g[CPL - 1], C.gx);
+        const float xeh = from_east(C.h[0], C.hx), xel = from_east(C.l[0], C.lx), xeg = from_east(C.g[0], C.gx);
+        eh[CPL - 1] = edge ? C.hx : xeh;
+        el[CPL - 1] = edge ? C.lx : xel;
+        eg[CPL - 1] = edge ? C.gx : xeg;
+#pragma unroll
+        for (int v = 1; v < CPL; ++v) {
+            wh[v] = C.h[v - 1]; wl[v] = C.l[v - 1]; wg[v] = C.g[v - 1];
+            eh[v - 1] = C.h[v]; el[v - 1] = C.l[v]; eg[v - 1] = C.g[v];
+        }
+        float lo1[CPL], g1[CPL];
+#pragma unroll
+        for (int v = 0; v < CPL; ++v)
+            foam_cell(B, G1, G2, cS, C.h[v], C.l[v], C.g[v], N.h[v], N.l[v], N.g[v], S.h[v], S.l[v], S.g[v], wh[v],
+                      wl[v], wg[v], eh[v], el[v], eg[v], lo1[v], g1[v]);
+        // The halo cell: lane 0 the west halo column (west of it the far
+        // column, east of it the lane's first column), the edge lane the
+        // east halo column (west of it the lane's last column, east of it the
+        // far column).
+        const bool l0 = lane == 0;
+        float xlo, xg;
+        foam_cell(B, G1, G2, cS, C.hx, C.lx, C.gx, N.hx, N.lx, N.gx, S.hx, S.lx, S.gx,
+                  l0 ? F.h : C.h[CPL - 1], l0 ? F.l : C.l[CPL - 1], l0 ? F.g : C.g[CPL - 1],
+                  l0 ? C.h[0] : F.h, l0 ? C.l[0] : F.l, l0 ? C.g[0] : F.g, xlo, xg);
+        // A pole row of step s+1 is not stepped: h and its low word are the
+        // ring mean of step s+1, Γ is the stored pole value.
+        const bool pole = k == 0 || k == nt - 1;
+        const float px = k == 0 ? p1top.x : p1bot.x, py = k == 0 ? p1top.y : p1bot.y;
+        float oh[CPL], ol[CPL], og[CPL];
+#pragma unroll
+        for (int v = 0; v < CPL; ++v) {
+            oh[v] = pole ? px : C.h[v];
+            ol[v] = pole ? py : lo1[v];
+            og[v] = pole ? C.g[v] : g1[v];
+        }
+        const float xoh = pole ? px : C.hx, xol = pole ? py : xlo, xog = pole ? C.gx : xg;
+        float* rh = slot(k, 0);
+        float* rl = slot(k, 1);
+        float* rg = slot(k, 2);
+        if (valid) {
+#pragma unroll
+            for (int v = 0; v < CPL; ++v) {
+                rh[lc + v] = oh[v];
+                rl[lc + v] = ol[v];
+                rg[lc + v] = og[v];
+            }
+        }
+        if (l0) { rh[3] = xoh; rl[3] = xol; rg[3] = xog; }
+        if (edge) { rh[4 + ncols] = xoh; rl[4 + ncols] = xol; rg[4 + ncols] = xog; }
+    };
+
+    // Step s+2 row i from ring rows i - 1, i, i + 1.
+    auto second_step = [&](int i) __attribute__((always_inline)) {
+        const float* nh_ = slot(i - 1, 0); const float* nl_ = slot(i - 1, 1); const float* ng_ = slot(i - 1, 2);
+        const float* ch_ = slot(i, 0);     const float* cl_ = slot(i, 1);     const float* cg_ = slot(i, 2);
+        const float* sh_ = slot(i + 1, 0); const float* sl_ = slot(i + 1, 1); const float* sg_ = slot(i + 1, 2);
+        float G1, G2, sn;
+        row_coef(i, G1, G2, sn);
+        const float cS = B.dtc * sn;
+        float lo2[CPL], g2[CPL], chh[CPL];
+#pragma unroll
+        for (int v = 0; v < CPL; ++v) {
+            const int p = lc + v;
+            chh[v] = ch_[p];
+            foam_cell(B, G1, G2, cS, ch_[p], cl_[p], cg_[p], nh_[p], nl_[p], ng_[p], sh_[p], sl_[p], sg_[p],
+                      ch_[p - 1], cl_[p - 1], cg_[p - 1], ch_[p + 1], cl_[p + 1], cg_[p + 1], lo2[v], g2[v]);
+        }
+        // Ring partials of step s+2 for the next launch's pole means.
+        if (i == 1) ring_partial<CPL>(chh, lo2, nvalid, lane, a.hpole_out + B.pole_off + it.chunk);
+        if (i == nt - 2) ring_partial<CPL>(chh, lo2, nvalid, lane, a.hpole_out + B.pole_off + B.nchunks + it.chunk);
+        if (valid) {
+            const int64_t o = B.off + static_cast<int64_t>(i) * np;
+            if (REBASE) {
+                float hh[CPL], ll[CPL];
+#pragma unroll
+                for (int v = 0; v < CPL; ++v) two_sum(chh[v], lo2[v], hh[v], ll[v]);
+                store_vec<CPL, V4>(a.hi_out + o, cc, full, hh);
+                store_vec<CPL, V4>(a.lo_out + o, cc, full, ll);
+            } else {
+                store_vec<CPL, V4>(a.lo_out + o, cc, full, lo2);
+            }
+            store_vec<CPL, V4>(a.g_out + o, cc, full, g2);
+        }
+    };
+
+    // Step-s rows k - 1, k, k + 1 form step s+1 row k (k = r0 - 1 .. r1);
+    // after row k is in the ring, step s+2 row k - 1 follows.  Four row
+    // buffers (one row in flight) unrolled over their rotations, so no buffer
+    // is copied while its load is outstanding; the far column of the centre
+    // row comes one row ahead in two more.
+    FRow2<CPL> R0, R1, R2, R3;
+    FFar F0, F1;
+    const int ks = r0 - 1;
+    foam_load2<CPL, V4>(a, B, ks - 1, cc, cc2, jx, p0top, p0bot, R0);
+    foam_load2<CPL, V4>(a, B, ks, cc, cc2, jx, p0top, p0bot, R1);
+    foam_load2<CPL, V4>(a, B, ks + 1, cc, cc2, jx, p0top, p0bot, R2);
+    foam_load2<CPL, V4>(a, B, ks + 2, cc, cc2, jx, p0top, p0bot, R3);
+    foam_far2(a, B, ks, jf, F0);
+#define SF_FOAM2_ROW(Nb, Cb, Sb, Fc, Fn)                                   \
+    {                                                                      \
+        if (k > r1) break;                                                 \
+        if (k + 1 <= r1) foam_far2(a, B, k + 1, jf, Fn);                 \
+        first_step(k, Nb, Cb, Sb, Fc);                                     \
+        __builtin_amdgcn_wave_barrier();                                   \
+        if (k - 1 >= r0) second_step(k - 1);                               \
+        __builtin_amdgcn_wave_barrier();                                   \
+        if (k + 3 <= r1 + 1) foam_load2<CPL, V4>(a, B, k + 3, cc, cc2, jx, p0top, p0bot, Nb); \
+        ++k;                                                               \
+    }
+    for (int k = ks;;) {
+        SF_FOAM2_ROW(R0, R1, R2, F0, F1)
+        SF_FOAM2_ROW(R1, R2, R3, F1, F0)
+        SF_FOAM2_ROW(R2, R3, R0, F0, F1)
+        SF_FOAM2_ROW(R3, R0, R1, F1, F0)
+    }
+#undef SF_FOAM2_ROW
+}
+
+// One two-step work item per wave; the ring of each wave is its own slice of
+// the block's LDS.
+template <int PATH, bool REBASE>
+__global__ __launch_bounds__(256) void foam_step2_kernel(const FoamArgs a, const FoamItem* items, int n) {
+    constexpr int CPL = PATH == 0 ? 1 : (PATH == 1 ? 2 : 4);
+    constexpr int W = foam_ring_width<CPL>();
+    __shared__ __align__(16) float ring[4][3 * 3 * W];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    const int v = (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
+    const int k = __builtin_amdgcn_readfirstlane(v * 4 + wave);
+    if (k >= n) return;
+    const FoamItem it = items[k];
+    const FoamBubble B = a.bub[it.bubble];
+    if constexpr (PATH == 0) foam_band2<1, false, REBASE>(a, it, B, lane, ring[wave]);
+    else if constexpr (PATH == 1) foam_band2<2, false, REBASE>(a, it, B, lane, ring[wave]);
+    else if constexpr (PATH == 2) foam_band2<4, true, REBASE>(a, it, B, lane, ring[wave]);
+    else foam_band2<4, false, REBASE>(a, it, B, lane, ring[wave]);
+}
+
 // Writes every bubble's pole rows from the ring partials (one block per bubble).
 __global__ void foam_poles_kernel(float* hi, float* lo, const float2* hpole, const FoamBubble* bub) {
     const FoamBubble B = bub[blockIdx.x];
@@ -410,6 +745,14 @@ unsigned cdivu(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / 
 
 int bucket_of(int np) { return np <= 96 ? 0 : (np <= 192 ? 1 : (np <= 384 ? 2 : 3)); }
 
+// Tile shape of a bubble with two steps per launch: a shape whose last column
+// chunk is exactly one lane wide would ask lane 0 for both halo columns, so
+// it gives way to the same band at half the columns per lane.
+int two_step_shape(int v, int np) {
+    while (kShapeCpl[v] >= 2 && np % (64 * kShapeCpl[v]) == kShapeCpl[v]) v = kShapeHalfCpl[v];
+    return v;
+}
+
 }  // namespace
 
 struct Foam {
@@ -429,6 +772,10 @@ struct Foam {
     float* g[2] = {nullptr, nullptr};
     int cur = 0, cur_hi = 0, cur_pole = 0, since_rebase = 0;
     bool poles_virtual = false;
+    // Two steps per launch (foam_step2_kernel): requested, and possible for
+    // this foam (every bubble at least 4 x 8).
+    int steps_per_launch = 2;
+    bool two_ok = false;
     float2* hpole[2] = {nullptr, nullptr};
     int pole_slots = 0;
     FoamRow* rows = nullptr;
@@ -465,9 +812,15 @@ struct Foam {
     }
 
     // Tile shape, chunking and pole-partial slots of bubble b.
+    bool two_step() const { return steps_per_launch == 2 && two_ok; }
+
     void shape_bubble(int b) {
         FoamBubble& B = bub[b];
         B.variant = variant_of_bucket[bucket_of(B.np)];
+        // Applied whenever two steps per launch are possible, so that both
+        // launch modes share one layout (and one summation order of the pole
+        // ring partials).
+        if (two_ok) B.variant = two_step_shape(B.variant, B.np);
         const int cpl = kShapeCpl[B.variant];
         B.path = cpl == 1 ? 0 : (cpl == 2 ? 1 : (B.np % 4 == 0 ? 2 : 3));
         B.nchunks = static_cast<int>(cdivu(B.np, 64 * cpl));
@@ -564,15 +917,35 @@ struct Foam {
             hipExtLaunchKernelGGL((foam_step_kernel<PATH, false>), grid, blk, 0, stream, t0, t1, 0, a, items, n);
     }
 
-    // One step: a launch per code path in use; t0 is recorded at the start
-    // of the first kernel and t1 at the end of the last.
-    void launch_one(bool rebase, const FoamArgs& a, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+    template <int PATH>
+    void launch_path2(bool rebase, const FoamArgs& a, hipEvent_t t0, hipEvent_t t1) {
+        const int n = path_begin[PATH + 1] - path_begin[PATH];
+        if (n == 0) return;
+        const dim3 grid((cdivu(static_cast<size_t>(n), 4) + kXcds - 1) / kXcds * kXcds), blk(256);
+        const FoamItem* items = d_items + path_begin[PATH];
+        if (rebase)
+            hipExtLaunchKernelGGL((foam_step2_kernel<PATH, true>), grid, blk, 0, stream, t0, t1, 0, a, items, n);
+        else
+            hipExtLaunchKernelGGL((foam_step2_kernel<PATH, false>), grid, blk, 0, stream, t0, t1, 0, a, items, n);
+    }
+
+    // One step (two with two_steps): a launch per code path in use; t0 is
+    // recorded at the start of the first kernel and t1 at the end of the last.
+    void launch_one(bool rebase, const FoamArgs& a, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
+                    bool two_steps = false) {
         int first = -1, last = -1;
         for (int p = 0; p < kPaths; ++p)
             if (path_begin[p + 1] > path_begin[p]) {
                 if (first < 0) first = p;
                 last = p;
             }
+        if (two_steps) {
+            launch_path2<0>(rebase, a, first == 0 ? t0 : nullptr, last == 0 ? t1 : nullptr);
+            launch_path2<1>(rebase, a, first == 1 ? t0 : nullptr, last == 1 ? t1 : nullptr);
+            launch_path2<2>(rebase, a, first == 2 ? t0 : nullptr, last == 2 ? t1 : nullptr);
+            launch_path2<3>(rebase, a, first == 3 ? t0 : nullptr, last == 3 ? t1 : nullptr);
+            return;
+        }
         launch_path<0>(rebase, a, first == 0 ? t0 : nullptr, last == 0 ? t1 : nullptr);
         launch_path<1>(rebase, a, first == 1 ? t0 : nullptr, last == 1 ? t1 : nullptr);
         launch_path<2>(rebase, a, first == 2 ? t0 : nullptr, last == 2 ? t1 : nullptr);
@@ -593,6 +966,24 @@ struct Foam {
         advance(rebase);
     }
 
+    // Two steps in one launch, from virtual poles at an even rebase phase
+    // (a rebase then only follows the second step).
+    void step_two_direct() {
+        const bool rebase = since_rebase + 2 >= 8;
+        const FoamArgs a = args_for(cur, cur_hi, cur_pole, true);
+        if (timer.enabled) {
+            auto& ev = timer.next(stream);
+            launch_one(rebase, a, ev.first, ev.second, true);
+        } else {
+            launch_one(rebase, a, nullptr, nullptr, true);
+        }
+        SF_HIP(hipGetLastError());
+        advance(false);
+        advance(rebase);
+        cur = 1 - cur;              // the launch flips the ping-pong buffers once
+        cur_pole = 1 - cur_pole;
+    }
+
     void advance(bool rebase) {
         cur = 1 - cur;
         cur_pole = 1 - cur_pole;
@@ -608,7 +999,7 @@ struct Foam {
             if (active[b]) time[b] += static_cast<double>(bub[b].dt);
     }
 
-    int phase() const { return (cur_hi << 4) | (cur << 3) | since_rebase; }
+    int phase() const { return (two_step() ? 32 : 0) | (cur_hi << 4) | (cur << 3) | since_rebase; }
 
     // Captures kFoamPeriod steps from the current phase into a graph.
     void capture() {
@@ -616,12 +1007,13 @@ struct Foam {
         hipGraph_t gr = nullptr;
         SF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int c = cur, ch = cur_hi, cp = cur_pole, sr = since_rebase;
-        for (int s = 0; s < kFoamPeriod; ++s) {
-            const bool rebase = sr + 1 >= 8;
-            launch_one(rebase, args_for(c, ch, cp, true));
+        const int per_launch = two_step() ? 2 : 1;
+        for (int s = 0; s < kFoamPeriod; s += per_launch) {
+            const bool rebase = sr + per_launch >= 8;
+            launch_one(rebase, args_for(c, ch, cp, true), nullptr, nullptr, per_launch == 2);
             c = 1 - c;
             cp = 1 - cp;
-            if (rebase) { ch = 1 - ch; sr = 0; } else { ++sr; }
+            if (rebase) { ch = 1 - ch; sr = 0; } else { sr += per_launch; }
         }
         hipError_t e = hipStreamEndCapture(stream, &gr);
         if (e == hipSuccess) e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
@@ -636,13 +1028,28 @@ struct Foam {
             step_direct();
             ++done;
         }
+        const bool two = two_step();
+        if (two && (since_rebase & 1) && done < n) {   // pairs start at an even rebase phase
+            step_direct();
+            ++done;
+        }
         if (use_graph && !timer.enabled) {
             while (n - done >= kFoamPeriod) {
                 if (graph == nullptr || graph_base != phase()) capture();
                 SF_HIP(hipGraphLaunch(graph, stream));
-                for (int s = 0; s < kFoamPeriod; ++s) advance(since_rebase + 1 >= 8);
+                for (int s = 0; s < kFoamPeriod; ++s) {
+                    advance(since_rebase + 1 >= 8);
+                    if (two && (s & 1)) {            // one ping-pong flip per launch
+                        cur = 1 - cur;
+                        cur_pole = 1 - cur_pole;
+                    }
+                }
                 done += kFoamPeriod;
             }
+        }
+        while (two && n - done >= 2) {
+            step_two_direct();
+            done += 2;
         }
         while (done < n) {
             step_direct();
@@ -799,12 +1206,20 @@ struct Foam {
                 layout_items(bk);
                 poles_virtual = false;      // first trial step reads the stored pole rows
                 step_direct();              // untimed warm-up
+                while (two_step() && (since_rebase & 1)) step_direct();
                 timer.enabled = true;
                 timer.ms = 0.0;
                 timer.launches = 0;
-                for (int s = 0; s < trial_steps; ++s) step_direct();
+                const int per_launch = two_step() ? 2 : 1;
+                for (int s = 0; s < trial_steps; ++s) {
+                    if (per_launch == 2)
+                        step_two_direct();
+                    else
+                        step_direct();
+                }
                 timer.drain(stream);
-                const double ms = timer.launches ? timer.ms / timer.launches : INFINITY;
+                // ms per step of the trial's launches
+                const double ms = timer.launches ? timer.ms / timer.launches / per_launch : INFINITY;
                 timer.enabled = false;
                 if (trial_ms) trial_ms[bk * kShapes + v] = ms;
                 if (ms < best) {
@@ -937,6 +1352,12 @@ extern "C" int sf_foam_create(const sf_params* params, int32_t n, const sf_bubbl
             }
             f->time.assign(n, 0.0);
             f->active.assign(n, 1);
+            f->two_ok = true;
+            for (int b = 0; b < n; ++b) f->two_ok &= bubbles[b].n_theta >= 4 && bubbles[b].n_phi >= 8;
+            // Two steps per launch by default where the foam is bandwidth-bound;
+            // smaller foams are latency-bound and their recomputed halo rows
+            // would outweigh the saved traffic.
+            f->steps_per_launch = f->total > kShortBandMaxCells ? 2 : 1;
             // Small foams are latency-bound: short bands give them waves
             // (one 256 x 512 bubble: 4.4 us per step with 2-row bands against
             // 22.7 us with the 32-row default; 16 such bubbles: 11.1 against
@@ -1040,6 +1461,26 @@ extern "C" int sf_foam_use_graph(sf_foam* f, int32_t enable) {
         SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
         F(f)->use_graph = enable != 0;
         if (!enable) F(f)->drop_graph();
+    });
+}
+
+extern "C" int sf_foam_set_steps_per_launch(sf_foam* f, int32_t steps) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr, SF_ERR_INVALID, "foam is NULL");
+        SF_REQUIRE(steps == 1 || steps == 2, SF_ERR_INVALID, "steps per launch must be 1 or 2 (got %d)", steps);
+        Foam* fm = F(f);
+        SF_REQUIRE(steps == 1 || fm->two_ok, SF_ERR_INVALID,
+                   "two steps per launch need every bubble at least 4 x 8 cells");
+        if (steps == fm->steps_per_launch) return;
+        fm->steps_per_launch = steps;
+        fm->drop_graph();
+    });
+}
+
+extern "C" int sf_foam_get_steps_per_launch(sf_foam* f, int32_t* steps) {
+    return guard([&] {
+        SF_REQUIRE(f != nullptr && steps != nullptr, SF_ERR_INVALID, "NULL argument");
+        *steps = F(f)->two_step() ? 2 : 1;
     });
 }
 

@@ -128,6 +128,16 @@ class FoamDrainage:
     def use_graph(self, enable: bool) -> None:
         L.check(self._lib.sf_foam_use_graph(self._h, 1 if enable else 0))
 
+    def set_steps_per_launch(self, steps: int) -> None:
+        """1 or 2 steps per stencil launch (two: the state is streamed once
+        per two steps, bit-identical to one-step launches)."""
+        L.check(self._lib.sf_foam_set_steps_per_launch(self._h, int(steps)))
+
+    def steps_per_launch(self) -> int:
+        k = C.c_int32()
+        L.check(self._lib.sf_foam_get_steps_per_launch(self._h, C.byref(k)))
+        return k.value
+
     def autotune(self, trial_steps: int = 8, return_times: bool = False):
         """Picks the fastest tile shape per size bucket; optionally also
         returns the trial kernel times [bucket][shape] in ms (-1: empty)."""
