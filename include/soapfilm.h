@@ -397,9 +397,9 @@ int sf_foam_use_graph(sf_foam* foam, int32_t enable);
  * launches).  Two need every bubble at least 4 x 8 cells; they are the
  * default for foams above 2^23 cells.  A step count that is odd, or that
  * starts at an odd step of the 8-step renormalisation cycle, takes one step
- * per launch where needed.  In foams where two steps are possible, tile
- * shapes whose last column chunk would be one full lane wide run at half the
- * columns per lane (in both modes, which give identical results).  No counterpart
+ * per launch where needed.  In foams where two steps are possible a column
+ * chunk holds 62 lanes' columns in both modes (lanes 0 and 63 form the
+ * two-step kernel's halo columns), so both give identical results.  No counterpart
  * in the reference (foam.rs keeps no per-bubble grids). */
 int sf_foam_set_steps_per_launch(sf_foam* foam, int32_t steps);
 int sf_foam_get_steps_per_launch(sf_foam* foam, int32_t* steps);

@@ -45,6 +45,8 @@ struct FoamBubble {
     int variant;          // tile shape index (kShapeCpl, kShapeBand)
     int path;             // 0: 1 column per lane, 1: 2, 2: 4 (float4), 3: 4 (two float2)
     int active;           // stepped (1) or held (0), see Foam::set_active
+    int lane0;            // lane of a chunk's first column: 1 where lanes 0 and 63 are halo lanes (two-step layout)
+    int cw;               // columns per chunk: (64 - 2 lane0) x columns per lane
     float dt;             // s
     float dtD;            // dt D_h / R^2
     float dtDs;           // dt D_s / R^2
@@ -89,9 +91,6 @@ constexpr int kShapeBand[kShapes] = {16, 32, 16, 32, 64, 16, 32, 64, 4, 2};
 // the short-band shapes there, whose extra halo reads a bucket timed on its
 // own (latency-bound) does not show.
 constexpr int64_t kShortBandMaxCells = int64_t{1} << 23;
-// The same band with half the columns per lane (CPL 4 -> 2 -> 1; the 64-row
-// band of 2 columns per lane falls back to 32 rows at one column).
-constexpr int kShapeHalfCpl[kShapes] = {0, 1, 0, 1, 1, 2, 3, 4, 8, 9};
 constexpr int kPaths = 4;     // 1, 2, 4 (float4) and 4 (two float2) columns per lane
 
 // One row of a lane's columns plus its west and east neighbours.
@@ -278,8 +277,8 @@ template <int CPL, bool V4, bool REBASE>
 __device__ void foam_band(const FoamArgs& a, const FoamItem& it, const FoamBubble& B, int lane) {
     const int np = B.np, nt = B.nt;
     BandLane L;
-    const int c = it.c0 + CPL * lane;
-    L.valid = c < np;
+    const int c = it.c0 + CPL * (lane - B.lane0);      // lanes lane0 .. 63 - lane0 hold the chunk
+    L.valid = lane >= B.lane0 && lane < 64 - B.lane0 && c < np;
     L.cc = L.valid ? c : 0;                            // idle lanes shadow column 0
     L.jw = L.cc == 0 ? np - 1 : L.cc - 1;
     L.je = (L.cc + CPL) % np;
@@ -346,36 +345,31 @@ __global__ __launch_bounds__(256) void foam_step_kernel(const FoamArgs a, const 
 //
 // foam_step2_kernel advances every bubble by two steps in one pass: a wave
 // streams the step-s rows r0 - 2 .. r1 + 1 of its item's columns, forms the
-// step-(s+1) rows r0 - 1 .. r1 into a three-row ring in its LDS (the chunk's
-// columns plus one halo column on each side), and forms the step-(s+2) rows
-// r0 .. r1 - 1 from that ring, so the foam's state is read and written once
-// per two steps.  Every cell update is foam_cell, the arithmetic of the
-// one-step kernel, and the ring partial sums and pole means are formed in
-// the same order, so a launch equals two launches of foam_step_kernel bit
-// for bit:
-//  * halo columns: lane 0 re-forms the chunk's west halo column (c0 - 1) and
-//    the chunk's last lane with a full set of columns its east halo column
-//    from the step-s columns one further out (the far columns); with a
-//    two-column tail lane (4 columns per lane, n_phi = 2 mod 4) the east halo
-//    is that lane's third column.  Halo columns wrap in phi, so a bubble that
-//    is a single chunk re-forms its own edge columns;
+// step-(s+1) rows r0 - 1 .. r1 into a three-row ring in its LDS, and forms the
+// step-(s+2) rows r0 .. r1 - 1 from that ring, so the foam's state is read
+// and written once per two steps.  Every cell update is foam_cell, the
+// arithmetic of the one-step kernel, and the ring partial sums and pole means
+// are formed in the same order, so a launch equals two launches of
+// foam_step_kernel bit for bit:
+//  * halo columns: with the two-step layout (FoamBubble::lane0 = 1) a chunk's
+//    columns sit in lanes 1 .. nl, so lane 0 holds the CPL columns before the
+//    chunk and lane nl + 1 those after it (both wrap in phi; a bubble that is
+//    a single chunk thus re-forms its own edge columns).  Every lane forms
+//    step s+1 of its columns, so the halo columns cost two lanes of 64, not an
+//    extra cell per lane.  With a two-column tail lane (4 columns per lane,
+//    n_phi = 2 mod 4) the east halo is that lane's third column;
 //  * poles: a band touching a pole first re-forms that pole's ring row of step
 //    s+1 over every chunk of its bubble and takes the step-(s+1) pole mean
 //    from those partial sums, in chunk order.
 // Scope: the pole rows are the ring means of the previous step (virtual
-// poles), a rebase only follows the second step, and no chunk may have lane
-// 0 as its only full lane (the host picks tile shapes accordingly).
+// poles), and a rebase only follows the second step.
 template <int CPL>
-constexpr int foam_ring_width() { return 64 * CPL + 8; }   // [3 pad][west halo][columns][east halo][pad]
+constexpr int foam_ring_width() { return 64 * CPL + 4; }   // lane L's columns at CPL L .. CPL L + CPL - 1
 
 template <int CPL>
 struct FRow2 {
     float h[CPL], l[CPL], g[CPL];
-    float hx, lx, gx;     // column jx: lane 0 the west halo column, the others the column after their own
-};
-
-struct FFar {             // column jf of the centre row: one beyond the halo column
-    float h, l, g;
+    float hx, lx, gx;     // column jx: lane 0 the one west of its columns, lane nl + 1 the one east
 };
 
 // Loads step-s row r (clamped into the grid) of the lane's columns and its
@@ -406,14 +400,6 @@ __device__ inline void foam_load2(const FoamArgs& a, const FoamBubble& B, int r,
     R.gx = a.g_in[base + jx];
 }
 
-__device__ inline void foam_far2(const FoamArgs& a, const FoamBubble& B, int r, int jf, FFar& F) {
-    const int rr = max(0, min(r, B.nt - 1));
-    const int64_t o = B.off + static_cast<int64_t>(rr) * B.np + jf;
-    F.h = a.hi_in[o];
-    F.l = a.lo_in[o];
-    F.g = a.g_in[o];
-}
-
 __device__ __forceinline__ float lane_value_f(float v, int t) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
 }
@@ -423,14 +409,15 @@ __device__ __forceinline__ float lane_value_f(float v, int t) {
 // chunk's float-float sum reduced over the wave as ring_partial does, and
 // the chunk sums added in chunk order as pole_mean adds the partials.
 template <int CPL, bool V4>
-__device__ __forceinline__ float2 foam_pole_after_step(const FoamArgs& a, const FoamBubble& B, int ring, float2 P0, int lane) {
+__device__ __forceinline__ float2 foam_pole_after_step(const FoamArgs& a, const FoamBubble& B, int ring, float2 P0,
+                                                       int lane) {
     const int np = B.np, nt = B.nt;
     const FoamRow rw = a.rows[B.row_off + ring];
     const float cS = B.dtc * rw.sin_t;
     float2 total = make_float2(0.f, 0.f);
     for (int ch = 0; ch < B.nchunks; ++ch) {
-        const int c = ch * 64 * CPL + CPL * lane;
-        const bool valid = c < np;
+        const int c = ch * B.cw + CPL * (lane - B.lane0);
+        const bool valid = lane >= B.lane0 && lane < 64 - B.lane0 && c < np;
         const int cc = valid ? c : 0;
         const int jw = cc == 0 ? np - 1 : cc - 1;
         const int je = (cc + CPL) % np;
@@ -472,22 +459,24 @@ __device__ __forceinline__ float2 foam_pole_after_step(const FoamArgs& a, const 
 }
 
 template <int CPL, bool V4, bool REBASE>
-__device__ __forceinline__ void foam_band2(const FoamArgs& a, const FoamItem& it, const FoamBubble& B, int lane, float* ring) {
+__device__ __forceinline__ void foam_band2(const FoamArgs& a, const FoamItem& it, const FoamBubble& B, int lane,
+                                           float* ring) {
     constexpr int W = foam_ring_width<CPL>();
     const int np = B.np, nt = B.nt;
     const int r0 = it.r0, r1 = it.r1;
-    const int c = it.c0 + CPL * lane;
-    const bool valid = c < np;
-    const int cc = valid ? c : 0;                      // idle lanes shadow column 0
-    const bool full = cc + CPL <= np;                  // false only for a 2-column tail lane
+    // Lanes 1 .. nl hold the chunk's columns, lane 0 the CPL columns before it
+    // and lane nl + 1 those after it; lanes past nl + 1 are idle.
+    const int ncols = min(B.cw, np - it.c0);           // columns of the chunk inside the bubble
+    const int nl = (ncols + CPL - 1) / CPL;            // at most 62
+    const bool valid = lane >= 1 && lane <= nl;
+    const bool active = lane <= nl + 1;
+    const int craw = it.c0 + CPL * (lane - 1);         // below 0 for lane 0 of chunk 0, past n_phi for a halo
+    const int cc = active ? (craw < 0 ? craw + np : (craw >= np ? craw - np : craw)) : 0;
+    const bool full = cc + CPL <= np;                  // false only where a 4-column lane wraps (n_phi = 2 mod 4)
     const int cc2 = full ? cc + 2 : cc + 2 - np;
     const int nvalid = valid ? (full ? CPL : 2) : 0;
-    const int ncols = min(64 * CPL, np - it.c0);       // columns of the chunk inside the bubble
-    const int last = (ncols - 1) / CPL;                // the chunk's last lane
-    const bool edge = lane == last && ncols % CPL == 0;   // forms the east halo column from jx
-    const int jw = it.c0 == 0 ? np - 1 : it.c0 - 1;
-    const int jx = lane == 0 ? jw : (cc + CPL) % np;   // halo column (lane 0 west, the edge lane east)
-    const int jf = lane == 0 ? (jw == 0 ? np - 1 : jw - 1) : (cc + CPL + 1) % np;   // far column
+    const bool east = lane == nl + 1;                  // the east halo lane: its east neighbour is jx
+    const int jx = lane == 0 ? (cc == 0 ? np - 1 : cc - 1) : (full ? (cc + CPL) % np : (cc2 + 2) % np);
 
     // Pole means of step s (p0) and of step s+1 (p1).
     float2 p0top = make_float2(0.f, 0.f), p0bot = make_float2(0.f, 0.f);
@@ -510,22 +499,24 @@ __device__ __forceinline__ void foam_band2(const FoamArgs& a, const FoamItem& it
         }
     };
     auto slot = [&](int k, int arr) __attribute__((always_inline)) { return ring + ((k % 3) * 3 + arr) * W; };
-    const int lc = 4 + CPL * lane;                     // the lane's first column in a ring row
+    const int lc = CPL * lane;                         // the lane's first column in a ring row
 
     // Step s+1 row k from step-s rows (N, C, S) into the ring.
-    auto first_step = [&](int k, const FRow2<CPL>& N, const FRow2<CPL>& C, const FRow2<CPL>& S,
-                          const FFar& F) __attribute__((always_inline)) {
+    auto first_step = [&](int k, const FRow2<CPL>& N, const FRow2<CPL>& C,
+                          const FRow2<CPL>& S) __attribute__((always_inline)) {
         float G1, G2, sn;
         row_coef(k, G1, G2, sn);
         const float cS = B.dtc * sn;
         float wh[CPL], wl[CPL], wg[CPL], eh[CPL], el[CPL], eg[CPL];
+        // West of the first column: lane - 1's last (lane 0: its column jx);
+        // east of the last: lane + 1's first (the east halo lane: its jx).
         wh[0] = from_west(C.h[CPL - 1], C.hx);
         wl[0] = from_west(C.l[CPL - 1], C.lx);
         wg[0] = from_west(C.g[CPL - 1], C.gx);
         const float xeh = from_east(C.h[0], C.hx), xel = from_east(C.l[0], C.lx), xeg = from_east(C.g[0], C.gx);
-        eh[CPL - 1] = edge ? C.hx : xeh;
-        el[CPL - 1] = edge ? C.lx : xel;
-        eg[CPL - 1] = edge ? C.gx : xeg;
+        eh[CPL - 1] = east ? C.hx : xeh;
+        el[CPL - 1] = east ? C.lx : xel;
+        eg[CPL - 1] = east ? C.gx : xeg;
 #pragma unroll
         for (int v = 1; v < CPL; ++v) {
             wh[v] = C.h[v - 1]; wl[v] = C.l[v - 1]; wg[v] = C.g[v - 1];
@@ -536,40 +527,21 @@ __device__ __forceinline__ void foam_band2(const FoamArgs& a, const FoamItem& it
         for (int v = 0; v < CPL; ++v)
             foam_cell(B, G1, G2, cS, C.h[v], C.l[v], C.g[v], N.h[v], N.l[v], N.g[v], S.h[v], S.l[v], S.g[v], wh[v],
                       wl[v], wg[v], eh[v], el[v], eg[v], lo1[v], g1[v]);
-        // The halo cell: lane 0 the west halo column (west of it the far
-        // column, east of it the lane's first column), the edge lane the
-        // east halo column (west of it the lane's last column, east of it the
-        // far column).
-        const bool l0 = lane == 0;
-        float xlo, xg;
-        foam_cell(B, G1, G2, cS, C.hx, C.lx, C.gx, N.hx, N.lx, N.gx, S.hx, S.lx, S.gx,
-                  l0 ? F.h : C.h[CPL - 1], l0 ? F.l : C.l[CPL - 1], l0 ? F.g : C.g[CPL - 1],
-                  l0 ? C.h[0] : F.h, l0 ? C.l[0] : F.l, l0 ? C.g[0] : F.g, xlo, xg);
         // A pole row of step s+1 is not stepped: h and its low word are the
         // ring mean of step s+1, Γ is the stored pole value.
         const bool pole = k == 0 || k == nt - 1;
         const float px = k == 0 ? p1top.x : p1bot.x, py = k == 0 ? p1top.y : p1bot.y;
-        float oh[CPL], ol[CPL], og[CPL];
-#pragma unroll
-        for (int v = 0; v < CPL; ++v) {
-            oh[v] = pole ? px : C.h[v];
-            ol[v] = pole ? py : lo1[v];
-            og[v] = pole ? C.g[v] : g1[v];
-        }
-        const float xoh = pole ? px : C.hx, xol = pole ? py : xlo, xog = pole ? C.gx : xg;
         float* rh = slot(k, 0);
         float* rl = slot(k, 1);
         float* rg = slot(k, 2);
-        if (valid) {
+        if (active) {
 #pragma unroll
             for (int v = 0; v < CPL; ++v) {
-                rh[lc + v] = oh[v];
-                rl[lc + v] = ol[v];
-                rg[lc + v] = og[v];
+                rh[lc + v] = pole ? px : C.h[v];
+                rl[lc + v] = pole ? py : lo1[v];
+                rg[lc + v] = pole ? C.g[v] : g1[v];
             }
         }
-        if (l0) { rh[3] = xoh; rl[3] = xol; rg[3] = xog; }
-        if (edge) { rh[4 + ncols] = xoh; rl[4 + ncols] = xol; rg[4 + ncols] = xog; }
     };
 
     // Step s+2 row i from ring rows i - 1, i, i + 1.
@@ -609,32 +581,28 @@ __device__ __forceinline__ void foam_band2(const FoamArgs& a, const FoamItem& it
     // Step-s rows k - 1, k, k + 1 form step s+1 row k (k = r0 - 1 .. r1);
     // after row k is in the ring, step s+2 row k - 1 follows.  Four row
     // buffers (one row in flight) unrolled over their rotations, so no buffer
-    // is copied while its load is outstanding; the far column of the centre
-    // row comes one row ahead in two more.
+    // is copied while its load is outstanding.
     FRow2<CPL> R0, R1, R2, R3;
-    FFar F0, F1;
     const int ks = r0 - 1;
     foam_load2<CPL, V4>(a, B, ks - 1, cc, cc2, jx, p0top, p0bot, R0);
     foam_load2<CPL, V4>(a, B, ks, cc, cc2, jx, p0top, p0bot, R1);
     foam_load2<CPL, V4>(a, B, ks + 1, cc, cc2, jx, p0top, p0bot, R2);
     foam_load2<CPL, V4>(a, B, ks + 2, cc, cc2, jx, p0top, p0bot, R3);
-    foam_far2(a, B, ks, jf, F0);
-#define SF_FOAM2_ROW(Nb, Cb, Sb, Fc, Fn)                                   \
-    {                                                                      \
-        if (k > r1) break;                                                 \
-        if (k + 1 <= r1) foam_far2(a, B, k + 1, jf, Fn);                 \
-        first_step(k, Nb, Cb, Sb, Fc);                                     \
-        __builtin_amdgcn_wave_barrier();                                   \
-        if (k - 1 >= r0) second_step(k - 1);                               \
-        __builtin_amdgcn_wave_barrier();                                   \
+#define SF_FOAM2_ROW(Nb, Cb, Sb)                                                         \
+    {                                                                                    \
+        if (k > r1) break;                                                               \
+        first_step(k, Nb, Cb, Sb);                                                       \
+        __builtin_amdgcn_wave_barrier();                                                 \
+        if (k - 1 >= r0) second_step(k - 1);                                             \
+        __builtin_amdgcn_wave_barrier();                                                 \
         if (k + 3 <= r1 + 1) foam_load2<CPL, V4>(a, B, k + 3, cc, cc2, jx, p0top, p0bot, Nb); \
-        ++k;                                                               \
+        ++k;                                                                             \
     }
     for (int k = ks;;) {
-        SF_FOAM2_ROW(R0, R1, R2, F0, F1)
-        SF_FOAM2_ROW(R1, R2, R3, F1, F0)
-        SF_FOAM2_ROW(R2, R3, R0, F0, F1)
-        SF_FOAM2_ROW(R3, R0, R1, F1, F0)
+        SF_FOAM2_ROW(R0, R1, R2)
+        SF_FOAM2_ROW(R1, R2, R3)
+        SF_FOAM2_ROW(R2, R3, R0)
+        SF_FOAM2_ROW(R3, R0, R1)
     }
 #undef SF_FOAM2_ROW
 }
@@ -745,13 +713,6 @@ unsigned cdivu(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / 
 
 int bucket_of(int np) { return np <= 96 ? 0 : (np <= 192 ? 1 : (np <= 384 ? 2 : 3)); }
 
-// Tile shape of a bubble with two steps per launch: a shape whose last column
-// chunk is exactly one lane wide would ask lane 0 for both halo columns, so
-// it gives way to the same band at half the columns per lane.
-int two_step_shape(int v, int np) {
-    while (kShapeCpl[v] >= 2 && np % (64 * kShapeCpl[v]) == kShapeCpl[v]) v = kShapeHalfCpl[v];
-    return v;
-}
 
 }  // namespace
 
@@ -817,26 +778,27 @@ struct Foam {
     void shape_bubble(int b) {
         FoamBubble& B = bub[b];
         B.variant = variant_of_bucket[bucket_of(B.np)];
-        // Applied whenever two steps per launch are possible, so that both
-        // launch modes share one layout (and one summation order of the pole
-        // ring partials).
-        if (two_ok) B.variant = two_step_shape(B.variant, B.np);
         const int cpl = kShapeCpl[B.variant];
         B.path = cpl == 1 ? 0 : (cpl == 2 ? 1 : (B.np % 4 == 0 ? 2 : 3));
-        B.nchunks = static_cast<int>(cdivu(B.np, 64 * cpl));
+        // Where two steps per launch are possible a chunk's columns sit in
+        // lanes 1..62 (lanes 0 and 63 form the halo columns of the two-step
+        // kernel), in both launch modes, so that both share one layout and
+        // one summation order of the pole ring partials.
+        B.lane0 = two_ok ? 1 : 0;
+        B.cw = (64 - 2 * B.lane0) * cpl;
+        B.nchunks = static_cast<int>(cdivu(B.np, B.cw));
     }
 
     // Work items of bubble b: column chunks x bands of at most kShapeBand
     // rows, the interior rows split as evenly as the band count allows.
     void append_items(int b, std::vector<FoamItem>& items) const {
         const FoamBubble& B = bub[b];
-        const int cpl = kShapeCpl[B.variant];
         const int rows = B.nt - 2;
         const int nbands = (rows + kShapeBand[B.variant] - 1) / kShapeBand[B.variant];
         for (int k = 0; k < nbands; ++k) {
             const int r0 = 1 + static_cast<int>(static_cast<int64_t>(rows) * k / nbands);
             const int r1 = 1 + static_cast<int>(static_cast<int64_t>(rows) * (k + 1) / nbands);
-            for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, r1, ch * 64 * cpl, ch, {0, 0, 0}});
+            for (int ch = 0; ch < B.nchunks; ++ch) items.push_back(FoamItem{b, r0, r1, ch * B.cw, ch, {0, 0, 0}});
         }
     }
 

@@ -18,15 +18,17 @@ from test_foam_gpu import make_foam, oracle_for, rel_l2, seeded_fields
 
 pytestmark = pytest.mark.gpu
 
-# Every code path and halo case: n_phi = 2 mod 4 (two-float2 lanes: 10, 66,
-# 130, 258), a last chunk that is one 4-column lane (260: the shape falls back
-# to 2 columns per lane) or one 2-column lane (130 at 2 columns per lane, 258
-# at 4: lane 0 holds the west halo and, in its third column, the east halo),
-# single-chunk bubbles whose halo columns wrap onto their own edge columns
-# (8, 10, 40, 66, 128), several chunks (520), several bands (98, 130, 258
-# interior rows) and a 4-row bubble (two interior rows).
+# Every code path and halo case.  Where two steps are possible a chunk holds
+# 62 lanes' columns (lanes 0 and 63 form its halo columns): n_phi = 2 mod 4
+# (two-float2 lanes: 10, 66, 130, 250, 258), a last chunk of exactly one lane
+# (126 at 2 columns per lane, 252 at 4), a lone two-column tail lane (250 at
+# 4 columns per lane: lane 1 holds the chunk, the west halo is lane 0 and the
+# east halo that lane's third column), single-chunk bubbles whose halo columns
+# wrap onto their own edge columns (8, 10, 40, 66 at 2 or 4 columns per lane),
+# several chunks (258, 520), several bands (98, 128, 248 interior rows) and a
+# 4-row bubble (two interior rows).
 SHAPES2 = [(4, 8), (9, 10), (20, 40), (33, 66), (65, 130), (100, 200), (64, 128), (130, 260),
-           (100, 258), (260, 520)]
+           (100, 258), (260, 520), (40, 126), (70, 252), (50, 250)]
 
 
 def pair(shapes, seed, shape_idx=None):
@@ -82,7 +84,7 @@ def test_foam_two_step_without_graphs_and_timed():
 
 def test_foam_two_step_active_mask():
     fa, fb, _, _, _ = pair(SHAPES2, seed=62)
-    mask = np.array([1, 0, 1, 1, 0, 1, 0, 1, 1, 0], np.uint8)
+    mask = np.array([1, 0, 1, 1, 0, 1, 0, 1, 1, 0, 1, 0, 1], np.uint8)
     for f in (fa, fb):
         f.step(5)
         f.set_active(mask)
