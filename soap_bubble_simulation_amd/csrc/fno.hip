@@ -671,11 +671,10 @@ struct Fno {
         return nullptr;
     }
 
-    size_t theta_fused_lds() const {
-        return static_cast<size_t>(2 * C) * (T + 8) * 2 + static_cast<size_t>(2 * Mt) * (2 * C + 1) * 4 +
-               static_cast<size_t>(C) * (2 * Mt + 8) * 2;
+    size_t theta_fused_lds() const { return static_cast<size_t>(theta_fused_lds_bytes(T, Mt, C)); }
+    bool theta_fused_ok() const {
+        return !(C % 16 || (2 * Mt) % 16 || T % 16 || kThetaThreads % (2 * C) || theta_fused_lds() > 64 * 1024);
     }
-    bool theta_fused_ok() const { return !(C % 16 || (2 * Mt) % 16 || T % 16 || theta_fused_lds() > 64 * 1024); }
 
     // Kernel paths the forward passes take at this shape (SF_FNO_PATH_* bits).
     int paths() const {
@@ -696,7 +695,13 @@ struct Fno {
             theta_stage(l, B);
             return;
         }
-        hipLaunchKernelGGL(theta_fused_kernel, dim3(static_cast<unsigned>(Mp), static_cast<unsigned>(B)), dim3(512), lds,
+        // the benchmark network (256 rows, width 32, 16 x 16 modes) has its
+        // own instantiation
+        using ThetaKernel = void (*)(const float*, const unsigned short*, const float*, const unsigned short*, float*,
+                                     int, int, int, int);
+        const bool fixed = T == 256 && C == 32 && Mt == 32 && Mp == 16;
+        const ThetaKernel tk = fixed ? theta_fused_kernel<256, 32, 32, 16> : theta_fused_kernel<0, 0, 0, 0>;
+        hipLaunchKernelGGL(tk, dim3(static_cast<unsigned>(Mp), static_cast<unsigned>(B)), dim3(kThetaThreads), lds,
                            stream, X1, Eth16, w + o_r[l], Ath16, Z, T, Mt, Mp, C);
         SF_HIP(hipGetLastError());
     }

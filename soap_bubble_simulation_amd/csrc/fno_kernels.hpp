@@ -976,41 +976,67 @@ __global__ void mode_mix_kernel(const float* Xs, const float* R, float* Y, int B
 //   Ath [2 T][2 Mt] BF16 inverse theta DFT
 //   Z   [B T][J][C]     output (the columns j = m and Mp + m of this block)
 // Dynamic LDS: X slice [2C][T + 8] BF16, G [2 Mt][2C + 1] FP32, Y [C][2 Mt + 8]
-// BF16.  Requires C, 2 Mt and T multiples of 16.
-__global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restrict__ X1,
-                                                          const unsigned short* __restrict__ Eth,
-                                                          const float* __restrict__ R,
-                                                          const unsigned short* __restrict__ Ath,
-                                                          float* __restrict__ Z, int T, int Mt, int Mp, int C) {
+// BF16.  Where every wave holds at most two G tiles and G and Y fit in the
+// X slice, G and Y take the slice's place once it has been read (each wave
+// keeps its G tiles in registers across a barrier), so the block needs only
+// the slice: 34 KB at T = 256, C = 32, Mt = 32, four blocks per CU instead of
+// two.  Requires C, 2 Mt and T multiples of 16 and 512 % 2C == 0.
+// TT, CT, MTT, MPT fix T, C, Mt and Mp at compile time (0: taken from the
+// arguments): with them fixed the index arithmetic folds into constants and
+// immediate offsets, where it otherwise costs more instructions than the
+// products.
+constexpr int kThetaThreads = 512;
+
+__host__ __device__ inline bool theta_fused_alias(int T, int Mt, int C) {
+    const long sx = 2L * C * (T + 8) * 2, sg = 2L * Mt * (2 * C + 1) * 4, sy = 1L * C * (2 * Mt + 8) * 2;
+    return ((2 * Mt) / 16) * ((2 * C) / 16) <= 2 * (kThetaThreads / 64) && sg + sy <= sx;
+}
+__host__ __device__ inline long theta_fused_lds_bytes(int T, int Mt, int C) {
+    const long sx = 2L * C * (T + 8) * 2, sg = 2L * Mt * (2 * C + 1) * 4, sy = 1L * C * (2 * Mt + 8) * 2;
+    return theta_fused_alias(T, Mt, C) ? sx : sx + sg + sy;
+}
+
+template <int TT, int CT, int MTT, int MPT>
+__global__ __launch_bounds__(kThetaThreads) __attribute__((amdgpu_waves_per_eu(8))) void theta_fused_kernel(const float* __restrict__ X1,
+                                                                    const unsigned short* __restrict__ Eth,
+                                                                    const float* __restrict__ R,
+                                                                    const unsigned short* __restrict__ Ath,
+                                                                    float* __restrict__ Z, int T_, int Mt_, int Mp_,
+                                                                    int C_) {
     extern __shared__ __align__(16) unsigned char th_lds[];
+    const int T = TT ? TT : T_, Mt = MTT ? MTT : Mt_, Mp = MPT ? MPT : Mp_, C = CT ? CT : C_;
     const int m = blockIdx.x;
     const long long b = blockIdx.y;
     const int J = 2 * Mp, C2 = 2 * C, M2 = 2 * Mt;
     const int xs_ld = T + 8, g_ld = C2 + 1, y_ld = M2 + 8;
+    const bool alias = theta_fused_alias(T, Mt, C);
     unsigned short* sx = reinterpret_cast<unsigned short*>(th_lds);                        // [C2][xs_ld]
-    float* sg = reinterpret_cast<float*>(th_lds + static_cast<size_t>(C2) * xs_ld * 2);    // [M2][g_ld]
+    float* sg = reinterpret_cast<float*>(alias ? th_lds : th_lds + static_cast<size_t>(C2) * xs_ld * 2);   // [M2][g_ld]
     unsigned short* sy = reinterpret_cast<unsigned short*>(sg + static_cast<size_t>(M2) * g_ld);   // [C][y_ld]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int nwaves = kThetaThreads / 64;
     const int mr = lane & 15, kq = 4 * (lane >> 4);
 
-    // 1. the m-th coefficients of every row, transposed to [q][t], q = ri C + c;
-    // eight loads per thread are in flight before their stores
-    const int nx = T * C2, step = static_cast<int>(blockDim.x);
-    for (int idx0 = tid; idx0 < nx; idx0 += 8 * step) {
-        float v[8];
+    // 1. the m-th coefficients of every row, transposed to [q][t], q = ri C + c:
+    // a thread keeps one q and takes every (512 / 2C)-th row; sixteen loads
+    // per thread are in flight before their stores
+    {
+        const int q = tid % C2, tstep = kThetaThreads / C2;
+        const int ri = q / C, c = q - ri * C;
+        const int JC = J * C;
+        const float* src = X1 + ((b * T) * J + ri * Mp + m) * C + c;
+        unsigned short* dst = sx + q * xs_ld;
+        for (int t0 = tid / C2; t0 < T; t0 += 16 * tstep) {
+            float v[16];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int idx = idx0 + s * step;
-            const int t = idx / C2, q = idx - t * C2;
-            const int ri = q / C, c = q - ri * C;
-            v[s] = idx < nx ? X1[((b * T + t) * J + ri * Mp + m) * C + c] : 0.f;
-        }
+            for (int s = 0; s < 16; ++s) {
+                const int t = t0 + s * tstep;
+                v[s] = t < T ? src[t * JC] : 0.f;
+            }
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int idx = idx0 + s * step;
-            if (idx < nx) {
-                const int t = idx / C2, q = idx - t * C2;
-                sx[q * xs_ld + t] = to_bf16(v[s]);
+            for (int s = 0; s < 16; ++s) {
+                const int t = t0 + s * tstep;
+                if (t < T) dst[t] = to_bf16(v[s]);
             }
         }
     }
@@ -1018,6 +1044,7 @@ __global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restric
 
     // 2. G[p][q] = sum_t Eth[p][t] X[t][q]
     const int gt_p = M2 / 16, gt_q = C2 / 16;
+    f32x4 keep0 = f32x4{0.f, 0.f, 0.f, 0.f}, keep1 = keep0;     // with alias: the wave's tiles
     for (int tile = wave; tile < gt_p * gt_q; tile += nwaves) {
         const int pt = tile / gt_q, qt = tile - pt * gt_q;
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -1036,8 +1063,26 @@ __global__ __launch_bounds__(512) void theta_fused_kernel(const float* __restric
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Af[s], Bf, acc, 0, 0, 0);
             }
         }
+        if (alias) {
+            if (tile == wave)
+                keep0 = acc;
+            else
+                keep1 = acc;
+        } else {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sg[(pt * 16 + kq + v) * g_ld + qt * 16 + mr] = acc[v];
+            for (int v = 0; v < 4; ++v) sg[(pt * 16 + kq + v) * g_ld + qt * 16 + mr] = acc[v];
+        }
+    }
+    if (alias) {
+        __syncthreads();                          // every wave has read the X slice
+        for (int i = 0; i < 2; ++i) {
+            const int tile = wave + i * nwaves;
+            if (tile >= gt_p * gt_q) break;
+            const int pt = tile / gt_q, qt = tile - pt * gt_q;
+            const f32x4 acc = i == 0 ? keep0 : keep1;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sg[(pt * 16 + kq + v) * g_ld + qt * 16 + mr] = acc[v];
+        }
     }
     __syncthreads();
 
