@@ -40,11 +40,16 @@ METRIC = "drainage cell-steps/s plain vs surrogate at ≤1% rel-L2 h; % HBM roof
 HBM_PEAK_GBS = 8000.0
 # Vector-ALU issue model of the spectral pass: per (cell, wavelength slot)
 # v_cos_f32 (8 cycles per wave64 instruction), one FMA (4), v_rcp_f32 (8) and
-# half a v_cvt_pk_f16_f32 (2); the phase products of 20 of a lane's 21 slots
-# run on 4x4x4 MFMAs (not counted), the 21st is one multiply (4 / 21); 84
-# slots per cell (81 wavelengths padded to 4 x 21); 1024 SIMDs at 2.4 GHz.
+# half a v_cvt_pk_f16_f32 (2); the phase products of slots 0..19 of a lane
+# run on 4x4x4 MFMAs (not counted), slot 20 is one multiply (4 / 21); 84
+# slots per cell (81 wavelengths padded to 4 x 21), or 80 where slot 20
+# (780 nm and padding) is skipped because the curves give 780 nm a weight
+# below SPECTRAL_TAIL_BUDGET; 1024 SIMDs at 2.4 GHz.
 SPECTRAL_CYCLES_PER_SLOT = 22.0 + 4.0 / 21.0
 SPECTRAL_SLOTS_PER_CELL = 84
+SPECTRAL_CYCLES_PER_SLOT_NO_TAIL = 22.0
+SPECTRAL_SLOTS_PER_CELL_NO_TAIL = 80
+SPECTRAL_TAIL_BUDGET = 1e-6          # max_c |cmf[80][c]| / 81 below which slot 20 is skipped
 SIMD_COUNT = 1024
 ENGINE_CLOCK_HZ = 2.4e9
 BYTES_PER_CELL_STEP = 16.0       # h and surfactant, FP32, read + write
@@ -770,14 +775,18 @@ def main():
         # The spectral pass is bound by vector-ALU issue (two transcendentals
         # per slot; its MFMAs take a small share): its fraction of the
         # issue-model peak.
-        peak = SIMD_COUNT * ENGINE_CLOCK_HZ * 64 / SPECTRAL_CYCLES_PER_SLOT
-        achieved = cells * SPECTRAL_SLOTS_PER_CELL / (ms_spec.value * 1e-3)
+        tail = float(np.max(np.abs(cmf[80].astype(np.float64)))) / 81.0 > SPECTRAL_TAIL_BUDGET
+        cps = SPECTRAL_CYCLES_PER_SLOT if tail else SPECTRAL_CYCLES_PER_SLOT_NO_TAIL
+        slots = SPECTRAL_SLOTS_PER_CELL if tail else SPECTRAL_SLOTS_PER_CELL_NO_TAIL
+        peak = SIMD_COUNT * ENGINE_CLOCK_HZ * 64 / cps
+        achieved = cells * slots / (ms_spec.value * 1e-3)
         out["kernels"]["spectral_valu"] = {
             "bound": "valu", "achieved": achieved, "peak": peak, "unit": "(cell, wavelength slot)/s",
             "frac": achieved / peak,
-            "model": "22.2 issue cycles per wave64 slot: v_cos 8 + fma 4 + v_rcp 8 + cvt_pk/2 2 + one multiply "
-                     "per 21 slots (the other phase products on 4x4x4 MFMAs, not counted); 84 slots/cell; "
-                     "1024 SIMDs at 2.4 GHz"}
+            "model": ("%.1f issue cycles per wave64 slot: v_cos 8 + fma 4 + v_rcp 8 + cvt_pk/2 2%s (phase products "
+                      "on 4x4x4 MFMAs, not counted); %d slots/cell%s; 1024 SIMDs at 2.4 GHz"
+                      % (cps, " + one multiply per 21 slots" if tail else "", slots,
+                         "" if tail else " (780 nm skipped: its curve weight is below 1e-6 per channel)"))}
     if args.workload == "c1":
         # The C1 configuration also builds the 2-D interference LUT: 1024
         # thicknesses (0-1000 nm) x 90 angles at 650/532/450 nm, host-timed

@@ -271,6 +271,14 @@ constexpr float kSpecFCos2 = 16.f;
 // Largest RGB change the FP16 rounding of r may cause for the FP16 operand
 // format to be chosen automatically (a quarter of the 1e-3 RGB tolerance).
 constexpr double kSpecF16Budget = 2.5e-4;
+// Slot 20 holds wavelength 80 (780 nm) in one lane of four and padding in
+// the others.  Where the curves give that wavelength no weight that matters,
+// max_c |cmf[80][c]| / 81 <= kSpecTailBudget, the slot is not evaluated: r is
+// taken as 0 there (I = 1), which moves channel c by cmf[80][c] r / 81, at
+// most that bound (a thousandth of the RGB tolerance).  Colour-matching
+// curves are near zero at 780 nm (CIE 1931 x-bar 4.2e-5), so the default
+// pass evaluates 20 slots per lane and cell instead of 21.
+constexpr double kSpecTailBudget = 1e-6;
 // Paths below which v_cos_f32 (argument in revolutions, exact to its
 // specification for |x| <= 256) takes opd / lambda unreduced.
 constexpr float kSpecOpFast = 5.0e4f;
@@ -414,9 +422,9 @@ __device__ inline uint32_t lane_read_u(uint32_t v, int src_lane) {
 }
 
 // One 16-cell group: the phases of slots 0..19 on the phase MFMAs, slot 20
-// (wavelengths 80..83, 81..83 padding) as one multiply; r on the VALU; r
-// contracted with the curves on 16x16x32 MFMAs into acc.
-template <bool FAST, bool F16>
+// (wavelengths 80..83, 81..83 padding) as one multiply when TAIL; r on the
+// VALU; r contracted with the curves on 16x16x32 MFMAs into acc.
+template <bool FAST, bool F16, bool TAIL>
 __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, float u20, const u32x4 (&bh)[kSpecMfma],
                                   const u32x4 (&bl)[kSpecMfma], bool b_lo, float P, float A, float B,
                                   v4f& acc) {
@@ -434,7 +442,7 @@ __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, floa
 #pragma unroll
             for (int v = 0; v < 4; ++v) r[4 + v] = spec_r_x<FAST>(x1[v], A, B);
         } else {
-            r[4] = spec_r_x<FAST>(P * u20, A, B);
+            r[4] = TAIL ? spec_r_x<FAST>(P * u20, A, B) : 0.f;
             r[5] = r[6] = r[7] = 0.f;
         }
         if (F16) {
@@ -465,7 +473,7 @@ __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, floa
 
 // Five waves per SIMD (96 VGPRs; the compiler moves a few per-batch values
 // to scratch outside the wavelength loop): 1.4 % faster than four.
-template <bool F16>
+template <bool F16, bool TAIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void spectral_kernel(const SpecArgs a) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 15;
@@ -516,9 +524,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             bop[1] = lane_read_u(pp.x1, src);
             acc[g] = v4f{S, S, S, S};
             if (fast)
-                spec_group<true, F16>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
+                spec_group<true, F16, TAIL>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
             else
-                spec_group<false, F16>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
+                spec_group<false, F16, TAIL>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
         }
         // acc[g][v] = D[4 q + v][m]: cell 64 sg + 16 g + 4 q + v, colour m.
         if (m < 3) {
@@ -799,6 +807,7 @@ struct PackedCmf {
     int has_lo[2];
     bool f16_bound;           // 2^-12 sum_k |cmf[k][c]| / 81 <= kSpecF16Budget for every c
     bool f16_range;           // every |cmf| <= 2^14
+    bool tail;                // max_c |cmf[80][c]| / 81 > kSpecTailBudget: slot 20 is evaluated
 };
 
 // Packs -cmf[81][3] into the B-operand layout of spectral_kernel: lane
@@ -819,6 +828,10 @@ static PackedCmf pack_cmf(const float* cmf) {
             maxabs = std::max(maxabs, std::fabs(static_cast<double>(cmf[3 * k + c])));
         }
     out.f16_range = maxabs <= 16384.0;
+    out.tail = false;
+    for (int c = 0; c < 3; ++c)
+        if (std::fabs(static_cast<double>(cmf[3 * (kSpecLambdas - 1) + c])) / kSpecLambdas > kSpecTailBudget)
+            out.tail = true;
     out.f16_bound = true;
     for (int c = 0; c < 3; ++c)
         if (std::ldexp(sabs[c], -12) / kSpecLambdas > kSpecF16Budget) out.f16_bound = false;
@@ -925,6 +938,7 @@ static void spectral_prepare(Drainage* d, const float* cmf) {
         }
         d->cmf_f16_bound = packed.f16_bound;
         d->cmf_f16_range = packed.f16_range;
+        d->cmf_tail = packed.tail;
     }
 }
 
@@ -990,7 +1004,8 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     // (145.8 us at 2048 x 4096, against 154.5 us with 2560 grid-striding
     // blocks and 149.5 us with one group per wave).
     const unsigned blocks = static_cast<unsigned>(std::max<size_t>(1, (supergroups + 7) / 8));
-    void (*kern)(const SpecArgs) = f ? spectral_kernel<true> : spectral_kernel<false>;
+    void (*kern)(const SpecArgs) = f ? (d->cmf_tail ? spectral_kernel<true, true> : spectral_kernel<true, false>)
+                                     : (d->cmf_tail ? spectral_kernel<false, true> : spectral_kernel<false, false>);
     if (d->spec_timer.enabled) {
         auto& ev = d->spec_timer.next(st);
         hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, st, ev.first, ev.second, 0, a);
