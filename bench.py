@@ -42,14 +42,28 @@ HBM_PEAK_GBS = 8000.0
 # v_cos_f32 (8 cycles per wave64 instruction), one FMA (4), v_rcp_f32 (8) and
 # half a v_cvt_pk_f16_f32 (2); the phase products of slots 0..19 of a lane
 # run on 4x4x4 MFMAs (not counted), slot 20 is one multiply (4 / 21); 84
-# slots per cell (81 wavelengths padded to 4 x 21), or 80 where slot 20
-# (780 nm and padding) is skipped because the curves give 780 nm a weight
-# below SPECTRAL_TAIL_BUDGET; 1024 SIMDs at 2.4 GHz.
-SPECTRAL_CYCLES_PER_SLOT = 22.0 + 4.0 / 21.0
-SPECTRAL_SLOTS_PER_CELL = 84
-SPECTRAL_CYCLES_PER_SLOT_NO_TAIL = 22.0
-SPECTRAL_SLOTS_PER_CELL_NO_TAIL = 80
-SPECTRAL_TAIL_BUDGET = 1e-6          # max_c |cmf[80][c]| / 81 below which slot 20 is skipped
+# slots per cell (81 wavelengths padded to 4 x 21), less 4 per trailing slot
+# the pass skips (slot 20, then 19, while the skipped wavelengths weigh at
+# most SPECTRAL_TAIL_BUDGET per channel); 1024 SIMDs at 2.4 GHz.
+SPECTRAL_CYCLES_PER_SLOT = 22.0
+SPECTRAL_SLOT20_CYCLES = 4.0 / 21.0
+SPECTRAL_TAIL_BUDGET = 1e-5
+SPECTRAL_MAX_SKIP = 2
+
+
+def spectral_skip(cmf):
+    """Trailing wavelength slots the spectral pass leaves out for these
+    curves (the library's rule): slot 20, then 19, while the skipped
+    wavelengths' sum_k |cmf[k][c]| / 81 stays within the budget."""
+    cm = np.abs(np.asarray(cmf, np.float64).reshape(81, 3)) / 81.0
+    skip, tail = 0, np.zeros(3)
+    for sk in range(1, SPECTRAL_MAX_SKIP + 1):
+        slot = 21 - sk
+        tail = tail + cm[4 * slot:min(4 * slot + 4, 81)].sum(axis=0)
+        if np.any(tail > SPECTRAL_TAIL_BUDGET):
+            break
+        skip = sk
+    return skip
 SIMD_COUNT = 1024
 ENGINE_CLOCK_HZ = 2.4e9
 BYTES_PER_CELL_STEP = 16.0       # h and surfactant, FP32, read + write
@@ -775,18 +789,18 @@ def main():
         # The spectral pass is bound by vector-ALU issue (two transcendentals
         # per slot; its MFMAs take a small share): its fraction of the
         # issue-model peak.
-        tail = float(np.max(np.abs(cmf[80].astype(np.float64)))) / 81.0 > SPECTRAL_TAIL_BUDGET
-        cps = SPECTRAL_CYCLES_PER_SLOT if tail else SPECTRAL_CYCLES_PER_SLOT_NO_TAIL
-        slots = SPECTRAL_SLOTS_PER_CELL if tail else SPECTRAL_SLOTS_PER_CELL_NO_TAIL
+        skip = spectral_skip(cmf)
+        cps = SPECTRAL_CYCLES_PER_SLOT + (SPECTRAL_SLOT20_CYCLES if skip == 0 else 0.0)
+        slots = 4 * (21 - skip)
         peak = SIMD_COUNT * ENGINE_CLOCK_HZ * 64 / cps
         achieved = cells * slots / (ms_spec.value * 1e-3)
         out["kernels"]["spectral_valu"] = {
             "bound": "valu", "achieved": achieved, "peak": peak, "unit": "(cell, wavelength slot)/s",
             "frac": achieved / peak,
             "model": ("%.1f issue cycles per wave64 slot: v_cos 8 + fma 4 + v_rcp 8 + cvt_pk/2 2%s (phase products "
-                      "on 4x4x4 MFMAs, not counted); %d slots/cell%s; 1024 SIMDs at 2.4 GHz"
-                      % (cps, " + one multiply per 21 slots" if tail else "", slots,
-                         "" if tail else " (780 nm skipped: its curve weight is below 1e-6 per channel)"))}
+                      "on 4x4x4 MFMAs, not counted); %d slots/cell (%d trailing slots skipped: the curves weigh "
+                      "their wavelengths below 1e-5 per channel); 1024 SIMDs at 2.4 GHz"
+                      % (cps, " + one multiply per 21 slots" if skip == 0 else "", slots, skip))}
     if args.workload == "c1":
         # The C1 configuration also builds the 2-D interference LUT: 1024
         # thicknesses (0-1000 nm) x 90 angles at 650/532/450 nm, host-timed

@@ -126,7 +126,7 @@ struct Drainage {
     int cmf_lo[2] = {0, 0};
     bool cmf_f16_bound = false;     // FP16 r keeps the RGB error within kSpecF16Budget
     bool cmf_f16_range = false;     // the curves fit FP16
-    bool cmf_tail = true;           // wavelength 80 (780 nm) weighs more than kSpecTailBudget
+    int cmf_skip = 0;               // trailing wavelength slots the spectral pass leaves out
     int spec_operand = 0;           // SF_SPECTRAL_OPERAND_*
     float4* spec_rows = nullptr;
     float2* spec_view = nullptr;    // per-cell (F, cos theta_t) of spec_view_key's camera and index

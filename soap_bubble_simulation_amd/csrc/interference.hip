@@ -271,14 +271,15 @@ constexpr float kSpecFCos2 = 16.f;
 // Largest RGB change the FP16 rounding of r may cause for the FP16 operand
 // format to be chosen automatically (a quarter of the 1e-3 RGB tolerance).
 constexpr double kSpecF16Budget = 2.5e-4;
-// Slot 20 holds wavelength 80 (780 nm) in one lane of four and padding in
-// the others.  Where the curves give that wavelength no weight that matters,
-// max_c |cmf[80][c]| / 81 <= kSpecTailBudget, the slot is not evaluated: r is
-// taken as 0 there (I = 1), which moves channel c by cmf[80][c] r / 81, at
-// most that bound (a thousandth of the RGB tolerance).  Colour-matching
-// curves are near zero at 780 nm (CIE 1931 x-bar 4.2e-5), so the default
-// pass evaluates 20 slots per lane and cell instead of 21.
-constexpr double kSpecTailBudget = 1e-6;
+// Slot s holds wavelengths 4 s .. 4 s + 3 (slot 20: wavelength 80, 780 nm,
+// and padding).  Trailing slots whose wavelengths the curves give no weight
+// that matters are not evaluated: slot 20, and then slot 19 (760-775 nm),
+// are skipped while sum_{skipped k} |cmf[k][c]| / 81 <= kSpecTailBudget for
+// every c.  r is taken as 0 there (I = 1), which moves channel c by at most
+// that sum (a hundredth of the RGB tolerance).  CIE 1931 curves skip slot 20
+// (x-bar 4.2e-5 at 780 nm); curves that end below 740 nm skip both.
+constexpr double kSpecTailBudget = 1e-5;
+constexpr int kSpecMaxSkip = 2;     // trailing slots that may be skipped
 // Paths below which v_cos_f32 (argument in revolutions, exact to its
 // specification for |x| <= 256) takes opd / lambda unreduced.
 constexpr float kSpecOpFast = 5.0e4f;
@@ -422,9 +423,10 @@ __device__ inline uint32_t lane_read_u(uint32_t v, int src_lane) {
 }
 
 // One 16-cell group: the phases of slots 0..19 on the phase MFMAs, slot 20
-// (wavelengths 80..83, 81..83 padding) as one multiply when TAIL; r on the
-// VALU; r contracted with the curves on 16x16x32 MFMAs into acc.
-template <bool FAST, bool F16, bool TAIL>
+// (wavelengths 80..83, 81..83 padding) as one multiply; r on the VALU, but
+// for the SKIP trailing slots; r contracted with the curves on 16x16x32 MFMAs
+// into acc.
+template <bool FAST, bool F16, int SKIP>
 __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, float u20, const u32x4 (&bh)[kSpecMfma],
                                   const u32x4 (&bl)[kSpecMfma], bool b_lo, float P, float A, float B,
                                   v4f& acc) {
@@ -435,14 +437,15 @@ __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, floa
         float r[8];
         const v4f x0 = __builtin_amdgcn_mfma_f32_4x4x4f16(__builtin_bit_cast(f16x4, pa[2 * t]), bp, z, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) r[v] = spec_r_x<FAST>(x0[v], A, B);
+        for (int v = 0; v < 4; ++v)
+            r[v] = 8 * t + v < kSpecSlots - SKIP ? spec_r_x<FAST>(x0[v], A, B) : 0.f;
         if (2 * t + 1 < kSpecPhase) {
             const v4f x1 =
                 __builtin_amdgcn_mfma_f32_4x4x4f16(__builtin_bit_cast(f16x4, pa[2 * t + 1]), bp, z, 0, 0, 0);
 #pragma unroll
             for (int v = 0; v < 4; ++v) r[4 + v] = spec_r_x<FAST>(x1[v], A, B);
         } else {
-            r[4] = TAIL ? spec_r_x<FAST>(P * u20, A, B) : 0.f;
+            r[4] = SKIP == 0 ? spec_r_x<FAST>(P * u20, A, B) : 0.f;
             r[5] = r[6] = r[7] = 0.f;
         }
         if (F16) {
@@ -473,7 +476,7 @@ __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, floa
 
 // Five waves per SIMD (96 VGPRs; the compiler moves a few per-batch values
 // to scratch outside the wavelength loop): 1.4 % faster than four.
-template <bool F16, bool TAIL>
+template <bool F16, int SKIP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void spectral_kernel(const SpecArgs a) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 15;
@@ -524,9 +527,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             bop[1] = lane_read_u(pp.x1, src);
             acc[g] = v4f{S, S, S, S};
             if (fast)
-                spec_group<true, F16, TAIL>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
+                spec_group<true, F16, SKIP>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
             else
-                spec_group<false, F16, TAIL>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
+                spec_group<false, F16, SKIP>(pa, bop, u20, bh, bl, b_lo, P, A, B, acc[g]);
         }
         // acc[g][v] = D[4 q + v][m]: cell 64 sg + 16 g + 4 q + v, colour m.
         if (m < 3) {
@@ -807,7 +810,7 @@ struct PackedCmf {
     int has_lo[2];
     bool f16_bound;           // 2^-12 sum_k |cmf[k][c]| / 81 <= kSpecF16Budget for every c
     bool f16_range;           // every |cmf| <= 2^14
-    bool tail;                // max_c |cmf[80][c]| / 81 > kSpecTailBudget: slot 20 is evaluated
+    int skip;                 // trailing slots left out (kSpecTailBudget)
 };
 
 // Packs -cmf[81][3] into the B-operand layout of spectral_kernel: lane
@@ -828,10 +831,19 @@ static PackedCmf pack_cmf(const float* cmf) {
             maxabs = std::max(maxabs, std::fabs(static_cast<double>(cmf[3 * k + c])));
         }
     out.f16_range = maxabs <= 16384.0;
-    out.tail = false;
-    for (int c = 0; c < 3; ++c)
-        if (std::fabs(static_cast<double>(cmf[3 * (kSpecLambdas - 1) + c])) / kSpecLambdas > kSpecTailBudget)
-            out.tail = true;
+    out.skip = 0;
+    double tail[3] = {0.0, 0.0, 0.0};
+    for (int sk = 1; sk <= kSpecMaxSkip; ++sk) {
+        const int slot = kSpecSlots - sk;
+        bool ok = true;
+        for (int c = 0; c < 3; ++c) {
+            for (int k = 4 * slot; k < std::min(4 * slot + 4, kSpecLambdas); ++k)
+                tail[c] += std::fabs(static_cast<double>(cmf[3 * k + c])) / kSpecLambdas;
+            if (tail[c] > kSpecTailBudget) ok = false;
+        }
+        if (!ok) break;
+        out.skip = sk;
+    }
     out.f16_bound = true;
     for (int c = 0; c < 3; ++c)
         if (std::ldexp(sabs[c], -12) / kSpecLambdas > kSpecF16Budget) out.f16_bound = false;
@@ -938,7 +950,7 @@ static void spectral_prepare(Drainage* d, const float* cmf) {
         }
         d->cmf_f16_bound = packed.f16_bound;
         d->cmf_f16_range = packed.f16_range;
-        d->cmf_tail = packed.tail;
+        d->cmf_skip = packed.skip;
     }
 }
 
@@ -1004,8 +1016,10 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     // (145.8 us at 2048 x 4096, against 154.5 us with 2560 grid-striding
     // blocks and 149.5 us with one group per wave).
     const unsigned blocks = static_cast<unsigned>(std::max<size_t>(1, (supergroups + 7) / 8));
-    void (*kern)(const SpecArgs) = f ? (d->cmf_tail ? spectral_kernel<true, true> : spectral_kernel<true, false>)
-                                     : (d->cmf_tail ? spectral_kernel<false, true> : spectral_kernel<false, false>);
+    static void (*const kernels[2][kSpecMaxSkip + 1])(const SpecArgs) = {
+        {spectral_kernel<false, 0>, spectral_kernel<false, 1>, spectral_kernel<false, 2>},
+        {spectral_kernel<true, 0>, spectral_kernel<true, 1>, spectral_kernel<true, 2>}};
+    void (*kern)(const SpecArgs) = kernels[f][d->cmf_skip];
     if (d->spec_timer.enabled) {
         auto& ev = d->spec_timer.next(st);
         hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, st, ev.first, ev.second, 0, a);

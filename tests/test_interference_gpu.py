@@ -252,25 +252,32 @@ def test_spectral_operand_formats(oracle, scale):
     L.check(lib.sf_spectral_set_operand(ctx.handle, L.SPECTRAL_OPERAND_AUTO))
 
 
-def test_spectral_780nm_slot(oracle):
-    """Wavelength 80 (780 nm) is evaluated only where the curves weigh it.
-    Curves with max_c |cmf[80][c]| / 81 <= 1e-6 (the workload curves, and
-    colour-matching functions generally) skip its slot, which moves the RGB
-    by at most that bound; curves that weigh it take the slot, where skipping
-    it would move the RGB by about 0.5 / 81 = 6e-3."""
+def test_spectral_trailing_slots(oracle):
+    """Trailing wavelength slots (slot 20: 780 nm and padding; slot 19:
+    760-775 nm) are evaluated only where the curves weigh them: they are
+    skipped while the skipped wavelengths' sum_k |cmf[k][c]| / 81 stays
+    within 1e-5 for every c, which bounds the RGB change.  Curves that weigh
+    them take the slots; skipping them there would move the RGB by 1e-2."""
     nt, npf = 64, 128
     ctx, h0, p, cam, cmf = _spectral_case(nt, npf, 9)
-    light = cmf.astype(np.float32)
-    light[80] = [4.2e-5, 1.5e-5, 0.0]             # CIE 1931 at 780 nm
-    assert np.max(np.abs(light[80].astype(np.float64))) / 81.0 <= 1e-6
-    heavy = light.copy()
-    heavy[80] = [0.5, 0.3, 0.2]
-    for name, cm, tail in (("light", light, 1e-6), ("heavy", heavy, 0.0)):
+    base = cmf.astype(np.float32)
+    cases = []
+    heavy = base.copy()
+    heavy[76:81] = [0.5, 0.3, 0.2]                 # both slots weigh: none skipped
+    cases.append(("both slots weighed", heavy, 0.0))
+    cie = base.copy()
+    cie[76:80] = [0.3, 0.1, 0.05]
+    cie[80] = [4.2e-5, 1.5e-5, 0.0]               # CIE 1931 at 780 nm: slot 20 skipped
+    cases.append(("780 nm light", cie, 1e-5))
+    short = base.copy()
+    short[76:81] = [1e-5, 2e-6, 0.0]               # curves ending below 760 nm: slots 19-20 skipped
+    cases.append(("760-780 nm light", short, 1e-5))
+    for name, cm, tail in cases:
         got = spectral_rgb(ctx, cam, cm).reshape(3, -1)
         want = oracle.spectral_rgb(h0, p.bubble_radius, cam, 1.33, cm.astype(np.float64)).reshape(3, -1)
         bound16 = 2.0 ** -12 * np.abs(cm.astype(np.float64)).sum(axis=0) / 81.0
         err = np.max(np.abs(got - want), axis=1)
-        print("%s 780 nm weight: max err per channel %s (bound %s)" % (name, err, bound16 + tail + 2e-5))
+        print("%s: max err per channel %s (bound %s)" % (name, err, bound16 + tail + 2e-5))
         assert np.all(err <= bound16 + tail + 2e-5), name
 
 
