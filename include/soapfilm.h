@@ -267,7 +267,7 @@ typedef struct sf_camera {
  *   rgb[c][cell] = (1/81) sum_k I_k cmf[k][c].
  * Operand format (sf_spectral_set_operand): by default FP16 when its
  * rounding of the reflectance moves no channel by more than 2.5e-4
- * (2^-12 sum_k |cmf[k][c]| / 81) and the curves fit FP16, else a BF16 hi/lo
+ * (2^-18 sum_k |cmf[k][c]|) and the curves fit FP16, else a BF16 hi/lo
  * pair that keeps about 16 bits.
  * Reads the context's current h; writes 3 planes of n_theta*n_phi floats
  * to host memory (rgb_host) and/or keeps them on the device. */

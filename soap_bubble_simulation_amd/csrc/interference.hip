@@ -244,13 +244,18 @@ __global__ void calc_kernel(CalcArgs a) {
 // small FP16 MFMAs (v_mfma_f32_4x4x4_16b_f16, see phase_parts) rather than
 // one VALU multiply per slot.
 //
-// Operand formats.  r lies in [0, 1], where FP16 (11-bit significand) rounds
-// to nearest within 2^-12, so one FP16 operand changes channel c by at most
-// 2^-12 sum_k |cmf[k][c]| / 81; that is the default (one v_cvt_pk_f16_f32 per
-// two slots, one MFMA per group and curve part) whenever this bound stays
-// within kSpecF16Budget and the curves fit FP16.  Otherwise r is split into
-// truncated BF16 hi and lo = r - hi (exact), each contracted with the BF16
-// curves, which keeps about 16 bits of r at three more VALU operations per
+// The mean over the 81 wavelengths is folded into r: the kernel forms
+// r' = r / 81 = 1 / (81 A + 81 B c) with the per-cell coefficients scaled,
+// and starts each accumulator at sum_k cmf[k][c] / 81, so the contraction
+// gives the RGB with no multiply per output.
+//
+// Operand formats.  r' lies in [0, 1/81], where FP16 (11-bit significand)
+// rounds to nearest within 2^-18, so one FP16 operand changes channel c by
+// at most 2^-18 sum_k |cmf[k][c]|; that is the default (one v_cvt_pk_f16_f32
+// per two slots, one MFMA per group and curve part) whenever this bound stays
+// within kSpecF16Budget and the curves fit FP16.  Otherwise r' is split into
+// truncated BF16 hi and lo = r' - hi (exact), each contracted with the BF16
+// curves, which keeps about 16 bits of r' at three more VALU operations per
 // slot.  Either way the curves are a hi part plus the remainder's lo part in
 // the same format, and hi times lo runs only when some lo is nonzero.
 
@@ -297,7 +302,7 @@ struct SpecArgs {
     size_t n;
     float cx, cy, cz, cam2;  // camera and |cam|^2
     float radius, n_film, inv_n_film;
-    float s0, s1, s2;       // sum over the wavelengths of each (hi + lo) curve
+    float s0, s1, s2;       // mean over the wavelengths of each (hi + lo) curve
     int b_lo;               // the curves have a nonzero lo part
     // Pole rows held as ring partials (lazy poles): their h is the ring mean
     // from hpole, as materialize_poles_kernel would write it.
@@ -497,9 +502,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         bl[t] = a.bfrag[(kSpecMfma + t) * 64 + lane];
     }
     const bool b_lo = a.b_lo != 0;
-    const float scale = 1.f / static_cast<float>(kSpecLambdas);
     // The B fragments hold -cmf, so starting the accumulator at the curve's
-    // sum S gives S - sum_k cmf_k r_k directly.
+    // sum S / 81 gives (S - sum_k cmf_k r_k) / 81 directly.
     const float S = m == 0 ? a.s0 : (m == 1 ? a.s1 : a.s2);
     const size_t last_row = static_cast<size_t>(a.nt - 1) * static_cast<size_t>(a.np);
     const bool vec_out = (a.n & 3) == 0;
@@ -509,11 +513,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         float Fc, opc;
         spec_cell_terms(a, cell < a.n ? cell : a.n - 1, last_row, Fc, opc);
         const bool fast = !__any(!(Fc <= kSpecFCos2 && opc <= kSpecOpFast));
-        // This lane's cell's (P, A, B) of the chosen form and the FP16 parts
-        // of P, read by the lanes of its 16-cell group.
+        // This lane's cell's (P, 81 A, 81 B) of the chosen form and the FP16
+        // parts of P, read by the lanes of its 16-cell group.
         const float Pc = fast ? 2.f * opc : opc;
-        const float Ac = fast ? fmaf(0.5f, Fc, 1.f) : 1.f;
-        const float Bc = fast ? 0.5f * Fc : Fc;
+        const float Ac = fast ? fmaf(40.5f, Fc, 81.f) : 81.f;
+        const float Bc = fast ? 40.5f * Fc : 81.f * Fc;
         const PhaseParts pp = phase_parts(Pc);
         v4f acc[4];
 #pragma unroll
@@ -540,8 +544,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                     // Streaming store: the RGB planes are not re-read here, so
                     // they should not displace the film state from the caches.
                     typedef float f4v __attribute__((ext_vector_type(4)));
-                    __builtin_nontemporal_store(f4v{acc[g][0] * scale, acc[g][1] * scale, acc[g][2] * scale,
-                                                    acc[g][3] * scale},
+                    __builtin_nontemporal_store(f4v{acc[g][0], acc[g][1], acc[g][2], acc[g][3]},
                                                 reinterpret_cast<f4v*>(plane + sg * 64 + 16 * g + 4 * q));
                 }
             } else {
@@ -550,7 +553,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                     const size_t c0 = sg * 64 + 16 * g + 4 * q;
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
-                        if (c0 + v < a.n) plane[c0 + v] = acc[g][v] * scale;
+                        if (c0 + v < a.n) plane[c0 + v] = acc[g][v];
                 }
             }
         }
@@ -808,7 +811,7 @@ struct PackedCmf {
     u32x2 pfrag[kSpecPhase * 64];
     float sum[2][3];          // [format][colour]
     int has_lo[2];
-    bool f16_bound;           // 2^-12 sum_k |cmf[k][c]| / 81 <= kSpecF16Budget for every c
+    bool f16_bound;           // 2^-18 sum_k |cmf[k][c]| <= kSpecF16Budget for every c
     bool f16_range;           // every |cmf| <= 2^14
     int skip;                 // trailing slots left out (kSpecTailBudget)
 };
@@ -846,7 +849,7 @@ static PackedCmf pack_cmf(const float* cmf) {
     }
     out.f16_bound = true;
     for (int c = 0; c < 3; ++c)
-        if (std::ldexp(sabs[c], -12) / kSpecLambdas > kSpecF16Budget) out.f16_bound = false;
+        if (std::ldexp(sabs[c], -18) > kSpecF16Budget) out.f16_bound = false;
     for (int f = 0; f < 2; ++f)
         for (int t = 0; t < kSpecMfma; ++t)
             for (int lane = 0; lane < 64; ++lane) {
@@ -979,9 +982,9 @@ void spectral_launch(Drainage* d, const sf_camera* cam, double n_film, const flo
     a.bfrag = reinterpret_cast<const u32x4*>(d->cmf_dev) + f * kCmfFragsPerFormat;
     a.ulam = d->cmf_dev + kCmfFrags * 4;
     a.pfrag = reinterpret_cast<const u32x2*>(d->cmf_dev + kCmfPhaseOffset);
-    a.s0 = d->cmf_sum[f][0];
-    a.s1 = d->cmf_sum[f][1];
-    a.s2 = d->cmf_sum[f][2];
+    a.s0 = d->cmf_sum[f][0] / kSpecLambdas;
+    a.s1 = d->cmf_sum[f][1] / kSpecLambdas;
+    a.s2 = d->cmf_sum[f][2] / kSpecLambdas;
     a.b_lo = d->cmf_lo[f];
     a.hpole = view ? view->hpole : d->hpole[d->cur_pole];
     a.pole_virtual = view ? (view->pole_virtual ? 1 : 0)
