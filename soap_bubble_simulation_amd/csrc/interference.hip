@@ -273,7 +273,7 @@ constexpr int kSpecSlots = 21;      // wavelength slots per lane and cell
 constexpr int kSpecMfma = 3;        // 16x16x32 MFMAs per group and operand part
 constexpr int kSpecPhase = 5;       // 4x4x4 (16-block) phase MFMAs per group (slots 0..19)
 constexpr float kSpecFCos2 = 16.f;
-// Largest RGB change the FP16 rounding of r may cause for the FP16 operand
+// Largest RGB change the FP16 rounding of r' may cause for the FP16 operand
 // format to be chosen automatically (a quarter of the 1e-3 RGB tolerance).
 constexpr double kSpecF16Budget = 2.5e-4;
 // Slot s holds wavelengths 4 s .. 4 s + 3 (slot 20: wavelength 80, 780 nm,
@@ -372,9 +372,9 @@ __device__ inline void spec_cell_terms(const SpecArgs& a, size_t cc, size_t last
     opc = 2.f * a.n_film * d_nm * cos_t;
 }
 
-// r of one slot from its phase x in revolutions.  FAST: x = opd / lambda,
-// A = 1 + F/2, B = F/2 (cos 2x form); otherwise x = opd / (2 lambda), A = 1,
-// B = F (square form, reduced argument).
+// r' = r / 81 of one slot from its phase x in revolutions.  FAST:
+// x = opd / lambda, A = 81 (1 + F/2), B = 81 F/2 (cos 2x form); otherwise
+// x = opd / (2 lambda), A = 81, B = 81 F (square form, reduced argument).
 template <bool FAST>
 __device__ inline float spec_r_x(float x, float A, float B) {
     if (FAST) return __builtin_amdgcn_rcpf(fmaf(B, __builtin_amdgcn_cosf(x), A));
@@ -479,8 +479,9 @@ __device__ inline void spec_group(const u32x2 (&pa)[kSpecPhase], u32x2 bop, floa
     }
 }
 
-// Five waves per SIMD (96 VGPRs; the compiler moves a few per-batch values
-// to scratch outside the wavelength loop): 1.4 % faster than four.
+// Five waves per SIMD (96 VGPRs; in some instantiations the compiler moves a
+// few per-batch values to scratch outside the wavelength loop): 1.4 % faster
+// than four.
 template <bool F16, int SKIP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void spectral_kernel(const SpecArgs a) {
     const int lane = threadIdx.x & 63;
