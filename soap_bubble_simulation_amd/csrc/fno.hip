@@ -151,6 +151,7 @@ struct FeatArgs {
     const float* hp_lo;
     const BubbleWin* win;
     const float* href;    // per bubble of the chunk
+    const float4* rows;   // per theta row: sin, lap_c, lap_p (feature_rows_kernel)
     int first, nb, nt, np;
     float inv_sx;         // 1 / scale of the relative perturbation
     float sink_scale, diff_scale;
@@ -158,76 +159,89 @@ struct FeatArgs {
     float* in;            // [nb][nt * np][kCin]
 };
 
-__global__ void features_kernel(const FeatArgs a) {
+// Per theta row of the grid: sin(theta_i) and, on interior rows, the weights
+// of the spherical Laplacian in radians (drainage.rs:352-372) scaled by
+// dth^2: lap_c = dth cot(theta_i) / 2 on (south - north) and
+// lap_p = (dth / (sin(theta_i) dph))^2 on (east + west - 2 centre).
+__global__ void feature_rows_kernel(int nt, int np, float4* rows) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const float dth = static_cast<float>(kPi) / static_cast<float>(nt - 1);
+    const float sin_i = sinf(dth * static_cast<float>(i));
+    float lap_c = 0.f, lap_p = 0.f;
+    if (i > 0 && i < nt - 1) {
+        const float st = sin_i, ct = cosf(dth * static_cast<float>(i));
+        const float dph = 2.f * static_cast<float>(kPi) / static_cast<float>(np);
+        const float rp = dth / (st * dph);
+        lap_c = 0.5f * dth * (ct / st);
+        lap_p = rp * rp;
+    }
+    rows[i] = make_float4(sin_i, lap_c, lap_p, 0.f);
+}
+
+// The kCin input features of every cell of a chunk, a block per grid row at
+// a time (per-row terms from the row table, two 16-byte stores per cell).
+__global__ __launch_bounds__(256) void features_kernel(const FeatArgs a) {
+    static_assert(kCin == 8, "a cell's features are two float4");
     const long long n_cells = static_cast<long long>(a.nt) * a.np;
-    const long long total = n_cells * a.nb;
-    const float dth = static_cast<float>(kPi) / static_cast<float>(a.nt - 1);
-    for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < total;
-         q += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const int b = static_cast<int>(q / n_cells);
-        const long long c = q - static_cast<long long>(b) * n_cells;
+    const int nrows = a.nb * a.nt;
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int b = row / a.nt, i = row - b * a.nt;
         const BubbleWin w = a.win[a.first + b];
-        // h / h_ref - 1 from the float-float pair: hi - h_ref is exact
-        // (Sterbenz) for a film within a factor 2 of its mean.
         const float hr = a.href[b];
-        const float r = ((a.hi[w.off + c] - hr) + a.lo[w.off + c]) / hr;
-        const int i = static_cast<int>(c / a.np);
-        float* o = a.in + q * kCin;
-        o[0] = static_cast<float>(r) * a.inv_sx;
-        o[1] = (a.g[w.off + c] - 0.5f) * 10.f;
-        const float sin_i = sinf(dth * static_cast<float>(i));
-        o[2] = sin_i;
-        o[3] = w.s_sink * a.sink_scale;
-        o[4] = w.s_diff * a.diff_scale;
-        // previous window's increment d, scaled as the targets, one
-        // diffusion step of it, dt D lap(d) = (s_diff / K) dth^2 lap~(d), and
-        // one explicit plain step of h itself, dt (D lap(h) - c h^3 sin),
-        // lap~ the spherical Laplacian in radians (drainage.rs:352-372); h
-        // differences are taken on the float-float pairs
+        const float4 rw = a.rows[i];
+        const float sin_i = rw.x;
+        const bool interior = i > 0 && i < a.nt - 1;
+        // stencil weights (zero on the pole rows, whose increments are not
+        // stepped): dt D / (R^2 dth^2), 1, lap_c, lap_p
+        const float lap_w = interior ? w.s_diff * a.inv_window : 0.f;
+        const float lap_s = interior ? 1.f : 0.f, lap_c = rw.y, lap_p = rw.z;
+        const float f3 = w.s_sink * a.sink_scale, f4 = w.s_diff * a.diff_scale;
+        const float iy = 1.f / w.yscale;
         const float* hh = a.hi + w.off;
         const float* hl = a.lo + w.off;
-        const bool interior = i > 0 && i < a.nt - 1;
-        float lap_w = 0.f, lap_s = 0.f, lap_c = 0.f, lap_p = 0.f;   // stencil weights
-        long long kn = c, ks = c, kw = c, ke = c;
-        if (interior) {
-            const int j = static_cast<int>(c - static_cast<long long>(i) * a.np);
-            const long long rowc = static_cast<long long>(i) * a.np;
-            kn = c - a.np;
-            ks = c + a.np;
-            kw = rowc + (j == 0 ? a.np - 1 : j - 1);
-            ke = rowc + (j == a.np - 1 ? 0 : j + 1);
-            const float st = sin_i, ct = cosf(dth * static_cast<float>(i));
-            const float dph = 2.f * static_cast<float>(kPi) / static_cast<float>(a.np);
-            const float rp = dth / (st * dph);
-            lap_c = 0.5f * dth * (ct / st);     // times (south - north)
-            lap_p = rp * rp;                    // times (east + west - 2 centre)
-            lap_s = 1.f;
-            lap_w = w.s_diff * a.inv_window;    // dt D / (R^2 dth^2)
+        const long long rowc = static_cast<long long>(i) * a.np;
+        float4* orow = reinterpret_cast<float4*>(a.in + (static_cast<long long>(b) * n_cells + rowc) * kCin);
+        for (int j = threadIdx.x; j < a.np; j += blockDim.x) {
+            const long long c = rowc + j;
+            // h / h_ref - 1 from the float-float pair: hi - h_ref is exact
+            // (Sterbenz) for a film within a factor 2 of its mean.
+            const float r = ((hh[c] - hr) + hl[c]) / hr;
+            // previous window's increment d, scaled as the targets, one
+            // diffusion step of it, dt D lap(d) = (s_diff / K) dth^2 lap~(d), and
+            // one explicit plain step of h itself, dt (D lap(h) - c h^3 sin),
+            // lap~ the spherical Laplacian in radians (drainage.rs:352-372); h
+            // differences are taken on the float-float pairs
+            long long kn = c, ks = c, kw = c, ke = c;
+            if (interior) {
+                kn = c - a.np;
+                ks = c + a.np;
+                kw = rowc + (j == 0 ? a.np - 1 : j - 1);
+                ke = rowc + (j == a.np - 1 ? 0 : j + 1);
+            }
+            auto lap_of = [&](auto f) {
+                const float fc = f(c), fn = f(kn), fs = f(ks);
+                return lap_s * ((fs - fc) + (fn - fc)) + lap_c * (fs - fn) + lap_p * ((f(ke) - fc) + (f(kw) - fc));
+            };
+            float dp = 0.f, ddp = 0.f;
+            if (a.hp_hi) {
+                const float* ph = a.hp_hi + w.off;
+                const float* pl = a.hp_lo + w.off;
+                auto inc = [&](long long k) { return (hh[k] - ph[k]) + (hl[k] - pl[k]); };
+                dp = inc(c);
+                if (interior) ddp = lap_w * lap_of(inc);
+            }
+            float step1 = 0.f;
+            if (interior) {
+                // h_k - h_c of the pairs: the hi difference is exact for neighbours within a factor 2
+                auto hd = [&](long long k) { return (hh[k] - hh[c]) + (hl[k] - hl[c]); };
+                const float diff = lap_s * (hd(ks) + hd(kn)) + lap_c * (hd(ks) - hd(kn)) + lap_p * (hd(ke) + hd(kw));
+                const float hrel = 1.f + r;
+                step1 = lap_w * diff - w.s_sink * a.inv_window * hrel * hrel * hrel * hr * sin_i;
+            }
+            orow[2 * j] = make_float4(r * a.inv_sx, (a.g[w.off + c] - 0.5f) * 10.f, sin_i, f3);
+            orow[2 * j + 1] = make_float4(f4, dp * iy, ddp * iy, step1 * iy);
         }
-        auto lap_of = [&](auto f) {
-            const float fc = f(c), fn = f(kn), fs = f(ks);
-            return lap_s * ((fs - fc) + (fn - fc)) + lap_c * (fs - fn) + lap_p * ((f(ke) - fc) + (f(kw) - fc));
-        };
-        float dp = 0.f, ddp = 0.f;
-        if (a.hp_hi) {
-            const float* ph = a.hp_hi + w.off;
-            const float* pl = a.hp_lo + w.off;
-            auto inc = [&](long long k) { return (hh[k] - ph[k]) + (hl[k] - pl[k]); };
-            dp = inc(c);
-            if (interior) ddp = lap_w * lap_of(inc);
-        }
-        float step1 = 0.f;
-        if (interior) {
-            // h_k - h_c of the pairs: the hi difference is exact for neighbours within a factor 2
-            auto hd = [&](long long k) { return (hh[k] - hh[c]) + (hl[k] - hl[c]); };
-            const float diff = lap_s * (hd(ks) + hd(kn)) + lap_c * (hd(ks) - hd(kn)) + lap_p * (hd(ke) + hd(kw));
-            const float hrel = 1.f + r;
-            step1 = lap_w * diff - w.s_sink * a.inv_window * hrel * hrel * hrel * hr * sin_i;
-        }
-        const float iy = 1.f / w.yscale;
-        o[5] = dp * iy;
-        o[6] = ddp * iy;
-        o[7] = step1 * iy;
     }
 }
 
@@ -406,6 +420,7 @@ struct Fno {
     float *ht_hi = nullptr, *ht_lo = nullptr, *ha_hi = nullptr, *ha_lo = nullptr, *hb_hi = nullptr,
           *hb_lo = nullptr, *sel_hi = nullptr, *sel_lo = nullptr, *hp_hi = nullptr, *hp_lo = nullptr;
     BubbleWin* d_win = nullptr;
+    float4* feat_rows = nullptr;  // per theta row terms of the input features
     uint8_t* d_take = nullptr;
     int win_cap = 0;
 
@@ -413,7 +428,7 @@ struct Fno {
         if (stream) (void)hipStreamSynchronize(stream);
         void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, Eth16, Ath16, wfold, in, out, target, dout, ybuf, X1, G, Y, Z,
                       dxa, dxb, dz, dXs, part, loss_part, dscratch, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
-                      sel_hi, sel_lo, hp_hi, hp_lo, d_win, d_take, rpart};
+                      sel_hi, sel_lo, hp_hi, hp_lo, d_win, feat_rows, d_take, rpart};
         for (void* p : ps) (void)hipFree(p);
         for (float* p : xs) (void)hipFree(p);
         for (float* p : zs) (void)hipFree(p);
@@ -994,7 +1009,15 @@ struct Fno {
         a.sink_scale = 1e6f;
         a.diff_scale = 1.f;
         a.in = in;
-        hipLaunchKernelGGL(features_kernel, dim3(blocks_for(static_cast<long long>(nb) * N, 256, 16384)), dim3(256),
+        if (!feat_rows) {
+            feat_rows = galloc<float4>(T);
+            hipLaunchKernelGGL(feature_rows_kernel, dim3((T + 255) / 256), dim3(256), 0, stream, T, P, feat_rows);
+            SF_HIP(hipGetLastError());
+        }
+        a.rows = feat_rows;
+        const long long nrows = static_cast<long long>(nb) * T;
+        SF_REQUIRE(nrows < (1LL << 31), SF_ERR_INVALID, "too many grid rows in one chunk");
+        hipLaunchKernelGGL(features_kernel, dim3(static_cast<unsigned>(std::min<long long>(nrows, 16384))), dim3(256),
                            0, stream, a);
         SF_HIP(hipGetLastError());
     }
