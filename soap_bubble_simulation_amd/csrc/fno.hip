@@ -380,6 +380,7 @@ struct Fno {
 
     // constant transforms
     float* Ephi = nullptr;     // [2 Mp][P]
+    float* Esum = nullptr;     // [2 Mp] row sums of Ephi
     float* Eth = nullptr;      // [2 Mt][T]
     float* Ath = nullptr;      // [2 T][2 Mt]
     float* Iphi = nullptr;     // [P][2 Mp]
@@ -426,7 +427,7 @@ struct Fno {
 
     ~Fno() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Eth, Ath, Iphi, Ephi16, Iphi16, Eth16, Ath16, wfold, in, out, target, dout, ybuf, X1, G, Y, Z,
+        void* ps[] = {w, grad, adam_m, adam_v, Ephi, Esum, Eth, Ath, Iphi, Ephi16, Iphi16, Eth16, Ath16, wfold, in, out, target, dout, ybuf, X1, G, Y, Z,
                       dxa, dxb, dz, dXs, part, loss_part, dscratch, ht_hi, ht_lo, ha_hi, ha_lo, hb_hi, hb_lo,
                       sel_hi, sel_lo, hp_hi, hp_lo, d_win, feat_rows, d_take, rpart};
         for (void* p : ps) (void)hipFree(p);
@@ -512,6 +513,16 @@ struct Fno {
                 ath[(2 * static_cast<size_t>(t) + 1) * 2 * Mt + Mt + kk] = c;
             }
         }
+        // Esum[j] = sum_p Ephi[j][p], the phi DFT of a constant input (FP64 sum
+        // of the FP32 table, rounded)
+        std::vector<float> esum(2 * Mp);
+        for (int j = 0; j < 2 * Mp; ++j) {
+            double t = 0.0;
+            for (int p = 0; p < P; ++p) t += ephi[static_cast<size_t>(j) * P + p];
+            esum[j] = static_cast<float>(t);
+        }
+        Esum = galloc<float>(esum.size());
+        SF_HIP(hipMemcpy(Esum, esum.data(), esum.size() * 4, hipMemcpyHostToDevice));
         Ephi = galloc<float>(ephi.size());
         Eth = galloc<float>(eth.size());
         Ath = galloc<float>(ath.size());
@@ -737,7 +748,7 @@ struct Fno {
             if (P % 16 == 0 && J == kLiftWaveJ && 64 % C == 0)
                 hipLaunchKernelGGL(lift_dft_wave_kernel<kCin>,
                                    dim3(static_cast<unsigned>((nrows + 4 * kLiftRows - 1) / (4 * kLiftRows))), dim3(256),
-                                   0, stream, in, Ephi, w + o_wl, w + o_bl, X1, P, C, nrows);
+                                   0, stream, in, Ephi, Esum, w + o_wl, w + o_bl, X1, P, C, nrows);
             else
                 hipLaunchKernelGGL(lift_dft_kernel<kCin>, dim3(static_cast<unsigned>(B * T)), dim3(256), lds, stream, in,
                                    Ephi, w + o_wl, w + o_bl, X1, P, J, C);

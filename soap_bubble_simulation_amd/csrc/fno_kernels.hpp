@@ -705,20 +705,22 @@ __global__ __launch_bounds__(256) void lift_dft_kernel(const float* __restrict__
 // The same product on FP32 MFMA (v_mfma_f32_16x16x4_f32, lane l supplies
 // A[l % 16][l / 16] and B[l / 16][l % 16]), one wave per kLiftRows rows of a
 // bubble's grid, so each 16-byte load of the Ephi rows (A) serves kLiftRows
-// products: S = Ephi [in | 1] (J x (NIN + 1), K = P split so that lane group
-// g sums cells g P/4 .. (g + 1) P/4 - 1), then X1 = S [W_l ; b_l] from a
-// per-wave LDS copy of S.  Needs P % 16 == 0, J == 32, NIN < 16 and
-// 64 % C == 0.
+// products: S_r = Ephi in_r (J x NIN, K = P split so that lane group g sums
+// cells g P/4 .. (g + 1) P/4 - 1), two rows per 16-wide tile (column
+// n = 8 (r % 2) + f), then X1 = S [W_l] + Esum b_l from a per-wave LDS copy of
+// S, Esum[j] = sum_p Ephi[j][p] (the DFT of the constant input, host-summed).
+// Needs P % 16 == 0, J == 32, NIN == 8 and 64 % C == 0.
 constexpr int kLiftWaveJ = 32;
 constexpr int kLiftRows = 4;
 
 template <int NIN>
 __global__ __launch_bounds__(256) void lift_dft_wave_kernel(const float* __restrict__ in, const float* __restrict__ Ephi,
+                                                            const float* __restrict__ Esum,
                                                             const float* __restrict__ Wl, const float* __restrict__ bl,
                                                             float* __restrict__ X1, int P, int C, long long rows) {
-    static_assert(NIN < 16, "the inputs and the constant column fill one 16-wide tile");
-    constexpr int SW = 17;                                // padded row of S in LDS
-    constexpr int J = kLiftWaveJ, JT = J / 16;
+    static_assert(NIN == 8, "two rows' inputs fill one 16-wide tile");
+    constexpr int SW = NIN + 1;                           // padded row of S in LDS
+    constexpr int J = kLiftWaveJ, JT = J / 16, RT = kLiftRows / 2;
     __shared__ float s_S[4][kLiftRows][J * SW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int mr = lane & 15, g = lane >> 4;
@@ -726,56 +728,61 @@ __global__ __launch_bounds__(256) void lift_dft_wave_kernel(const float* __restr
     if (row0 >= rows) return;                             // no barrier below: waves are independent
     const int nr = static_cast<int>(rows - row0 < kLiftRows ? rows - row0 : kLiftRows);
     const int P4 = P >> 2;
-    f32x4 acc[kLiftRows][JT];
+    f32x4 acc[RT][JT];
 #pragma unroll
-    for (int r = 0; r < kLiftRows; ++r)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int jt = 0; jt < JT; ++jt) acc[r][jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int jt = 0; jt < JT; ++jt) acc[rt][jt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const float* e0 = Ephi + static_cast<long long>(mr) * P + g * P4;
+    const int rl = mr >> 3, f = mr & 7;                   // this lane's row of the pair and feature
     for (int kk = 0; kk < P4; kk += 4) {
         float4 ev[JT];
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) ev[jt] = *reinterpret_cast<const float4*>(e0 + static_cast<long long>(16 * jt) * P + kk);
 #pragma unroll
-        for (int r = 0; r < kLiftRows; ++r) {
-            if (r < nr) {
-                const float* inr = in + (row0 + r) * P * NIN + static_cast<long long>(g * P4 + kk) * NIN + mr;
-                float bv[4];
+        for (int rt = 0; rt < RT; ++rt) {
+            if (2 * rt < nr) {
+                const int r = 2 * rt + rl;
+                float bv[4] = {0.f, 0.f, 0.f, 0.f};
+                if (r < nr) {
+                    const float* inr = in + (row0 + r) * P * NIN + static_cast<long long>(g * P4 + kk) * NIN + f;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) bv[u] = mr < NIN ? inr[u * NIN] : (mr == NIN ? 1.f : 0.f);
+                    for (int u = 0; u < 4; ++u) bv[u] = inr[u * NIN];
+                }
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt) {
-                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].x, bv[0], acc[r][jt], 0, 0, 0);
-                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].y, bv[1], acc[r][jt], 0, 0, 0);
-                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].z, bv[2], acc[r][jt], 0, 0, 0);
-                    acc[r][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].w, bv[3], acc[r][jt], 0, 0, 0);
+                    acc[rt][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].x, bv[0], acc[rt][jt], 0, 0, 0);
+                    acc[rt][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].y, bv[1], acc[rt][jt], 0, 0, 0);
+                    acc[rt][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].z, bv[2], acc[rt][jt], 0, 0, 0);
+                    acc[rt][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[jt].w, bv[3], acc[rt][jt], 0, 0, 0);
                 }
             }
         }
     }
-    // lane (mr, g) holds S_r[16 jt + 4 g + v][mr]
+    // lane (mr, g) holds S_{2 rt + mr / 8}[16 jt + 4 g + v][mr % 8]
 #pragma unroll
-    for (int r = 0; r < kLiftRows; ++r)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) s_S[wave][r][(16 * jt + 4 * g + v) * SW + mr] = acc[r][jt][v];
+            for (int v = 0; v < 4; ++v) s_S[wave][2 * rt + rl][(16 * jt + 4 * g + v) * SW + f] = acc[rt][jt][v];
     __builtin_amdgcn_wave_barrier();
-    // X1[j][c] = S[j][NIN] b_l[c] + sum_f S[j][f] W_l[f][c]; c = lane % C is
+    // X1[j][c] = Esum[j] b_l[c] + sum_f S[j][f] W_l[f][c]; c = lane % C is
     // fixed per lane (64 % C == 0), its W_l column held in registers
     const int c = lane % C;
     float wl[NIN];
 #pragma unroll
-    for (int f = 0; f < NIN; ++f) wl[f] = Wl[f * C + c];
+    for (int q = 0; q < NIN; ++q) wl[q] = Wl[q * C + c];
     const float b = bl[c];
     for (int r = 0; r < nr; ++r) {
         const float* S = s_S[wave][r];
         float* xr = X1 + (row0 + r) * J * C;
         for (int t = lane; t < J * C; t += 64) {
-            const float* sj = S + (t / C) * SW;
-            float sum = sj[NIN] * b;
+            const int j = t / C;
+            const float* sj = S + j * SW;
+            float sum = Esum[j] * b;
 #pragma unroll
-            for (int f = 0; f < NIN; ++f) sum = fmaf(sj[f], wl[f], sum);
+            for (int q = 0; q < NIN; ++q) sum = fmaf(sj[q], wl[q], sum);
             xr[t] = sum;
         }
     }
