@@ -268,7 +268,9 @@ typedef struct sf_camera {
  * Operand format (sf_spectral_set_operand): by default FP16 when its
  * rounding of the reflectance moves no channel by more than 2.5e-4
  * (2^-18 sum_k |cmf[k][c]|) and the curves fit FP16, else a BF16 hi/lo
- * pair that keeps about 16 bits.
+ * pair that keeps about 16 bits.  The longest wavelengths (780 nm, then
+ * 760-775 nm) are left out while their sum_k |cmf[k][c]| / 81 stays within
+ * 1e-5 for every c, which bounds the change of each channel by 1e-5.
  * Reads the context's current h; writes 3 planes of n_theta*n_phi floats
  * to host memory (rgb_host) and/or keeps them on the device. */
 int sf_spectral_rgb(sf_drainage* ctx, const sf_camera* cam, double refractive_index,
