@@ -375,3 +375,30 @@ def test_judged_foam_at_benchmark_shape():
     fm.close()
     ref.close()
     m.close()
+
+
+@pytest.mark.gpu
+def test_judged_foam_full_benchmark_size():
+    """The full surrogate benchmark foam (1024 bubbles of 256 x 512, 1.3e8
+    cells, sixteen 64-bubble network chunks): with a tolerance no prediction
+    meets, every bubble-window falls back and the foam ends bit-identical to
+    the plain solver; with the identity predictor every bubble-window is
+    accounted as accepted or fallen back."""
+    from soap_bubble_simulation_amd import _lib as L
+    from soap_bubble_simulation_amd.surrogate import FnoSurrogate
+    cfg = dict(C4_CFG, max_batch=64)
+    m = FnoSurrogate(**cfg)
+    m.set_scales(0.05, 1e-3)
+    grid = dict(n_theta=256, n_phi=512)
+    fm, ref = small_foam(1024, 78, **grid), small_foam(1024, 78, **grid)
+    assert fm.info()["cells"] >= 1024 * 256 * 512
+    st = m.advance_foam(fm, 2, tol=1e-30, predictor=L.PREDICT_NETWORK)
+    ref.step(2 * cfg["window"])
+    assert st.windows == 2 and st.bubble_windows == 2048
+    assert st.fallbacks == 2048 and st.accepted == 0
+    assert np.array_equal(fm.fields()[0], ref.fields()[0])
+    st = m.advance_foam(fm, 1, tol=1e-3, predictor=L.PREDICT_IDENTITY)
+    assert st.accepted + st.fallbacks == st.bubble_windows == 1024
+    fm.close()
+    ref.close()
+    m.close()
