@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1164,7 +1165,6 @@ struct Fno {
                 SF_HIP(hipMemcpyAsync(d_win, tmp.data(), fi.nb * sizeof(BubbleWin), hipMemcpyHostToDevice, stream));
                 SF_HIP(hipStreamSynchronize(stream));
             }
-            copy_state(ht_hi, ht_lo, hp_hi, hp_lo, fi.total);
             have_prev = true;
             SF_HIP(hipStreamSynchronize(stream));
             const double ms_net = ms_since(t0);
@@ -1222,6 +1222,10 @@ struct Fno {
                 st->judge_ms += ms_judge;
                 st->fallback_ms += fallback_ms;
             }
+            // h(t) becomes the next window's previous state: the buffers trade
+            // places (the next window's h(t) overwrites the other one)
+            std::swap(ht_hi, hp_hi);
+            std::swap(ht_lo, hp_lo);
         }
     }
 };
